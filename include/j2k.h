@@ -1,0 +1,186 @@
+/* libj2k: MI355X (gfx950) HIP implementation of the jacobi2000 engine forward
+ * pass behind a plain C ABI.
+ *
+ * Conventions (one rule for every entry point):
+ *  - Every function returns an int status (J2K_OK = 0). On failure
+ *    j2k_last_error() returns a message for the calling thread.
+ *  - Handles are opaque. A j2k_model owns its device, its weights in HBM, one
+ *    HIP stream and its scratch; calls on one handle must not overlap (the
+ *    reference GpuModel serialises on one wgpu queue the same way,
+ *    crates/engine/src/gpu.rs:374-437).
+ *  - Host buffers are caller-owned and only read or written during the call.
+ *    `_dev` entry points take device pointers and a hipStream_t (as void*,
+ *    NULL = the model's own stream) and return without synchronising.
+ *  - Grid layout is the reference's: channel planes c*ny*nx + y*nx + x with
+ *    channel 0 = h, 1 = u, 2 = v (crates/engine/src/cpu.rs:7-10). A space-time
+ *    grid adds the time slice: c*T*ny*nx + t*ny*nx + y*nx + x.
+ */
+#ifndef J2K_H
+#define J2K_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. 1-5 mirror lean_runtime::Error (crates/lean-runtime/src/error.rs:6-13);
+ * 6-9 name the conditions on which the reference loader panics
+ * (config.rs:71 missing field, weights.rs:39 non-F32, weights.rs:53 missing
+ * tensor, shape mismatches). */
+#define J2K_OK 0
+#define J2K_ERR_NO_ADAPTER 1     /* no HIP device visible                         */
+#define J2K_ERR_NO_DEVICE 2      /* device index invalid or context failed        */
+#define J2K_ERR_LIMIT 3          /* a device limit the model needs is not met     */
+#define J2K_ERR_MAP 4            /* a host<->device copy failed                   */
+#define J2K_ERR_POLL 5           /* a kernel launch or synchronisation failed     */
+#define J2K_ERR_BAD_CONFIG 6     /* config.json invalid or missing a field        */
+#define J2K_ERR_MISSING_TENSOR 7 /* a tensor the config needs is absent           */
+#define J2K_ERR_BAD_DTYPE 8      /* a tensor is not F32                           */
+#define J2K_ERR_SHAPE 9          /* a tensor or argument has the wrong shape      */
+#define J2K_ERR_INVALID 10       /* bad argument (NULL, out of range, wrong form) */
+#define J2K_ERR_OOM 11           /* device memory exhausted / over budget         */
+#define J2K_PENDING 12           /* server: result not ready yet                  */
+
+/* Arithmetic of the transformer trunk. LayerNorm statistics, softmax, the
+ * residual stream, the fixed-point state and the projections stay f32 (sums
+ * in f64) in both. */
+#define J2K_PREC_F32 0  /* plain form: f32 operands, f32 MFMA, f32 accumulate   */
+#define J2K_PREC_BF16 1 /* served form: bf16 weights/activations, f32 accumulate */
+
+#define J2K_FORM_PATCH2D 0 /* the reference forward (crates/engine/src/cpu.rs:249-352) */
+#define J2K_FORM_CELL_ST 1 /* per-cell space-time tokens, fixed-point iteration     */
+
+#define J2K_MASK_FULL 0
+#define J2K_MASK_STENCIL 1
+
+/* Config as parsed from config.json (crates/engine/src/config.rs:44-100),
+ * plus the cell_st keys. head_dim and ff_inner are inferred from tensor
+ * shapes as config.rs:72-73 does. */
+typedef struct {
+    int form, precision;
+    int nx, ny, patch, channels, dim, layers, heads, head_dim, ff_inner, meta_dim;
+    int tokens_x, tokens_y, tokens, patch_dim;
+    int mask_kind, radius, dilation; /* patch2d: token stencil (mask.rs:18-30)      */
+    int time_radius;                 /* cell_st: space radius = radius, periodic    */
+    float norm_mean[3], norm_std[3], norm_depth;
+    float dx, dt, g, depth, nu, energy_r_max;
+    int max_iter;
+    float tol;
+} j2k_config;
+
+/* Thread-local message of the last failure on this thread ("" if none). */
+const char* j2k_last_error(void);
+
+/* Number of visible HIP devices (0 with J2K_ERR_NO_ADAPTER when none). */
+int j2k_device_count(int* n);
+
+/* Model::from_bytes (crates/engine/src/lib.rs:45-50) + GpuModel::new
+ * (crates/engine/src/gpu.rs:212-218): parse config.json, read the F32
+ * safetensors bytes, upload every weight to `device`. `precision` is
+ * J2K_PREC_F32 or J2K_PREC_BF16. */
+int j2k_model_create(const uint8_t* safetensors, size_t n_bytes, const char* config_json, int device, int precision,
+                     void** out_model);
+/* GpuModel's Drop (gpu.rs:199-209): frees every device buffer of the model. */
+int j2k_model_destroy(void* model);
+/* GpuModel::cfg (gpu.rs:427-429). */
+int j2k_model_config(const void* model, j2k_config* out);
+
+/* ---- patch2d form (the reference forward) ----------------------------- */
+
+/* Model::forward / GpuModel::forward (lib.rs:53-55, gpu.rs:374-425):
+ * state [channels, ny, nx] normalized, meta [meta_dim] -> increment, same shape. */
+int j2k_forward(void* model, const float* state, const float* meta, float* incr);
+/* Model::step / GpuModel::step (lib.rs:58-60, cpu.rs:84-88, gpu.rs:431-437): state + forward. */
+int j2k_step(void* model, const float* state, const float* meta, float* next);
+/* Batched forward on device: `batch` grids [batch, channels, ny, nx], one meta
+ * vector for all; out = forward(state) (+ state when add_state). No reference
+ * counterpart (the reference runs batch 1, lib.rs:4); it is the batched form of
+ * j2k_forward that rollout windows and serving use. */
+int j2k_forward_batch_dev(void* model, int batch, const float* d_state, const float* d_meta, float* d_out,
+                          int add_state, void* stream);
+/* The rollout loop of crates/engine-cli/src/main.rs:117-121 and the projected
+ * loop of eval.rs:144-156, kept on the device: `steps` times
+ *   x <- step(x); if proj_mass: project_mass(x, mass(x_0));
+ *   if proj_energy: project_energy(x, energy(x_prev)) (mass first, eval.rs order)
+ * with the projection sums in f64 on the device. energy_trace (may be NULL)
+ * receives steps+1 energies (f64, physical units); max_violation (may be NULL)
+ * receives max_t max(0, (E_t - r_max E_{t-1}) / E_{t-1}) (eval.rs:156). */
+int j2k_rollout(void* model, float* state_io, const float* meta, int steps, int proj_mass, int proj_energy,
+                double* energy_trace, double* max_violation);
+/* The same loop on device buffers; energy_trace is a device buffer of steps+1 doubles or NULL. */
+int j2k_rollout_dev(void* model, float* d_state_io, const float* d_meta, int steps, int proj_mass, int proj_energy,
+                    double* d_energy_trace, void* stream);
+/* Jacobi-over-time window: window_io holds T grids [T, channels, ny, nx];
+ * slice 0 is the initial state. Each iteration replaces slices 1..T-1 by
+ * project(step(slice t-1 of the previous iterate)) in one batched forward;
+ * after k iterations slices 0..k equal a k-step rollout exactly (the fixed
+ * point of the rollout, main.rs:117-121, reached after T-1 iterations). */
+int j2k_iterate_window(void* model, float* window_io, const float* meta, int T, int iters, int proj_mass,
+                       int proj_energy);
+
+/* project::mass / project_mass / energy / project_energy (project.rs:9-54) on
+ * one host grid [3, n], sums in f64 on the device. */
+int j2k_mass(void* model, const float* x, int n, double* out);
+int j2k_energy(void* model, const float* x, int n, double* out);
+int j2k_project_mass(void* model, float* x, int n, float target);
+int j2k_project_energy(void* model, float* x, int n, float prev_energy);
+
+/* ---- cell_st form (per-cell space-time fixed point) ------------------- */
+
+/* Blank fill used before iterating (host only): a blank cell takes the same
+ * (y, x) cell of the nearest fully known slice (ties to the earlier slice),
+ * else the mean of its slice's known cells in that channel, else the mean of
+ * all known cells, else 0. grid [3,T,ny,nx], known [T,ny,nx] (0/1). */
+int j2k_fill_blank(const float* grid, const uint8_t* known, int T, int ny, int nx, float* out);
+
+/* Runs the fixed point x_{k+1} = P(clamp_known(x_0 + f(x_k))) from x_0 =
+ * grid_io (blank cells already filled) until max|x_{k+1} - x_k| over blank
+ * cells < tol or max_iter iterations (tol < 0 / max_iter <= 0: config values).
+ * P projects every slice on its blank cells: mass to the mass of x_0's slice,
+ * then energy to r_max times x_0's slice energy, sums in f64 on the device.
+ * Writes the final iterate to grid_io; iters, resid, violation may be NULL. */
+int j2k_iterate(void* model, float* grid_io, const uint8_t* known, const float* meta, int T, int ny, int nx,
+                int max_iter, float tol, int* iters, float* resid, double* violation);
+/* Device-pointer form; synchronises only for the per-iteration convergence
+ * test. d_grid [3,T,ny,nx], d_known [T,ny,nx] u8, d_meta [meta_dim]. */
+int j2k_iterate_dev(void* model, float* d_grid, const uint8_t* d_known, const float* d_meta, int T, int ny, int nx,
+                    int max_iter, float tol, int* iters, float* resid, double* violation, void* stream);
+/* One model evaluation y = f(x) of the cell_st form: y [T*ny*nx, 3]. */
+int j2k_cell_forward_dev(void* model, const float* d_grid, const uint8_t* d_known, const float* d_meta, int T, int ny,
+                         int nx, float* d_y, void* stream);
+
+/* ---- serving (continuous batching on one GPU) -------------------------- */
+
+typedef struct {
+    uint64_t id;
+    int iters;          /* fixed-point iterations run                            */
+    int converged;      /* 1 if max|dx| < tol, 0 if it hit max_iter              */
+    float resid;        /* last max|dx|                                          */
+    double violation;   /* max relative energy violation over slices             */
+    double submit_s, start_s, done_s; /* monotonic seconds: submit, admit, retire */
+} j2k_result;
+
+/* A persistent request loop on the model's device. Requests are admitted
+ * while their device state fits `hbm_budget_bytes` and at most `max_active`
+ * are on the GPU; all active requests' tokens are packed into one ragged
+ * launch per layer; a request retires on its own convergence. */
+int j2k_server_create(void* model, size_t hbm_budget_bytes, int max_active, void** out_server);
+/* Queues a request (copies its inputs). grid [3,T,ny,nx] with blank cells
+ * filled or not (the server fills them), known [T,ny,nx], meta [meta_dim]. */
+int j2k_server_submit(void* server, const float* grid, const uint8_t* known, const float* meta, int T, int ny, int nx,
+                      uint64_t* out_id);
+/* Non-blocking: J2K_OK and the result (and the grid into grid_out if not
+ * NULL) once retired, J2K_PENDING before. */
+int j2k_server_poll(void* server, uint64_t id, j2k_result* out, float* grid_out);
+/* Blocks until every submitted request has retired. */
+int j2k_server_drain(void* server);
+/* Counters: iterations launched, graph replays, peak active requests, peak bytes. */
+int j2k_server_stats(void* server, uint64_t* launches, uint64_t* graph_replays, int* peak_active, size_t* peak_bytes);
+int j2k_server_destroy(void* server);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

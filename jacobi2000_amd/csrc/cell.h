@@ -1,0 +1,47 @@
+// The cell_st fixed-point iteration over a packed set of requests.
+#pragma once
+
+#include <vector>
+
+#include "model.h"
+
+namespace j2k {
+
+// Host description of one request's grid.
+struct CellDims {
+    int T, ny, nx;
+    int tokens() const { return T * ny * nx; }
+};
+
+// Device tables of a packed set of requests (request r owns tokens
+// [segs[r].base, segs[r].base + T*ny*nx) of every packed array).
+struct CellPack {
+    int M = 0, n_req = 0, n_slices = 0, n_tiles = 0;
+    Seg* segs = nullptr;
+    Tile* tiles = nullptr;
+    Slice* slices = nullptr;
+    int* tok_req = nullptr;  // per token request index (nullptr for one request)
+    float* mvec = nullptr;   // [n_req, dim] meta embedding per request
+    double* mass_t = nullptr;
+    double* eref_t = nullptr;
+    unsigned* resid = nullptr;  // [n_req] float bits of max |dx|
+};
+
+// Host-side tables for a list of requests (packed in order).
+struct CellTables {
+    std::vector<Seg> segs;
+    std::vector<Tile> tiles;
+    std::vector<Slice> slices;
+    std::vector<int> tok_req;
+    int M = 0;
+    void build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req);
+};
+
+void check_cell_dims(const Model& m, const CellDims& d);
+
+// One iteration: xn = P(clamp_known(x0 + f(xc))), resid[r] = max|xn - xc| over blank cells.
+void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn);
+// y = f(x) only (no clamp, no projection): y [M, 3].
+void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y);
+
+}  // namespace j2k

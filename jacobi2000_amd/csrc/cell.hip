@@ -1,0 +1,62 @@
+// The cell_st fixed-point iteration over a packed set of requests.
+#include "cell.h"
+
+namespace j2k {
+
+void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req) {
+    segs.clear();
+    slices.clear();
+    tok_req.clear();
+    M = 0;
+    for (size_t r = 0; r < reqs.size(); r++) {
+        const CellDims& d = reqs[r];
+        segs.push_back(Seg{M, d.T, d.ny, d.nx});
+        const int n = d.ny * d.nx;
+        for (int t = 0; t < d.T; t++) slices.push_back(Slice{M + t * n, n, (int)r});
+        if (per_token_req) tok_req.insert(tok_req.end(), (size_t)d.tokens(), (int)r);
+        M += d.tokens();
+    }
+    build_tiles(segs, tpb, tiles);
+}
+
+void check_cell_dims(const Model& m, const CellDims& d) {
+    if (m.cfg.form != J2K_FORM_CELL_ST) fail(J2K_ERR_INVALID, "this entry point needs a cell_st model");
+    if (d.T <= 0 || d.ny <= 0 || d.nx <= 0) fail(J2K_ERR_SHAPE, "grid dimensions must be positive");
+    const int w = 2 * m.cfg.radius + 1;
+    if (d.ny < w || d.nx < w)
+        fail(J2K_ERR_SHAPE, "grid " + std::to_string(d.ny) + "x" + std::to_string(d.nx) +
+                                " is smaller than the periodic stencil width " + std::to_string(w));
+    if ((long)d.T * d.ny * d.nx > (1L << 28)) fail(J2K_ERR_SHAPE, "grid too large (over 2^28 cells)");
+}
+
+static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X) {
+    const int d = m.cfg.dim;
+    const LayerW& L0 = m.L.at(0);
+    if (m.bf16()) {
+        uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)p.M * d);
+        cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, nullptr, Hb);
+    } else {
+        float* H = m.get<float>(S_H, (size_t)p.M * d);
+        cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, H, nullptr);
+    }
+    trunk(m, s, p.M, X, true, p.segs, p.tiles, p.n_tiles);
+}
+
+void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn) {
+    const j2k_config& c = m.cfg;
+    float* X = m.get<float>(S_X, (size_t)p.M * c.dim);
+    embed_and_trunk(m, s, p, xc, X);
+    cell_readout(s, p.M, c.dim, X, m.norm_g, m.norm_b, m.ro_w, m.ro_b, x0, xn, nullptr);
+    fill_u32(s, p.resid, p.n_req, 0u);
+    cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
+                 c.norm_depth, c.energy_r_max);
+}
+
+void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y) {
+    const j2k_config& c = m.cfg;
+    float* X = m.get<float>(S_X, (size_t)p.M * c.dim);
+    embed_and_trunk(m, s, p, x4, X);
+    cell_readout(s, p.M, c.dim, X, m.norm_g, m.norm_b, m.ro_w, m.ro_b, nullptr, nullptr, y);
+}
+
+}  // namespace j2k

@@ -1,0 +1,126 @@
+// Launchers for libj2k's HIP kernels. Every launcher enqueues on `s` and
+// checks the launch; none synchronises.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace j2k {
+
+enum Epi : int {
+    EPI_STORE = 0,  // C = acc + bias
+    EPI_GELU = 1,   // C = gelu(acc + bias)
+    EPI_RESID = 2,  // C += acc + bias            (residual stream update)
+};
+
+// ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
+// C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.
+void gemm_f32(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, const float* bias, float* C,
+              int ldc, int epi);
+// Form P embed: X[b*tokens + t] = patches @ We + be + pos[t] + mvec.
+void gemm_embed_p(hipStream_t s, int M, int N, int K, const float* A, const float* B, const float* bias,
+                  const float* pos, int tokens, const float* mvec, float* C);
+// Zero-padded cross-correlation (cpu.rs:171-205) as an implicit GEMM:
+// out[b, co, y, x] = bias[co] + sum w[co, ci, ky, kx] in[b, ci, y+ky-k/2, x+kx-k/2],
+// in [B, cin, h, w] (channel stride `in_bstride` floats per batch item),
+// optional GELU; with add_src, out = add_src + gelu?(conv).
+// `in2`/`cin1`: channels >= cin1 come from in2 ([B, cin-cin1, h, w]) so a
+// concatenated input (cpu.rs:340-342) needs no copy.
+void conv_f32(hipStream_t s, int B, int cin, int h, int w, const float* in, const float* in2, int cin1,
+              const float* wt, const float* bias, int cout, int k, float* out, int gelu, const float* add_src,
+              const float* add_out);
+
+// ---- GEMM, bf16 operands on v_mfma_f32_16x16x32_bf16, f32 accumulate ------
+// A [M,K] bf16 row-major, Bt [N,K] bf16 (the weight transposed), f32 bias.
+// EPI_STORE/EPI_GELU write bf16 C (ldc); EPI_RESID adds into f32 Cf.
+void gemm_bf16(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
+               uint16_t* C, float* Cf, int ldc, int epi);
+
+// ---- LayerNorm (cpu.rs:49-65): one wave per row; f32 and/or bf16 out. ----
+void layernorm(hipStream_t s, int rows, int dim, const float* x, const float* g, const float* b, float* out,
+               uint16_t* out_bf);
+
+// ---- Stencil gather attention -------------------------------------------
+// Token geometry of one segment (a batch item or a request): tokens
+// [base, base + T*ny*nx), index t*ny*nx + y*nx + x.
+struct Seg {
+    int base, T, ny, nx;
+};
+enum AttnKind : int {
+    ATTN_PATCH_STENCIL = 0,  // mask.rs:18-30 on the token grid, clipped, dilated
+    ATTN_FULL = 1,           // every token of the segment
+    ATTN_CELL_ST = 2,        // (2r+1)^2 periodic in space x (2rt+1) clipped in time
+};
+struct AttnGeo {
+    int kind, radius, dilation, time_radius;
+};
+// A block covers `tokens_per_block` consecutive tokens of one segment: tiles[i] = {seg, first local token}.
+struct Tile {
+    int seg, start;
+};
+// qkv [M, 3*dim] (q | k | v per row, heads contiguous within each), out [M, dim].
+void attention_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, AttnGeo geo, const Seg* segs,
+                   const Tile* tiles, int n_tiles, int tokens_per_block);
+void attention_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, AttnGeo geo,
+                    const Seg* segs, const Tile* tiles, int n_tiles, int tokens_per_block);
+int attn_tokens_per_block(int heads);
+
+// ---- patch2d helpers -----------------------------------------------------
+// cpu.rs:253-269: patches[b*tokens + t, c*p*p + py*p + px].
+void patchify(hipStream_t s, int B, int C, int ny, int nx, int p, const float* state, float* patches);
+// cpu.rs:276-277: mvec = meta @ Wm + bm (one row).
+void meta_linear(hipStream_t s, int meta_dim, int dim, const float* meta, const float* w, const float* b, float* out);
+// cpu.rs:319-324: xn [B, tokens, dim] -> t0 [B, dim, tokens].
+void transpose_td(hipStream_t s, int B, int tokens, int dim, const float* in, float* out);
+// cpu.rs:211-245, both axes: in [B, c, h, w] -> out [B, c, h*p, w*p] (out batch stride given).
+void upsample2(hipStream_t s, int B, int c, int h, int w, int p, const float* in, float* out, long out_bstride);
+
+// ---- projections ---------------------------------------------------------
+// Per grid b of a batch [B, 3, n]: project_mass to target (mass_t[b] or,
+// when mass_t is null, mass of `ref`), project_energy against prev energy
+// (eprev[b]), then store the energy after projection in eout[b] (f64),
+// violation terms into viol[b]. Sums in f64. (project.rs:9-54, eval.rs:144-156)
+void project_p(hipStream_t s, int B, int n, float* x, const double* mass_t, int do_mass, const double* eprev,
+               int do_energy, double* eout, float g, const float* mean, const float* stdv, float depth,
+               float r_max);
+// f64 sums of mass (mean of channel 0) and energy per grid b.
+void grid_sums(hipStream_t s, int B, int n, const float* x, double* mass, double* energy, float g, const float* mean,
+               const float* stdv, float depth);
+// x[i] += y[i]
+void add_inplace(hipStream_t s, long n, float* x, const float* y);
+
+// ---- cell_st -----------------------------------------------------------
+// Per slice: {first token, cells, request}.
+struct Slice {
+    int start, n, req;
+};
+// [3,T,ny,nx] planes <-> token-major float4 (h, u, v, known).
+void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4);
+void cell_unpack(hipStream_t s, int M, const float* x4, float* grid);
+// X = embed([h,u,v,known]) + mvec (mvec per request), H = LN1_0(X).
+void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* we, const float* be,
+                   const float* mvec, const int* tok_req, const float* g, const float* b, float* X, float* H,
+                   uint16_t* Hb);
+// y = LN(X) @ Wr + br; xn = known ? x0 : x0 + y (either output may be null).
+void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g, const float* b, const float* wr,
+                  const float* br, const float* x0, float* xn, float* y);
+// Per-slice targets from x0: mass (mean normalized h) and energy (f64).
+void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
+                  float g, const float* mean, const float* stdv, float depth);
+// Projection of every slice on its blank cells, then max|xn - xc| over blank
+// cells into resid_bits[req] (float bits, atomic max; caller zeroes it).
+void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,
+                  const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
+                  float depth, float r_max);
+// Per-slice relative energy violation, max into viol_bits[req] (double bits, atomic max).
+void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x4, const double* eref_t,
+                    unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,
+                    float r_max);
+
+}  // namespace j2k
+
+namespace j2k {
+// p[i] = v for i < n (a kernel, so it can sit inside a captured graph).
+void fill_u32(hipStream_t s, unsigned* p, int n, unsigned v);
+}  // namespace j2k

@@ -1,0 +1,310 @@
+// Checkpoint loading for libj2k (host only).
+#include "loader.h"
+
+#include <cmath>
+#include <cstring>
+
+namespace j2k {
+
+namespace {
+
+struct Parser {
+    const std::string& s;
+    size_t i = 0;
+
+    [[noreturn]] void bad(const char* what) {
+        fail(J2K_ERR_BAD_CONFIG, std::string("invalid JSON: ") + what + " at byte " + std::to_string(i));
+    }
+    void ws() {
+        while (i < s.size() && (s[i] == ' ' || s[i] == '\n' || s[i] == '\r' || s[i] == '\t')) i++;
+    }
+    bool lit(const char* w) {
+        size_t n = std::strlen(w);
+        if (s.compare(i, n, w) == 0) {
+            i += n;
+            return true;
+        }
+        return false;
+    }
+    std::string string() {
+        if (s[i] != '"') bad("expected string");
+        i++;
+        std::string out;
+        while (i < s.size() && s[i] != '"') {
+            char c = s[i++];
+            if (c != '\\') {
+                out.push_back(c);
+                continue;
+            }
+            if (i >= s.size()) bad("truncated escape");
+            char e = s[i++];
+            switch (e) {
+                case '"': out.push_back('"'); break;
+                case '\\': out.push_back('\\'); break;
+                case '/': out.push_back('/'); break;
+                case 'b': out.push_back('\b'); break;
+                case 'f': out.push_back('\f'); break;
+                case 'n': out.push_back('\n'); break;
+                case 'r': out.push_back('\r'); break;
+                case 't': out.push_back('\t'); break;
+                case 'u': {
+                    if (i + 4 > s.size()) bad("truncated \\u escape");
+                    unsigned cp = std::stoul(s.substr(i, 4), nullptr, 16);
+                    i += 4;
+                    if (cp < 0x80) {
+                        out.push_back((char)cp);
+                    } else if (cp < 0x800) {
+                        out.push_back((char)(0xc0 | (cp >> 6)));
+                        out.push_back((char)(0x80 | (cp & 0x3f)));
+                    } else {
+                        out.push_back((char)(0xe0 | (cp >> 12)));
+                        out.push_back((char)(0x80 | ((cp >> 6) & 0x3f)));
+                        out.push_back((char)(0x80 | (cp & 0x3f)));
+                    }
+                    break;
+                }
+                default: bad("unknown escape");
+            }
+        }
+        if (i >= s.size()) bad("unterminated string");
+        i++;
+        return out;
+    }
+    Json value() {
+        ws();
+        if (i >= s.size()) bad("unexpected end");
+        Json v;
+        char c = s[i];
+        if (c == '{') {
+            v.kind = Json::Obj;
+            i++;
+            ws();
+            if (i < s.size() && s[i] == '}') {
+                i++;
+                return v;
+            }
+            for (;;) {
+                ws();
+                std::string k = string();
+                ws();
+                if (i >= s.size() || s[i] != ':') bad("expected ':'");
+                i++;
+                v.obj[k] = value();
+                ws();
+                if (i < s.size() && s[i] == ',') {
+                    i++;
+                    continue;
+                }
+                if (i < s.size() && s[i] == '}') {
+                    i++;
+                    return v;
+                }
+                bad("expected ',' or '}'");
+            }
+        }
+        if (c == '[') {
+            v.kind = Json::Arr;
+            i++;
+            ws();
+            if (i < s.size() && s[i] == ']') {
+                i++;
+                return v;
+            }
+            for (;;) {
+                v.arr.push_back(value());
+                ws();
+                if (i < s.size() && s[i] == ',') {
+                    i++;
+                    continue;
+                }
+                if (i < s.size() && s[i] == ']') {
+                    i++;
+                    return v;
+                }
+                bad("expected ',' or ']'");
+            }
+        }
+        if (c == '"') {
+            v.kind = Json::Str;
+            v.str = string();
+            return v;
+        }
+        if (lit("true")) {
+            v.kind = Json::Bool;
+            v.b = true;
+            return v;
+        }
+        if (lit("false")) {
+            v.kind = Json::Bool;
+            return v;
+        }
+        if (lit("null")) return v;
+        size_t st = i;
+        while (i < s.size() && (std::strchr("+-0123456789.eE", s[i]) != nullptr)) i++;
+        if (st == i) bad("unexpected character");
+        v.kind = Json::Num;
+        try {
+            v.num = std::stod(s.substr(st, i - st));
+        } catch (...) {
+            bad("bad number");
+        }
+        return v;
+    }
+};
+
+const Json& need(const Json& j, const std::string& key) {
+    const Json* v = j.find(key);
+    if (!v) fail(J2K_ERR_BAD_CONFIG, "config.json: missing field `" + key + "`");
+    return *v;
+}
+
+double num(const Json& j, const std::string& key) {
+    const Json& v = need(j, key);
+    if (v.kind != Json::Num) fail(J2K_ERR_BAD_CONFIG, "config.json: field `" + key + "` is not a number");
+    return v.num;
+}
+
+int uint_field(const Json& j, const std::string& key) {
+    double d = num(j, key);
+    if (d < 0 || d != std::floor(d) || d > 1e9) fail(J2K_ERR_BAD_CONFIG, "config.json: field `" + key + "` is not a size");
+    return (int)d;
+}
+
+}  // namespace
+
+Json Json::parse(const std::string& text) {
+    Parser p{text};
+    Json v = p.value();
+    p.ws();
+    if (p.i != text.size()) p.bad("trailing characters");
+    return v;
+}
+
+const Json* Json::find(const std::string& key) const {
+    if (kind != Obj) return nullptr;
+    auto it = obj.find(key);
+    return it == obj.end() ? nullptr : &it->second;
+}
+
+Weights read_safetensors(const uint8_t* bytes, size_t n) {
+    if (!bytes || n < 8) fail(J2K_ERR_SHAPE, "safetensors: buffer shorter than its 8-byte header length");
+    uint64_t hl = 0;
+    for (int b = 7; b >= 0; b--) hl = (hl << 8) | bytes[b];
+    if (hl > n - 8) fail(J2K_ERR_SHAPE, "safetensors: header length past the end of the buffer");
+    Json h = Json::parse(std::string((const char*)bytes + 8, (size_t)hl));
+    if (h.kind != Json::Obj) fail(J2K_ERR_SHAPE, "safetensors: header is not a JSON object");
+    size_t data_start = 8 + (size_t)hl;
+    Weights out;
+    for (auto& [name, v] : h.obj) {
+        if (name == "__metadata__") continue;
+        const Json* dt = v.find("dtype");
+        if (!dt || dt->kind != Json::Str) fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` has no dtype");
+        if (dt->str != "F32")
+            fail(J2K_ERR_BAD_DTYPE, "tensor `" + name + "`: only F32 is supported (got " + dt->str + ")");
+        const Json* shp = v.find("shape");
+        const Json* off = v.find("data_offsets");
+        if (!shp || shp->kind != Json::Arr || !off || off->kind != Json::Arr || off->arr.size() != 2)
+            fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` lacks shape/data_offsets");
+        Tensor t;
+        for (auto& d : shp->arr) t.shape.push_back((int64_t)d.num);
+        size_t a = (size_t)off->arr[0].num, b = (size_t)off->arr[1].num;
+        if (b < a || data_start + b > n || (b - a) != (size_t)t.numel() * 4)
+            fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` data range does not match its shape");
+        t.data.resize((size_t)t.numel());
+        std::memcpy(t.data.data(), bytes + data_start + a, b - a);  // little-endian host
+        out.emplace(name, std::move(t));
+    }
+    return out;
+}
+
+const Tensor& get(const Weights& w, const std::string& name) {
+    auto it = w.find(name);
+    if (it == w.end()) fail(J2K_ERR_MISSING_TENSOR, "missing tensor `" + name + "`");
+    return it->second;
+}
+
+j2k_config parse_config(const std::string& json_text, const Weights& w, int precision) {
+    Json j = Json::parse(json_text);
+    if (j.kind != Json::Obj) fail(J2K_ERR_BAD_CONFIG, "config.json: not an object");
+    j2k_config c{};
+    c.precision = precision;
+    const Json* form = j.find("form");
+    if (!form || (form->kind == Json::Str && form->str == "patch2d")) {
+        c.form = J2K_FORM_PATCH2D;
+    } else if (form->kind == Json::Str && form->str == "cell_st") {
+        c.form = J2K_FORM_CELL_ST;
+    } else {
+        fail(J2K_ERR_BAD_CONFIG, "config.json: unknown `form` (want patch2d | cell_st)");
+    }
+    c.channels = uint_field(j, "channels");
+    c.dim = uint_field(j, "dim");
+    c.layers = uint_field(j, "layers");
+    c.heads = uint_field(j, "heads");
+    c.meta_dim = uint_field(j, "meta_dim");
+    const Json& nrm = need(j, "norm");
+    const Json* mean = nrm.find("mean");
+    const Json* stdv = nrm.find("std");
+    const Json* depth = nrm.find("depth");
+    if (!mean || !stdv || !depth || mean->kind != Json::Arr || stdv->kind != Json::Arr || mean->arr.size() != 3 ||
+        stdv->arr.size() != 3 || depth->kind != Json::Num)
+        fail(J2K_ERR_BAD_CONFIG, "config.json: `norm` needs mean[3], std[3], depth");
+    for (int i = 0; i < 3; i++) {
+        c.norm_mean[i] = (float)mean->arr[i].num;
+        c.norm_std[i] = (float)stdv->arr[i].num;
+    }
+    c.norm_depth = (float)depth->num;
+    c.dx = (float)num(j, "dx");
+    c.dt = (float)num(j, "dt");
+    c.g = (float)num(j, "g");
+    c.depth = (float)num(j, "depth");
+    c.nu = (float)num(j, "nu");
+    const Json* rm = j.find("energy_r_max");
+    c.energy_r_max = rm ? (float)rm->num : 1.0f;  // config.rs:33-35 default
+    if (c.channels != 3) fail(J2K_ERR_BAD_CONFIG, "config.json: channels must be 3 (h, u, v)");
+    if (c.dim == 0 || c.heads == 0 || c.dim % c.heads != 0)
+        fail(J2K_ERR_BAD_CONFIG, "config.json: dim must be a positive multiple of heads");
+    c.ff_inner = (int)get(w, "blocks.0.ff1.weight").shape.at(1);
+    c.max_iter = 32;
+    c.tol = 1e-4f;
+
+    if (c.form == J2K_FORM_PATCH2D) {
+        c.nx = uint_field(j, "nx");
+        c.ny = uint_field(j, "ny");
+        c.patch = uint_field(j, "patch");
+        if (c.patch == 0 || c.nx % c.patch || c.ny % c.patch)
+            fail(J2K_ERR_BAD_CONFIG, "config.json: nx and ny must be multiples of patch");
+        c.head_dim = (int)get(w, "head.to_cells.weight").shape.at(0);
+        c.tokens_x = c.nx / c.patch;
+        c.tokens_y = c.ny / c.patch;
+        c.tokens = c.tokens_x * c.tokens_y;
+        c.patch_dim = c.channels * c.patch * c.patch;
+        const Json* m = j.find("mask");  // config.rs:37-39: default Full
+        if (!m || (m->kind == Json::Str && m->str == "Full")) {
+            c.mask_kind = J2K_MASK_FULL;
+            c.radius = 0;
+            c.dilation = 1;
+        } else if (m->kind == Json::Obj && m->find("Stencil")) {
+            const Json& s = *m->find("Stencil");
+            c.mask_kind = J2K_MASK_STENCIL;
+            c.radius = uint_field(s, "radius");
+            c.dilation = uint_field(s, "dilation");
+            if (c.dilation == 0) fail(J2K_ERR_BAD_CONFIG, "config.json: stencil dilation must be >= 1");
+        } else {
+            fail(J2K_ERR_BAD_CONFIG, "config.json: `mask` must be \"Full\" or {\"Stencil\": {radius, dilation}}");
+        }
+    } else {
+        const Json& s = need(j, "stencil");
+        c.mask_kind = J2K_MASK_STENCIL;
+        c.radius = uint_field(s, "radius");
+        c.time_radius = uint_field(s, "time_radius");
+        c.dilation = 1;
+        c.head_dim = 0;
+        c.patch = 1;
+        c.patch_dim = c.channels + 1;
+        if (const Json* mi = j.find("max_iter")) c.max_iter = (int)mi->num;
+        if (const Json* t = j.find("tol")) c.tol = (float)t->num;
+    }
+    return c;
+}
+
+}  // namespace j2k

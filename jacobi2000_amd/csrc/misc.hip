@@ -1,0 +1,311 @@
+// LayerNorm, patch2d glue kernels (patchify, meta linear, transpose,
+// upsample) and the cell_st per-token kernels (pack, embed+LN, readout).
+#include "common.h"
+#include "kernels.h"
+
+namespace j2k {
+
+// ---- LayerNorm: one wave per row, cpu.rs:49-65 (two-pass mean/variance,
+// biased, eps 1e-5 inside the sqrt). dim <= 1024.
+__global__ __launch_bounds__(256) void k_layernorm(int rows, int dim, const float* __restrict__ x,
+                                                   const float* __restrict__ g, const float* __restrict__ b,
+                                                   float* __restrict__ out, uint16_t* __restrict__ out_bf) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;
+    const float* xr = x + (long)row * dim;
+    float v[16];
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        v[i] = d < dim ? xr[d] : 0.f;
+        s += v[i];
+    }
+    const float mean = wave_sum(s) / (float)dim;
+    float q = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        float c = v[i] - mean;
+        q += d < dim ? c * c : 0.f;
+    }
+    const float inv = 1.0f / sqrtf(wave_sum(q) / (float)dim + 1e-5f);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        if (d < dim) {
+            float o = (v[i] - mean) * inv * g[d] + b[d];
+            if (out) out[(long)row * dim + d] = o;
+            if (out_bf) out_bf[(long)row * dim + d] = f2bf(o);
+        }
+    }
+}
+
+void layernorm(hipStream_t s, int rows, int dim, const float* x, const float* g, const float* b, float* out,
+               uint16_t* out_bf) {
+    if (dim > 1024) fail(J2K_ERR_LIMIT, "layernorm: dim > 1024 is not supported");
+    if (rows <= 0) return;
+    hipLaunchKernelGGL(k_layernorm, dim3(cdiv(rows, 4)), dim3(256), 0, s, rows, dim, x, g, b, out, out_bf);
+    LAUNCHCHK("layernorm");
+}
+
+// ---- patch2d glue ----------------------------------------------------------
+
+__global__ void k_patchify(int B, int C, int ny, int nx, int p, const float* __restrict__ st, float* __restrict__ pt) {
+    const int tx = nx / p, ty = ny / p, tokens = tx * ty, pd = C * p * p;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x, total = (long)B * tokens * pd;
+    if (i >= total) return;
+    int d = (int)(i % pd);
+    long bt = i / pd;
+    int t = (int)(bt % tokens), b = (int)(bt / tokens);
+    int c = d / (p * p), r = d % (p * p), py = r / p, px = r % p;
+    int y = (t / tx) * p + py, x = (t % tx) * p + px;
+    pt[i] = st[(((long)b * C + c) * ny + y) * nx + x];
+}
+
+void patchify(hipStream_t s, int B, int C, int ny, int nx, int p, const float* state, float* patches) {
+    long total = (long)B * (nx / p) * (ny / p) * C * p * p;
+    hipLaunchKernelGGL(k_patchify, dim3(cdiv(total, 256)), dim3(256), 0, s, B, C, ny, nx, p, state, patches);
+    LAUNCHCHK("patchify");
+}
+
+// cpu.rs:26-45 for one row: bias first, then inputs in order.
+__global__ void k_meta_linear(int md, int dim, const float* meta, const float* w, const float* b, float* out) {
+    int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= dim) return;
+    float acc = b[o];
+    for (int i = 0; i < md; i++) acc += meta[i] * w[i * dim + o];
+    out[o] = acc;
+}
+
+void meta_linear(hipStream_t s, int meta_dim, int dim, const float* meta, const float* w, const float* b, float* out) {
+    hipLaunchKernelGGL(k_meta_linear, dim3(cdiv(dim, 256)), dim3(256), 0, s, meta_dim, dim, meta, w, b, out);
+    LAUNCHCHK("meta_linear");
+}
+
+__global__ void k_transpose_td(int B, int tokens, int dim, const float* __restrict__ in, float* __restrict__ out) {
+    __shared__ float tile[32][33];
+    const int b = blockIdx.z, t0 = blockIdx.x * 32, d0 = blockIdx.y * 32;
+    const float* ib = in + (long)b * tokens * dim;
+    float* ob = out + (long)b * tokens * dim;
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        int t = t0 + r, d = d0 + threadIdx.x;
+        if (t < tokens && d < dim) tile[r][threadIdx.x] = ib[(long)t * dim + d];
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += 8) {
+        int d = d0 + r, t = t0 + threadIdx.x;
+        if (t < tokens && d < dim) ob[(long)d * tokens + t] = tile[threadIdx.x][r];
+    }
+}
+
+void transpose_td(hipStream_t s, int B, int tokens, int dim, const float* in, float* out) {
+    dim3 grid(cdiv(tokens, 32), cdiv(dim, 32), B);
+    hipLaunchKernelGGL(k_transpose_td, grid, dim3(32, 8), 0, s, B, tokens, dim, in, out);
+    LAUNCHCHK("transpose_td");
+}
+
+// cpu.rs:211-245: upsample along y then along x, fused. The x pass reads the
+// y-upsampled values a(1-w)+b*w of columns xt and min(xt+1, w-1).
+__global__ void k_upsample2(int B, int c, int h, int w, int p, const float* __restrict__ in, float* __restrict__ out,
+                            long out_bstride) {
+    const int H = h * p, W = w * p;
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x, total = (long)B * c * H * W;
+    if (i >= total) return;
+    int X = (int)(i % W);
+    long r = i / W;
+    int Y = (int)(r % H);
+    long bc = r / H;
+    int ci = (int)(bc % c), b = (int)(bc / c);
+    const float* src = in + ((long)b * c + ci) * h * w;
+    int yt = Y / p, iy = Y % p, ynext = yt + 1 < h - 1 ? yt + 1 : h - 1;
+    int xt = X / p, ix = X % p, xnext = xt + 1 < w - 1 ? xt + 1 : w - 1;
+    float wy = ((float)iy + 0.5f) / (float)p, wx = ((float)ix + 0.5f) / (float)p;
+    float a = src[yt * w + xt] * (1.0f - wy) + src[ynext * w + xt] * wy;
+    float bb = src[yt * w + xnext] * (1.0f - wy) + src[ynext * w + xnext] * wy;
+    out[(long)b * out_bstride + ((long)ci * H + Y) * W + X] = a * (1.0f - wx) + bb * wx;
+}
+
+void upsample2(hipStream_t s, int B, int c, int h, int w, int p, const float* in, float* out, long out_bstride) {
+    long total = (long)B * c * h * p * w * p;
+    hipLaunchKernelGGL(k_upsample2, dim3(cdiv(total, 256)), dim3(256), 0, s, B, c, h, w, p, in, out, out_bstride);
+    LAUNCHCHK("upsample2");
+}
+
+__global__ void k_add(long n, float* x, const float* y) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += y[i];
+}
+
+void add_inplace(hipStream_t s, long n, float* x, const float* y) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_add, dim3(cdiv(n, 256)), dim3(256), 0, s, n, x, y);
+    LAUNCHCHK("add_inplace");
+}
+
+// ---- cell_st per-token kernels ---------------------------------------------
+
+__global__ void k_cell_pack(int M, const float* __restrict__ g, const uint8_t* __restrict__ kn, float4* __restrict__ x4) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    x4[m] = make_float4(g[m], g[(long)M + m], g[2 * (long)M + m], kn[m] ? 1.0f : 0.0f);
+}
+
+void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4) {
+    hipLaunchKernelGGL(k_cell_pack, dim3(cdiv(M, 256)), dim3(256), 0, s, M, grid, known, (float4*)x4);
+    LAUNCHCHK("cell_pack");
+}
+
+__global__ void k_cell_unpack(int M, const float4* __restrict__ x4, float* __restrict__ g) {
+    int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    float4 v = x4[m];
+    g[m] = v.x;
+    g[(long)M + m] = v.y;
+    g[2 * (long)M + m] = v.z;
+}
+
+void cell_unpack(hipStream_t s, int M, const float* x4, float* grid) {
+    hipLaunchKernelGGL(k_cell_unpack, dim3(cdiv(M, 256)), dim3(256), 0, s, M, (const float4*)x4, grid);
+    LAUNCHCHK("cell_unpack");
+}
+
+// One wave per token: X = be + sum_i f_i We[i] (inputs in order, as cpu.rs:26-45)
+// + mvec, then H = LN(X) with the first block's norm1.
+__global__ __launch_bounds__(256) void k_cell_embed_ln(int M, int dim, const float4* __restrict__ x4,
+                                                       const float* __restrict__ we, const float* __restrict__ be,
+                                                       const float* __restrict__ mvec, const int* __restrict__ tok_req,
+                                                       const float* __restrict__ g, const float* __restrict__ b,
+                                                       float* __restrict__ X, float* __restrict__ H,
+                                                       uint16_t* __restrict__ Hb) {
+    const int lane = threadIdx.x & 63;
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= M) return;
+    const float4 f = x4[m];
+    const float* mv = mvec + (tok_req ? (long)tok_req[m] * dim : 0);
+    float v[16];
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        float a = 0.f;
+        if (d < dim) {
+            a = be[d];
+            if (f.x != 0.f) a += f.x * we[d];
+            if (f.y != 0.f) a += f.y * we[dim + d];
+            if (f.z != 0.f) a += f.z * we[2 * dim + d];
+            if (f.w != 0.f) a += f.w * we[3 * dim + d];
+            a += mv[d];
+            X[(long)m * dim + d] = a;
+        }
+        v[i] = a;
+        s += a;
+    }
+    const float mean = wave_sum(s) / (float)dim;
+    float q = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        float c = v[i] - mean;
+        q += d < dim ? c * c : 0.f;
+    }
+    const float inv = 1.0f / sqrtf(wave_sum(q) / (float)dim + 1e-5f);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        if (d < dim) {
+            float o = (v[i] - mean) * inv * g[d] + b[d];
+            if (H) H[(long)m * dim + d] = o;
+            if (Hb) Hb[(long)m * dim + d] = f2bf(o);
+        }
+    }
+}
+
+void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* we, const float* be, const float* mvec,
+                   const int* tok_req, const float* g, const float* b, float* X, float* H, uint16_t* Hb) {
+    if (dim > 1024) fail(J2K_ERR_LIMIT, "cell_embed_ln: dim > 1024 is not supported");
+    hipLaunchKernelGGL(k_cell_embed_ln, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, (const float4*)x4, we, be, mvec,
+                       tok_req, g, b, X, H, Hb);
+    LAUNCHCHK("cell_embed_ln");
+}
+
+// One wave per token: final LN of X, y = LN(X) @ Wr + br (3 outputs, wave
+// reductions), xn = known ? x0 : x0 + y (known flag carried in .w).
+__global__ __launch_bounds__(256) void k_cell_readout(int M, int dim, const float* __restrict__ X,
+                                                      const float* __restrict__ g, const float* __restrict__ b,
+                                                      const float* __restrict__ wr, const float* __restrict__ br,
+                                                      const float4* __restrict__ x0, float4* __restrict__ xn,
+                                                      float* __restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (m >= M) return;
+    const float* xr = X + (long)m * dim;
+    float v[16];
+    float s = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        v[i] = d < dim ? xr[d] : 0.f;
+        s += v[i];
+    }
+    const float mean = wave_sum(s) / (float)dim;
+    float q = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        float c = v[i] - mean;
+        q += d < dim ? c * c : 0.f;
+    }
+    const float inv = 1.0f / sqrtf(wave_sum(q) / (float)dim + 1e-5f);
+    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        int d = lane + i * 64;
+        if (d < dim) {
+            float o = (v[i] - mean) * inv * g[d] + b[d];
+            y0 += o * wr[d * 3 + 0];
+            y1 += o * wr[d * 3 + 1];
+            y2 += o * wr[d * 3 + 2];
+        }
+    }
+    y0 = wave_sum(y0) + br[0];
+    y1 = wave_sum(y1) + br[1];
+    y2 = wave_sum(y2) + br[2];
+    if (lane == 0) {
+        if (y) {
+            y[(long)m * 3] = y0;
+            y[(long)m * 3 + 1] = y1;
+            y[(long)m * 3 + 2] = y2;
+        }
+        if (xn) {
+            float4 a = x0[m];
+            xn[m] = a.w != 0.f ? a : make_float4(a.x + y0, a.y + y1, a.z + y2, 0.f);
+        }
+    }
+}
+
+void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g, const float* b, const float* wr,
+                  const float* br, const float* x0, float* xn, float* y) {
+    if (dim > 1024) fail(J2K_ERR_LIMIT, "cell_readout: dim > 1024 is not supported");
+    hipLaunchKernelGGL(k_cell_readout, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, X, g, b, wr, br, (const float4*)x0,
+                       (float4*)xn, y);
+    LAUNCHCHK("cell_readout");
+}
+
+}  // namespace j2k
+
+namespace j2k {
+
+__global__ void k_fill_u32(unsigned* p, int n, unsigned v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void fill_u32(hipStream_t s, unsigned* p, int n, unsigned v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_u32, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, v);
+    LAUNCHCHK("fill_u32");
+}
+
+}  // namespace j2k

@@ -1,0 +1,72 @@
+// The device-side model: weights in HBM, one stream, growable scratch, and
+// the forward passes both forms are built from.
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+#include "kernels.h"
+#include "loader.h"
+
+namespace j2k {
+
+struct LayerW {
+    float *n1g, *n1b, *wqkv, *bqkv, *wp, *bp, *n2g, *n2b, *w1, *b1, *w2, *b2;
+    uint16_t *wqkv_t, *wp_t, *w1_t, *w2_t;  // bf16 copies, transposed to [N, K]
+};
+
+// Scratch slots, each grown on demand (the stream is drained before a regrow).
+enum Slot : int {
+    S_X, S_H, S_HB, S_QKV, S_O, S_F, S_XN, S_T0, S_T1, S_Z0, S_Z2, S_C1, S_PATCH, S_MVEC,
+    S_WIN_A, S_WIN_B, S_SUMS, S_SEGS, S_TILES, S_SLICES, S_X0, S_XA, S_XB, S_TARG, S_RESID, S_GRID, S_KNOWN, S_META,
+    S_TOKREQ, S_COUNT
+};
+
+struct Model {
+    j2k_config cfg{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> owned;
+    float *embed_w = nullptr, *embed_b = nullptr, *pos = nullptr, *meta_w = nullptr, *meta_b = nullptr;
+    float *norm_g = nullptr, *norm_b = nullptr;
+    float *tm_w = nullptr, *tm_b = nullptr, *tc_w = nullptr, *tc_b = nullptr, *c1_w = nullptr, *c1_b = nullptr,
+          *c2_w = nullptr, *c2_b = nullptr;
+    float *ro_w = nullptr, *ro_b = nullptr;
+    std::vector<LayerW> L;
+    void* slot[S_COUNT] = {};
+    size_t cap[S_COUNT] = {};
+    int p_tables_batch = -1;  // batch size the patch2d seg/tile tables were built for
+    int p_tiles = 0;
+
+    ~Model();
+    void* scratch(int s, size_t bytes);
+    template <class T>
+    T* get(int s, size_t count) {
+        return static_cast<T*>(scratch(s, count * sizeof(T)));
+    }
+    AttnGeo geo() const;
+    bool bf16() const { return cfg.precision == J2K_PREC_BF16; }
+};
+
+// Makes `m.device` current for the scope of a call.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev);
+    ~DeviceGuard();
+};
+
+Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision);
+
+// The transformer trunk (cpu.rs:287-313) over M packed tokens: X [M, dim] is
+// the residual stream; H must hold LN1 of block 0 when h_ready, else it is
+// computed. Leaves the stream un-normalised (callers apply the final norm).
+void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles, int n_tiles);
+
+// patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
+// add_state the result is state + forward (cpu.rs:84-88).
+void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state);
+
+// Tile table for a list of segments (host side).
+void build_tiles(const std::vector<Seg>& segs, int tpb, std::vector<Tile>& tiles);
+
+}  // namespace j2k

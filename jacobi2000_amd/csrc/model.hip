@@ -1,0 +1,280 @@
+// Model construction (weights to HBM) and the two forward passes' shared parts.
+#include "model.h"
+
+#include <cstring>
+
+namespace j2k {
+
+DeviceGuard::DeviceGuard(int dev) {
+    HIPCHK(hipGetDevice(&prev));
+    if (prev != dev) HIPCHK(hipSetDevice(dev));
+}
+DeviceGuard::~DeviceGuard() {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
+}
+
+Model::~Model() {
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : owned) (void)hipFree(p);
+    for (int i = 0; i < S_COUNT; i++)
+        if (slot[i]) (void)hipFree(slot[i]);
+    if (stream) (void)hipStreamDestroy(stream);
+    if (prev >= 0) (void)hipSetDevice(prev);
+}
+
+void* Model::scratch(int s, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (cap[s] >= bytes) return slot[s];
+    if (slot[s]) {
+        HIPCHK(hipStreamSynchronize(stream));
+        HIPCHK(hipFree(slot[s]));
+        slot[s] = nullptr;
+        cap[s] = 0;
+    }
+    size_t want = bytes + bytes / 4;  // headroom so small growth does not reallocate every call
+    hipError_t e = hipMalloc(&slot[s], want);
+    if (e != hipSuccess) {
+        slot[s] = nullptr;
+        fail(J2K_ERR_OOM, "device scratch of " + std::to_string(want) + " bytes: " + hipGetErrorString(e));
+    }
+    cap[s] = want;
+    return slot[s];
+}
+
+AttnGeo Model::geo() const {
+    AttnGeo g{};
+    if (cfg.form == J2K_FORM_CELL_ST) {
+        g.kind = ATTN_CELL_ST;
+    } else {
+        g.kind = cfg.mask_kind == J2K_MASK_FULL ? ATTN_FULL : ATTN_PATCH_STENCIL;
+    }
+    g.radius = cfg.radius;
+    g.dilation = cfg.dilation;
+    g.time_radius = cfg.time_radius;
+    return g;
+}
+
+namespace {
+
+uint16_t host_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+struct Uploader {
+    Model& m;
+    template <class T>
+    T* put(const T* host, size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, count * sizeof(T) + 16), "hipMalloc(weights)", J2K_ERR_OOM);
+        m.owned.push_back(p);
+        hip_check(hipMemcpy(p, host, count * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(weights)", J2K_ERR_MAP);
+        return static_cast<T*>(p);
+    }
+    float* tensor(const Weights& w, const std::string& name, std::vector<int64_t> shape) {
+        const Tensor& t = get(w, name);
+        if (t.shape != shape) {
+            std::string want, have;
+            for (auto s : shape) want += std::to_string(s) + ",";
+            for (auto s : t.shape) have += std::to_string(s) + ",";
+            fail(J2K_ERR_SHAPE, "tensor `" + name + "` has shape [" + have + "], expected [" + want + "]");
+        }
+        return put(t.data.data(), t.data.size());
+    }
+    // [K, N] row-major f32 -> bf16 [N, K].
+    uint16_t* bf16_t(const std::vector<float>& w, int K, int N) {
+        std::vector<uint16_t> t((size_t)K * N);
+        for (int k = 0; k < K; k++)
+            for (int n = 0; n < N; n++) t[(size_t)n * K + k] = host_bf16(w[(size_t)k * N + n]);
+        return put(t.data(), t.size());
+    }
+};
+
+}  // namespace
+
+Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision) {
+    if (precision != J2K_PREC_F32 && precision != J2K_PREC_BF16) fail(J2K_ERR_INVALID, "precision must be F32 or BF16");
+    if (!cfg_json) fail(J2K_ERR_INVALID, "config_json is NULL");
+    Weights w = read_safetensors(st, n);
+    j2k_config cfg = parse_config(cfg_json, w, precision);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) fail(J2K_ERR_NO_ADAPTER, "no HIP device is visible");
+    if (device < 0 || device >= count)
+        fail(J2K_ERR_NO_DEVICE, "device " + std::to_string(device) + " out of range (" + std::to_string(count) + " visible)");
+    DeviceGuard dg(device);
+    auto m = std::make_unique<Model>();
+    m->cfg = cfg;
+    m->device = device;
+    hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
+    Uploader up{*m};
+    const int d = cfg.dim, ff = cfg.ff_inner;
+    if (cfg.dim / cfg.heads != 8 && cfg.dim / cfg.heads != 16 && cfg.dim / cfg.heads != 32 && cfg.dim / cfg.heads != 64)
+        fail(J2K_ERR_LIMIT, "head size dim/heads must be 8, 16, 32 or 64");
+    if (d > 1024) fail(J2K_ERR_LIMIT, "dim above 1024 is not supported");
+    if (cfg.form == J2K_FORM_PATCH2D) {
+        m->embed_w = up.tensor(w, "embed.weight", {cfg.patch_dim, d});
+        m->embed_b = up.tensor(w, "embed.bias", {d});
+        m->pos = up.tensor(w, "pos", {1, cfg.tokens, d});
+    } else {
+        m->embed_w = up.tensor(w, "embed.weight", {cfg.channels + 1, d});
+        m->embed_b = up.tensor(w, "embed.bias", {d});
+    }
+    m->meta_w = up.tensor(w, "meta.weight", {cfg.meta_dim, d});
+    m->meta_b = up.tensor(w, "meta.bias", {d});
+    for (int i = 0; i < cfg.layers; i++) {
+        const std::string p = "blocks." + std::to_string(i) + ".";
+        LayerW L{};
+        L.n1g = up.tensor(w, p + "norm1.gamma", {d});
+        L.n1b = up.tensor(w, p + "norm1.beta", {d});
+        L.n2g = up.tensor(w, p + "norm2.gamma", {d});
+        L.n2b = up.tensor(w, p + "norm2.beta", {d});
+        // Fused QKV weight [d, 3d]: row i = q.w[i] | k.w[i] | v.w[i] (gpu.rs:288-301 layout).
+        const Tensor& q = get(w, p + "q.weight");
+        const Tensor& k = get(w, p + "k.weight");
+        const Tensor& v = get(w, p + "v.weight");
+        for (const Tensor* t : {&q, &k, &v})
+            if (t->shape != std::vector<int64_t>{d, d}) fail(J2K_ERR_SHAPE, "tensor `" + p + "{q,k,v}.weight` must be [dim, dim]");
+        std::vector<float> wqkv((size_t)d * 3 * d), bqkv(3 * (size_t)d);
+        for (int r = 0; r < d; r++) {
+            std::memcpy(&wqkv[(size_t)r * 3 * d], &q.data[(size_t)r * d], d * 4);
+            std::memcpy(&wqkv[(size_t)r * 3 * d + d], &k.data[(size_t)r * d], d * 4);
+            std::memcpy(&wqkv[(size_t)r * 3 * d + 2 * d], &v.data[(size_t)r * d], d * 4);
+        }
+        const char* bn[3] = {"q.bias", "k.bias", "v.bias"};
+        for (int j = 0; j < 3; j++) {
+            const Tensor& b = get(w, p + bn[j]);
+            if (b.shape != std::vector<int64_t>{d}) fail(J2K_ERR_SHAPE, "tensor `" + p + bn[j] + "` must be [dim]");
+            std::memcpy(&bqkv[(size_t)j * d], b.data.data(), d * 4);
+        }
+        L.wqkv = up.put(wqkv.data(), wqkv.size());
+        L.bqkv = up.put(bqkv.data(), bqkv.size());
+        L.wp = up.tensor(w, p + "proj.weight", {d, d});
+        L.bp = up.tensor(w, p + "proj.bias", {d});
+        L.w1 = up.tensor(w, p + "ff1.weight", {d, ff});
+        L.b1 = up.tensor(w, p + "ff1.bias", {ff});
+        L.w2 = up.tensor(w, p + "ff2.weight", {ff, d});
+        L.b2 = up.tensor(w, p + "ff2.bias", {d});
+        if (precision == J2K_PREC_BF16) {
+            L.wqkv_t = up.bf16_t(wqkv, d, 3 * d);
+            L.wp_t = up.bf16_t(get(w, p + "proj.weight").data, d, d);
+            L.w1_t = up.bf16_t(get(w, p + "ff1.weight").data, d, ff);
+            L.w2_t = up.bf16_t(get(w, p + "ff2.weight").data, ff, d);
+        }
+        m->L.push_back(L);
+    }
+    m->norm_g = up.tensor(w, "norm.gamma", {d});
+    m->norm_b = up.tensor(w, "norm.beta", {d});
+    if (cfg.form == J2K_FORM_PATCH2D) {
+        const int hd = cfg.head_dim, c = cfg.channels;
+        m->tm_w = up.tensor(w, "head.token_mix.weight", {d, d, 3, 3});
+        m->tm_b = up.tensor(w, "head.token_mix.bias", {d});
+        m->tc_w = up.tensor(w, "head.to_cells.weight", {hd, d, 1, 1});
+        m->tc_b = up.tensor(w, "head.to_cells.bias", {hd});
+        m->c1_w = up.tensor(w, "head.conv1.weight", {hd, hd + c, 3, 3});
+        m->c1_b = up.tensor(w, "head.conv1.bias", {hd});
+        m->c2_w = up.tensor(w, "head.conv2.weight", {c, hd, 3, 3});
+        m->c2_b = up.tensor(w, "head.conv2.bias", {c});
+    } else {
+        m->ro_w = up.tensor(w, "readout.weight", {d, cfg.channels});
+        m->ro_b = up.tensor(w, "readout.bias", {cfg.channels});
+    }
+    return m.release();
+}
+
+void build_tiles(const std::vector<Seg>& segs, int tpb, std::vector<Tile>& tiles) {
+    tiles.clear();
+    for (size_t i = 0; i < segs.size(); i++) {
+        const int n = segs[i].T * segs[i].ny * segs[i].nx;
+        for (int st = 0; st < n; st += tpb) tiles.push_back(Tile{(int)i, st});
+    }
+}
+
+void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles, int n_tiles) {
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
+    const AttnGeo geo = m.geo();
+    const int tpb = attn_tokens_per_block(H);
+    if (!m.bf16()) {
+        float* Hh = m.get<float>(S_H, (size_t)M * d);
+        float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
+        float* O = m.get<float>(S_O, (size_t)M * d);
+        float* F = m.get<float>(S_F, (size_t)M * ff);
+        for (int l = 0; l < m.cfg.layers; l++) {
+            const LayerW& L = m.L[l];
+            if (l > 0 || !h_ready) layernorm(s, M, d, X, L.n1g, L.n1b, Hh, nullptr);
+            gemm_f32(s, M, 3 * d, d, Hh, d, L.wqkv, L.bqkv, QKV, 3 * d, EPI_STORE);
+            attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            gemm_f32(s, M, d, d, O, d, L.wp, L.bp, X, d, EPI_RESID);
+            layernorm(s, M, d, X, L.n2g, L.n2b, Hh, nullptr);
+            gemm_f32(s, M, ff, d, Hh, d, L.w1, L.b1, F, ff, EPI_GELU);
+            gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID);
+        }
+        return;
+    }
+    uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)M * d);
+    uint16_t* QKV = m.get<uint16_t>(S_QKV, (size_t)M * 3 * d);
+    uint16_t* O = m.get<uint16_t>(S_O, (size_t)M * d);
+    uint16_t* F = m.get<uint16_t>(S_F, (size_t)M * ff);
+    for (int l = 0; l < m.cfg.layers; l++) {
+        const LayerW& L = m.L[l];
+        if (l > 0 || !h_ready) layernorm(s, M, d, X, L.n1g, L.n1b, nullptr, Hb);
+        gemm_bf16(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, 3 * d, EPI_STORE);
+        attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+        gemm_bf16(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, d, EPI_RESID);
+        layernorm(s, M, d, X, L.n2g, L.n2b, nullptr, Hb);
+        gemm_bf16(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, ff, EPI_GELU);
+        gemm_bf16(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, d, EPI_RESID);
+    }
+}
+
+void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state) {
+    const j2k_config& c = m.cfg;
+    const int d = c.dim, tokens = c.tokens, M = B * tokens, hd = c.head_dim, cells = c.ny * c.nx;
+    float* patches = m.get<float>(S_PATCH, (size_t)M * c.patch_dim);
+    float* mvec = m.get<float>(S_MVEC, d);
+    float* X = m.get<float>(S_X, (size_t)M * d);
+    patchify(s, B, c.channels, c.ny, c.nx, c.patch, d_state, patches);
+    meta_linear(s, c.meta_dim, d, d_meta, m.meta_w, m.meta_b, mvec);
+    gemm_embed_p(s, M, d, c.patch_dim, patches, m.embed_w, m.embed_b, m.pos, tokens, mvec, X);
+
+    if (m.p_tables_batch != B) {
+        std::vector<Seg> segs(B);
+        for (int b = 0; b < B; b++) segs[b] = Seg{b * tokens, 1, c.tokens_y, c.tokens_x};
+        std::vector<Tile> tiles;
+        build_tiles(segs, attn_tokens_per_block(c.heads), tiles);
+        HIPCHK(hipStreamSynchronize(s));
+        Seg* ds = m.get<Seg>(S_SEGS, segs.size());
+        Tile* dt = m.get<Tile>(S_TILES, tiles.size());
+        HIPCHK(hipMemcpy(ds, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        m.p_tables_batch = B;
+        m.p_tiles = (int)tiles.size();
+    }
+    const Seg* d_segs = static_cast<const Seg*>(m.slot[S_SEGS]);
+    const Tile* d_tiles = static_cast<const Tile*>(m.slot[S_TILES]);
+    trunk(m, s, M, X, false, d_segs, d_tiles, m.p_tiles);
+
+    // Final norm and CellHead (cpu.rs:315-349).
+    float* xn = m.get<float>(S_XN, (size_t)M * d);
+    float* t0 = m.get<float>(S_T0, (size_t)M * d);
+    float* t1 = m.get<float>(S_T1, (size_t)M * d);
+    float* z0 = m.get<float>(S_Z0, (size_t)B * hd * tokens);
+    float* z2 = m.get<float>(S_Z2, (size_t)B * hd * cells);
+    float* c1 = m.get<float>(S_C1, (size_t)B * hd * cells);
+    layernorm(s, M, d, X, m.norm_g, m.norm_b, xn, nullptr);
+    transpose_td(s, B, tokens, d, xn, t0);
+    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t0, nullptr, d, m.tm_w, m.tm_b, d, 3, t1, 1, t0, nullptr);
+    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t1, nullptr, d, m.tc_w, m.tc_b, hd, 1, z0, 0, nullptr, nullptr);
+    upsample2(s, B, hd, c.tokens_y, c.tokens_x, c.patch, z0, z2, (long)hd * cells);
+    conv_f32(s, B, hd + c.channels, c.ny, c.nx, z2, d_state, hd, m.c1_w, m.c1_b, hd, 3, c1, 1, nullptr, nullptr);
+    conv_f32(s, B, hd, c.ny, c.nx, c1, nullptr, hd, m.c2_w, m.c2_b, c.channels, 3, d_out, 0,
+             add_state ? d_state : nullptr, nullptr);
+}
+
+}  // namespace j2k

@@ -1,0 +1,474 @@
+// Continuous-batching request loop for the cell_st form on one GPU.
+//
+// A worker thread owns the model's stream. Each turn it (1) admits queued
+// requests while the active set fits the HBM budget and max_active, (2)
+// repacks the active requests' tokens into one contiguous layout when the set
+// changed, (3) runs one fixed-point iteration over all of them (one ragged
+// launch per kernel per layer), replayed from a captured hipGraph while the
+// packing is unchanged, and (4) retires every request whose own residual fell
+// under tol, or that reached max_iter, copying its grid back to the host.
+//
+// A request's result does not depend on the mix: every kernel computes a
+// token from its own row and its own request's neighbours, and every
+// reduction (mass, energy, residual) runs per slice in a fixed order.
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "abi.h"
+#include "cell.h"
+
+namespace j2k {
+
+namespace {
+
+double now_s() {
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    CellDims dims{};
+    std::vector<float> grid;     // [3, T, ny, nx], blank cells filled; the result on retirement
+    std::vector<uint8_t> known;  // [T, ny, nx]
+    std::vector<float> meta;
+    j2k_result res{};
+    int iters = 0;
+    float resid = 0.f;
+    size_t bytes = 0;
+    int base = -1;  // first token in the current packing (-1: not yet on the device)
+};
+
+// Device bytes one token of an active request holds: three float4 state
+// vectors, its request index, and its rows of every activation buffer.
+size_t bytes_per_token(const Model& m) {
+    const size_t d = m.cfg.dim, ff = m.cfg.ff_inner, e = m.bf16() ? 2 : 4;
+    return 3 * 16 + 4 + d * 4 + d * e + 3 * d * e + d * e + ff * e;
+}
+
+}  // namespace
+
+struct Server {
+    Model& m;
+    size_t budget;
+    int max_active;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::unique_ptr<Request>> pending;
+    std::vector<std::unique_ptr<Request>> active;  // owned by the worker thread once admitted
+    std::map<uint64_t, std::unique_ptr<Request>> done;
+    size_t n_active = 0;  // for drain(): active.size() as last published under mu
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::string worker_error;
+    int worker_code = J2K_OK;
+    std::thread worker;
+    std::atomic<uint64_t> launches{0}, replays{0};
+    int peak_active = 0;
+    size_t peak_bytes = 0;
+
+    // Packed device state: x0 (anchors), xc (current iterate), xn (next).
+    // alt0/altc receive a repack, then swap with x0/xc.
+    CellTables tab;
+    CellPack pack;
+    bool packed = false;
+    float *x0 = nullptr, *xc = nullptr, *xn = nullptr, *alt0 = nullptr, *altc = nullptr;
+    int* d_tokreq = nullptr;
+    size_t cap_tok = 0;
+    Seg* d_segs = nullptr;
+    Tile* d_tiles = nullptr;
+    Slice* d_slices = nullptr;
+    float* d_mvec = nullptr;
+    unsigned* d_resid = nullptr;
+    double* d_targ = nullptr;
+    size_t cap_tab = 0;
+    std::vector<void*> owned;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0]: xc->xn with the current naming, [1]: swapped
+    int parity = 0;
+    bool graphs_enabled = true;
+    unsigned* h_resid = nullptr;  // pinned
+    size_t h_resid_cap = 0;
+
+    Server(Model& model, size_t b, int ma) : m(model), budget(b), max_active(ma) {}
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = true;
+        }
+        cv_work.notify_all();
+        if (worker.joinable()) worker.join();
+        int prev = -1;
+        (void)hipGetDevice(&prev);
+        (void)hipSetDevice(m.device);
+        (void)hipStreamSynchronize(m.stream);
+        drop_graphs();
+        for (void* p : owned) (void)hipFree(p);
+        if (h_resid) (void)hipHostFree(h_resid);
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+
+    template <class T>
+    T* dev_alloc(size_t count) {
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)), "server hipMalloc", J2K_ERR_OOM);
+        owned.push_back(p);
+        return static_cast<T*>(p);
+    }
+    void dev_free(void* p) {
+        if (!p) return;
+        auto it = std::find(owned.begin(), owned.end(), p);
+        if (it != owned.end()) owned.erase(it);
+        (void)hipFree(p);
+    }
+    void drop_graphs() {
+        for (auto& g : graph)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
+
+    // Rebuild the packing for `active`: surviving requests carry x0 and the
+    // current iterate to their new place, new requests are uploaded and packed.
+    void repack() {
+        hipStream_t s = m.stream;
+        HIPCHK(hipStreamSynchronize(s));
+        drop_graphs();
+        std::vector<CellDims> dims;
+        for (auto& r : active) dims.push_back(r->dims);
+        const bool multi = active.size() > 1;
+        tab.build(dims, attn_tokens_per_block(m.cfg.heads), multi);
+        const size_t M = (size_t)tab.M;
+        const bool grow = M > cap_tok;
+        const size_t want = grow ? M + M / 4 : cap_tok;
+        if (grow) {
+            dev_free(alt0);
+            dev_free(altc);
+            alt0 = dev_alloc<float>(4 * want);
+            altc = dev_alloc<float>(4 * want);
+        }
+        for (size_t i = 0; i < active.size(); i++) {
+            Request& r = *active[i];
+            const size_t nb = (size_t)tab.segs[i].base, T = (size_t)r.dims.tokens();
+            if (r.base < 0) {
+                float* dgrid = m.get<float>(S_GRID, 3 * T);
+                uint8_t* dk = m.get<uint8_t>(S_KNOWN, T);
+                HIPCHK(hipMemcpyAsync(dgrid, r.grid.data(), 12 * T, hipMemcpyHostToDevice, s));
+                HIPCHK(hipMemcpyAsync(dk, r.known.data(), T, hipMemcpyHostToDevice, s));
+                cell_pack(s, (int)T, dgrid, dk, alt0 + 4 * nb);
+                HIPCHK(hipMemcpyAsync(altc + 4 * nb, alt0 + 4 * nb, 16 * T, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipStreamSynchronize(s));  // S_GRID/S_KNOWN are reused by the next request
+            } else {
+                HIPCHK(hipMemcpyAsync(alt0 + 4 * nb, x0 + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
+                HIPCHK(hipMemcpyAsync(altc + 4 * nb, xc + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
+            }
+            r.base = (int)nb;
+        }
+        std::swap(x0, alt0);
+        std::swap(xc, altc);
+        if (grow) {
+            HIPCHK(hipStreamSynchronize(s));
+            dev_free(alt0);
+            dev_free(altc);
+            dev_free(xn);
+            dev_free(d_tokreq);
+            alt0 = dev_alloc<float>(4 * want);
+            altc = dev_alloc<float>(4 * want);
+            xn = dev_alloc<float>(4 * want);
+            d_tokreq = dev_alloc<int>(want);
+            cap_tok = want;
+        }
+        parity = 0;
+
+        const size_t nseg = tab.segs.size(), ntile = tab.tiles.size(), nsl = tab.slices.size();
+        const size_t need = std::max({nseg, ntile, nsl, (size_t)1});
+        if (need > cap_tab) {
+            for (void* p : {(void*)d_segs, (void*)d_tiles, (void*)d_slices, (void*)d_mvec, (void*)d_resid, (void*)d_targ})
+                dev_free(p);
+            const size_t w = need + need / 2;
+            d_segs = dev_alloc<Seg>(w);
+            d_tiles = dev_alloc<Tile>(w);
+            d_slices = dev_alloc<Slice>(w);
+            d_mvec = dev_alloc<float>(w * m.cfg.dim);
+            d_resid = dev_alloc<unsigned>(w + 4);
+            d_targ = dev_alloc<double>(2 * w);
+            cap_tab = w;
+        }
+        HIPCHK(hipMemcpy(d_segs, tab.segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_tiles, tab.tiles.data(), ntile * sizeof(Tile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_slices, tab.slices.data(), nsl * sizeof(Slice), hipMemcpyHostToDevice));
+        if (multi) HIPCHK(hipMemcpy(d_tokreq, tab.tok_req.data(), tab.tok_req.size() * sizeof(int), hipMemcpyHostToDevice));
+        float* dmeta = m.get<float>(S_META, (size_t)m.cfg.meta_dim * active.size());
+        for (size_t i = 0; i < active.size(); i++)
+            HIPCHK(hipMemcpy(dmeta + i * m.cfg.meta_dim, active[i]->meta.data(), m.cfg.meta_dim * 4, hipMemcpyHostToDevice));
+        for (size_t i = 0; i < active.size(); i++)
+            meta_linear(s, m.cfg.meta_dim, m.cfg.dim, dmeta + i * m.cfg.meta_dim, m.meta_w, m.meta_b,
+                        d_mvec + i * m.cfg.dim);
+        pack.M = (int)M;
+        pack.n_req = (int)active.size();
+        pack.n_slices = (int)nsl;
+        pack.n_tiles = (int)ntile;
+        pack.segs = d_segs;
+        pack.tiles = d_tiles;
+        pack.slices = d_slices;
+        pack.tok_req = multi ? d_tokreq : nullptr;
+        pack.mvec = d_mvec;
+        pack.mass_t = d_targ;
+        pack.eref_t = d_targ + nsl;
+        pack.resid = d_resid;
+        // targets depend on each request's x0 only, so recomputing them is exact
+        cell_targets(s, (int)nsl, d_slices, x0, pack.mass_t, pack.eref_t, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
+                     m.cfg.norm_depth);
+        // grow the packed activation scratch now, never inside a capture
+        const size_t d = m.cfg.dim, ff = m.cfg.ff_inner, e = m.bf16() ? 2 : 4;
+        m.get<float>(S_X, M * d);
+        if (m.bf16()) m.get<uint16_t>(S_HB, M * d);
+        else m.get<float>(S_H, M * d);
+        m.get<char>(S_QKV, M * 3 * d * e);
+        m.get<char>(S_O, M * d * e);
+        m.get<char>(S_F, M * ff * e);
+        m.p_tables_batch = -1;
+        if (active.size() > h_resid_cap) {
+            if (h_resid) HIPCHK(hipHostFree(h_resid));
+            h_resid = nullptr;
+            h_resid_cap = active.size() + 64;
+            hip_check(hipHostMalloc((void**)&h_resid, h_resid_cap * sizeof(unsigned)), "hipHostMalloc", J2K_ERR_OOM);
+        }
+        HIPCHK(hipStreamSynchronize(s));
+        packed = true;
+    }
+
+    // One iteration xc -> xn, then the names swap.
+    void run_iteration() {
+        hipStream_t s = m.stream;
+        if (graphs_enabled && !graph[parity]) {
+            hipGraph_t g = nullptr;
+            bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                try {
+                    cell_iteration(m, s, pack, x0, xc, xn);
+                } catch (const Error&) {
+                    ok = false;
+                }
+                hipError_t e = hipStreamEndCapture(s, &g);
+                ok = ok && e == hipSuccess && g && hipGraphInstantiate(&graph[parity], g, nullptr, nullptr, 0) == hipSuccess;
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (!ok) {
+                graph[parity] = nullptr;
+                graphs_enabled = false;
+                (void)hipGetLastError();
+            }
+        }
+        if (graphs_enabled && graph[parity]) {
+            HIPCHK(hipGraphLaunch(graph[parity], s));
+            replays++;
+        } else {
+            cell_iteration(m, s, pack, x0, xc, xn);
+        }
+        launches++;
+        std::swap(xc, xn);
+        parity ^= 1;
+    }
+
+    // Final energy violation and grid of active[i] (reads xc).
+    void finish(size_t i) {
+        Request& r = *active[i];
+        hipStream_t s = m.stream;
+        size_t first = 0;
+        for (size_t j = 0; j < i; j++) first += (size_t)active[j]->dims.T;
+        const int n = r.dims.ny * r.dims.nx;
+        std::vector<Slice> sl;
+        for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, 0});
+        Slice* dsl = m.get<Slice>(S_SLICES, sl.size());
+        unsigned* vb32 = m.get<unsigned>(S_RESID, 4);
+        HIPCHK(hipMemcpyAsync(dsl, sl.data(), sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
+        fill_u32(s, vb32, 2, 0u);
+        cell_violation(s, (int)sl.size(), dsl, xc, pack.eref_t + first, reinterpret_cast<unsigned long long*>(vb32),
+                       m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth, m.cfg.energy_r_max);
+        const size_t T = (size_t)r.dims.tokens();
+        float* dgrid = m.get<float>(S_GRID, 3 * T);
+        cell_unpack(s, (int)T, xc + 4 * (size_t)r.base, dgrid);
+        double v = 0;
+        HIPCHK(hipMemcpyAsync(&v, vb32, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(r.grid.data(), dgrid, 12 * T, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        r.res.iters = r.iters;
+        r.res.resid = r.resid;
+        r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
+        r.res.violation = v;
+        r.res.done_s = now_s();
+    }
+
+    void loop() {
+        DeviceGuard dg(m.device);
+        for (;;) {
+            bool changed = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+                if (stop) return;
+                size_t used = 0;
+                for (auto& r : active) used += r->bytes;
+                while (!pending.empty() && (int)active.size() < max_active) {
+                    Request* r = pending.front().get();
+                    if (used + r->bytes > budget) break;
+                    used += r->bytes;
+                    r->res.start_s = now_s();
+                    active.push_back(std::move(pending.front()));
+                    pending.pop_front();
+                    changed = true;
+                }
+                n_active = active.size();
+                peak_active = std::max(peak_active, (int)active.size());
+                peak_bytes = std::max(peak_bytes, used);
+            }
+            if (active.empty()) continue;
+            if (changed || !packed) repack();
+            run_iteration();
+            HIPCHK(hipMemcpyAsync(h_resid, pack.resid, 4 * active.size(), hipMemcpyDeviceToHost, m.stream));
+            HIPCHK(hipStreamSynchronize(m.stream));
+            std::vector<bool> out(active.size(), false);
+            bool any = false;
+            for (size_t i = 0; i < active.size(); i++) {
+                Request& r = *active[i];
+                r.iters++;
+                std::memcpy(&r.resid, &h_resid[i], 4);
+                if (r.resid < m.cfg.tol || r.iters >= m.cfg.max_iter) {
+                    finish(i);
+                    out[i] = any = true;
+                }
+            }
+            if (!any) continue;
+            std::lock_guard<std::mutex> lk(mu);
+            std::vector<std::unique_ptr<Request>> keep;
+            for (size_t i = 0; i < active.size(); i++) {
+                if (out[i]) done[active[i]->id] = std::move(active[i]);
+                else keep.push_back(std::move(active[i]));
+            }
+            active = std::move(keep);
+            n_active = active.size();
+            packed = false;  // survivors move to a compact layout next turn
+            cv_done.notify_all();
+        }
+    }
+
+    void start() {
+        worker = std::thread([this] {
+            try {
+                loop();
+            } catch (const Error& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                worker_error = e.what();
+                worker_code = e.code;
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                worker_error = e.what();
+                worker_code = J2K_ERR_POLL;
+            }
+            cv_done.notify_all();
+        });
+    }
+};
+
+}  // namespace j2k
+
+using namespace j2k;
+
+static Server& server_of(void* h) {
+    if (!h) fail(J2K_ERR_INVALID, "server handle is NULL");
+    return *static_cast<Server*>(h);
+}
+
+extern "C" {
+
+int j2k_server_create(void* model, size_t budget, int max_active, void** out) {
+    return guard([&] {
+        if (!model) fail(J2K_ERR_INVALID, "model handle is NULL");
+        if (!out) fail(J2K_ERR_INVALID, "out_server is NULL");
+        Model& m = *static_cast<Model*>(model);
+        if (m.cfg.form != J2K_FORM_CELL_ST) fail(J2K_ERR_INVALID, "the server runs cell_st models");
+        if (max_active <= 0) fail(J2K_ERR_INVALID, "max_active must be positive");
+        auto s = std::make_unique<Server>(m, budget, max_active);
+        s->start();
+        *out = s.release();
+    });
+}
+
+int j2k_server_submit(void* h, const float* grid, const uint8_t* known, const float* meta, int T, int ny, int nx,
+                      uint64_t* out_id) {
+    return guard([&] {
+        Server& s = server_of(h);
+        if (!grid || !known || !meta) fail(J2K_ERR_INVALID, "grid, known and meta must be non-NULL");
+        CellDims d{T, ny, nx};
+        check_cell_dims(s.m, d);
+        auto r = std::make_unique<Request>();
+        r->dims = d;
+        const size_t M = (size_t)d.tokens();
+        r->grid.resize(3 * M);
+        r->known.assign(known, known + M);
+        r->meta.assign(meta, meta + s.m.cfg.meta_dim);
+        const int frc = j2k_fill_blank(grid, known, T, ny, nx, r->grid.data());
+        if (frc) fail(frc, "blank fill failed");
+        r->bytes = M * bytes_per_token(s.m);
+        if (r->bytes > s.budget) fail(J2K_ERR_OOM, "request needs more device memory than the whole budget");
+        r->res.submit_s = now_s();
+        std::lock_guard<std::mutex> lk(s.mu);
+        if (s.worker_code) fail(s.worker_code, "server stopped: " + s.worker_error);
+        r->id = r->res.id = s.next_id++;
+        if (out_id) *out_id = r->id;
+        s.pending.push_back(std::move(r));
+        s.cv_work.notify_one();
+    });
+}
+
+int j2k_server_poll(void* h, uint64_t id, j2k_result* out, float* grid_out) {
+    return guard([&] {
+        Server& s = server_of(h);
+        std::lock_guard<std::mutex> lk(s.mu);
+        auto it = s.done.find(id);
+        if (it == s.done.end()) {
+            if (s.worker_code) fail(s.worker_code, "server stopped: " + s.worker_error);
+            if (id == 0 || id >= s.next_id) fail(J2K_ERR_INVALID, "unknown request id");
+            throw Error(J2K_PENDING, "request still running");
+        }
+        if (out) *out = it->second->res;
+        if (grid_out) std::memcpy(grid_out, it->second->grid.data(), it->second->grid.size() * sizeof(float));
+    });
+}
+
+int j2k_server_drain(void* h) {
+    return guard([&] {
+        Server& s = server_of(h);
+        std::unique_lock<std::mutex> lk(s.mu);
+        s.cv_done.wait(lk, [&] { return s.worker_code != 0 || (s.pending.empty() && s.n_active == 0); });
+        if (s.worker_code) fail(s.worker_code, "server stopped: " + s.worker_error);
+    });
+}
+
+int j2k_server_stats(void* h, uint64_t* launches, uint64_t* replays, int* peak_active, size_t* peak_bytes) {
+    return guard([&] {
+        Server& s = server_of(h);
+        std::lock_guard<std::mutex> lk(s.mu);
+        if (launches) *launches = s.launches;
+        if (replays) *replays = s.replays;
+        if (peak_active) *peak_active = s.peak_active;
+        if (peak_bytes) *peak_bytes = s.peak_bytes;
+    });
+}
+
+int j2k_server_destroy(void* h) {
+    return guard([&] { delete static_cast<Server*>(h); });
+}
+
+}  // extern "C"

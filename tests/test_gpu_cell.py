@@ -1,0 +1,232 @@
+"""GPU parity of the cell_st form (per-cell space-time tokens, fixed point)
+through the C ABI, against the CPU oracle: the model evaluation, every
+iteration of the fixed point, the energy bound, bf16 error against the f64
+oracle, and the continuous-batching server (each result the same alone and
+in the mix)."""
+
+import ctypes
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import jacobi2000_amd as j
+from jacobi2000_amd import _lib
+from jacobi2000_amd import checkpoint as ck
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+META = ck.meta_of(1.0, 0.09, 9.81, 1.0, 1e-3)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return ck.random_cell_st(dim=32, layers=2, heads=2, ff=64, seed=3, std=0.1)
+
+
+@pytest.fixture(scope="module")
+def model20_cell():
+    return ck.random_cell_st(seed=0)  # d=128, 4 layers, 4x32 heads, ff 512, 27-key stencil
+
+
+def gpu_forward(model: j.Model, grid, known):
+    _, T, ny, nx = grid.shape
+    dev = torch.device("cuda:0")
+    g = torch.from_numpy(np.ascontiguousarray(grid)).to(dev)
+    k = torch.from_numpy(np.ascontiguousarray(known, dtype=np.uint8)).to(dev)
+    m = torch.from_numpy(META).to(dev)
+    y = torch.empty((T * ny * nx, 3), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    model.cell_forward_dev(g.data_ptr(), k.data_ptr(), m.data_ptr(), T, ny, nx, y.data_ptr(), stream)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("shape", [(4, 16, 16), (1, 3, 3), (3, 5, 7)])
+def test_cell_forward_matches_oracle(small, shape):
+    T, ny, nx = shape
+    st, cj = small
+    gpu = j.Model.from_bytes(st, cj)
+    cpu = O.FormC.load(st, cj)
+    rng = np.random.default_rng(sum(shape))
+    g = rng.normal(0, 0.3, (3, T, ny, nx)).astype(np.float32)
+    k = (rng.random((T, ny, nx)) < 0.3).astype(np.uint8)
+    d = np.abs(gpu_forward(gpu, g, k) - cpu.forward(g, k, META)).max()
+    print(f"cell forward {shape} max_abs={d:.3e}")
+    assert d < 1e-5
+
+
+def test_config1_forward_matches_oracle(model20_cell):
+    """32x32 x 8 slices, model20 shape, 27-key stencil."""
+    st, cj = model20_cell
+    gpu = j.Model.from_bytes(st, cj)
+    cpu = O.FormC.load(st, cj)
+    g = ck.drops_grid(8, 32, 32, 1, 0, centred=True)
+    k = ck.mask_of("forward", 8, 32, 32)
+    x0 = O.fill_blank(g, k)
+    y = gpu_forward(gpu, x0, k)
+    ref = cpu.forward(x0, k, META)
+    per_field = np.abs(y - ref).max(0)
+    print("config1 forward max_abs per field", per_field)
+    assert per_field.max() < 1e-4
+
+
+@pytest.mark.parametrize("kind", ["forward", "rewind", "reconstruction"])
+def test_each_iteration_matches_oracle(small, kind):
+    st, cj = small
+    gpu = j.Model.from_bytes(st, cj)
+    cpu = O.FormC.load(st, cj)
+    T, ny, nx = 4, 12, 12
+    g = ck.drops_grid(T, ny, nx, 2, 5)
+    k = ck.mask_of(kind, T, ny, nx, seed=1, frac=0.1)
+    x0 = O.fill_blank(g, k)
+    for it in (1, 2, 3):
+        xg, itg, rg, vg = gpu.iterate(x0, k, META, max_iter=it, tol=0.0)
+        xc, itc, trc = cpu.iterate(x0, k, META, it, 0.0)
+        per_field = np.abs(xg - xc).reshape(3, -1).max(1)
+        print(f"{kind} iteration {it}: max_abs per field {per_field}, resid gpu={rg:.3e} cpu={trc[-1]:.3e}")
+        assert itg == itc == it
+        assert per_field.max() < 1e-4
+        assert vg <= 1e-6
+
+
+@pytest.mark.parametrize("kind", ["forward", "rewind", "reconstruction"])
+def test_config1_fixed_point(model20_cell, kind):
+    """Config 1: 32x32 x 8, drop h = 1 + 0.1 exp(-r^2/18), 8 iterations cap,
+    projection on; per-field error per iteration against the f32 oracle and
+    the energy bound."""
+    st, cj = model20_cell
+    gpu = j.Model.from_bytes(st, cj)
+    cpu = O.FormC.load(st, cj)
+    g = ck.drops_grid(8, 32, 32, 1, 0, centred=True)
+    k = ck.mask_of(kind, 8, 32, 32, seed=1)
+    x0 = O.fill_blank(g, k)
+    xg, itg, rg, vg = gpu.iterate(x0, k, META, max_iter=8, tol=1e-4)
+    xc, itc, trc = cpu.iterate(x0, k, META, 8, 1e-4)
+    per_field = np.abs(xg - xc).reshape(3, -1).max(1)
+    print(f"config1 {kind}: iters gpu={itg} cpu={itc} max_abs per field {per_field} violation {vg:.2e}")
+    assert itg == itc
+    assert per_field.max() < 1e-4
+    assert vg <= 1e-6
+
+
+def test_bf16_error_vs_f64_oracle(model20_cell):
+    st, cj = model20_cell
+    gpu = j.Model.from_bytes(st, cj, precision="bf16")
+    cpu = O.FormC.load(st, cj)
+    T, ny, nx = 8, 32, 32
+    g = ck.drops_grid(T, ny, nx, 5, 11)
+    k = ck.mask_of("forward", T, ny, nx)
+    x0 = O.fill_blank(g, k)
+    xg, itg, rg, vg = gpu.iterate(x0, k, META, max_iter=32, tol=1e-4)
+    xd, _, _ = cpu.iterate(x0, k, META, itg, 0.0, f64=True)
+    rel = np.linalg.norm((xg - xd).reshape(3, -1), axis=1) / np.linalg.norm(xd.reshape(3, -1), axis=1).clip(1e-30)
+    rel_all = np.linalg.norm(xg - xd) / np.linalg.norm(xd)
+    print(f"bf16 iters={itg} rel L2 per field {rel} overall {rel_all:.3e} violation {vg:.2e}")
+    assert rel_all < 1e-3
+    assert vg <= 1e-6
+
+
+def test_grid_smaller_than_stencil_is_rejected(small):
+    gpu = j.Model.from_bytes(*small)
+    with pytest.raises(j.J2KError) as e:
+        gpu.iterate(np.zeros((3, 2, 2, 2), np.float32), np.zeros((2, 2, 2), np.uint8), META)
+    assert e.value.kind == "ShapeMismatch"
+
+
+def _server(model: j.Model, budget: int, max_active: int):
+    h = ctypes.c_void_p()
+    _lib.call("j2k_server_create", model.handle, budget, max_active, ctypes.byref(h))
+    return h
+
+
+def _submit(h, grid, known):
+    _, T, ny, nx = grid.shape
+    rid = ctypes.c_uint64()
+    _lib.call("j2k_server_submit", h, grid.ctypes.data_as(ctypes.c_void_p), known.ctypes.data_as(ctypes.c_void_p),
+              META.ctypes.data_as(ctypes.c_void_p), T, ny, nx, ctypes.byref(rid))
+    return rid.value
+
+
+def _result(h, rid, shape):
+    out = np.empty(shape, np.float32)
+    res = _lib.Result()
+    _lib.call("j2k_server_poll", h, rid, ctypes.byref(res), out.ctypes.data_as(ctypes.c_void_p))
+    return res, out
+
+
+def _requests():
+    reqs = []
+    sizes = [(4, 12, 12), (8, 16, 16), (4, 10, 14), (8, 12, 12), (4, 16, 16), (8, 10, 10)]
+    kinds = ["forward", "rewind", "reconstruction"]
+    for i, (T, ny, nx) in enumerate(sizes):
+        g = ck.drops_grid(T, ny, nx, 2, 100 + i)
+        k = ck.mask_of(kinds[i % 3], T, ny, nx, seed=i, frac=0.1)
+        reqs.append((g, k))
+    return reqs
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16"])
+def test_server_results_same_alone_and_in_the_mix(small, precision):
+    model = j.Model.from_bytes(*small, precision=precision)
+    reqs = _requests()
+    alone = [model.iterate(j.fill_blank(g, k), k, META) for g, k in reqs]
+    h = _server(model, 1 << 34, 16)
+    try:
+        ids = []
+        for i, (g, k) in enumerate(reqs):
+            ids.append(_submit(h, g, k))
+            if i == 2:
+                time.sleep(0.01)  # later requests arrive while the first ones run
+        _lib.call("j2k_server_drain", h)
+        for (g, k), rid, (xa, ita, ra, va) in zip(reqs, ids, alone):
+            res, out = _result(h, rid, g.shape)
+            d = np.abs(out - xa).max()
+            print(f"request {rid}: iters mix={res.iters} alone={ita} max_abs={d:.2e} violation={res.violation:.1e}")
+            assert res.iters == ita
+            assert d <= 1e-6
+            assert res.violation <= 1e-6
+            assert res.done_s >= res.start_s >= res.submit_s
+        launches, replays, peak = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        _lib.call("j2k_server_stats", h, ctypes.byref(launches), ctypes.byref(replays), ctypes.byref(peak), None)
+        print(f"server: {launches.value} iterations launched, {replays.value} graph replays, peak {peak.value} active")
+        assert peak.value >= 2
+    finally:
+        _lib.call("j2k_server_destroy", h)
+
+
+def test_server_budget_admits_one_at_a_time(small):
+    model = j.Model.from_bytes(*small)
+    reqs = _requests()[:3]
+    h = _server(model, 16 * 16 * 8 * 4096, 16)  # fits one request's tokens, not two
+    try:
+        ids = [_submit(h, g, k) for g, k in reqs]
+        _lib.call("j2k_server_drain", h)
+        peak = ctypes.c_int()
+        _lib.call("j2k_server_stats", h, None, None, ctypes.byref(peak), None)
+        assert peak.value == 1
+        for (g, k), rid in zip(reqs, ids):
+            res, out = _result(h, rid, g.shape)
+            xa, ita, _, _ = model.iterate(j.fill_blank(g, k), k, META)
+            assert res.iters == ita and np.abs(out - xa).max() <= 1e-6
+    finally:
+        _lib.call("j2k_server_destroy", h)
+
+
+def test_server_pending_and_unknown_ids(small):
+    model = j.Model.from_bytes(*small)
+    h = _server(model, 1 << 30, 4)
+    try:
+        res = _lib.Result()
+        rc = _lib.lib().j2k_server_poll(h, 12345, ctypes.byref(res), None)
+        assert rc == _lib.ERR_INVALID
+        g, k = _requests()[0]
+        rid = _submit(h, g, k)
+        rc = _lib.lib().j2k_server_poll(h, rid, ctypes.byref(res), None)
+        assert rc in (_lib.OK, _lib.PENDING)
+        _lib.call("j2k_server_drain", h)
+        assert _lib.lib().j2k_server_poll(h, rid, ctypes.byref(res), None) == _lib.OK
+    finally:
+        _lib.call("j2k_server_destroy", h)

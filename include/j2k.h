@@ -87,6 +87,14 @@ int j2k_model_destroy(void* model);
 /* GpuModel::cfg (gpu.rs:427-429). */
 int j2k_model_config(const void* model, j2k_config* out);
 
+/* Profiling hook, no reference counterpart (the reference timed passes with
+ * GPU timestamp queries it did not commit): enable = 1 / 0 turns HIP-event
+ * timing of the model's launches on or off and clears the counters (-1 keeps
+ * the state). `out` (may be NULL) receives 12 doubles, per kernel class
+ * (GEMM, attention, LayerNorm, projection): total ms, total algorithmic work
+ * (FLOPs for GEMM, bytes otherwise), launches. Synchronises the model's stream. */
+int j2k_model_timing(void* model, int enable, double* out);
+
 /* ---- patch2d form (the reference forward) ----------------------------- */
 
 /* Model::forward / GpuModel::forward (lib.rs:53-55, gpu.rs:374-425):

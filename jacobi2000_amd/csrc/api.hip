@@ -178,6 +178,24 @@ int j2k_model_config(const void* h, j2k_config* out) {
     });
 }
 
+int j2k_model_timing(void* h, int enable, double* out) {
+    return guard([&] {
+        Model& m = model_of(h);
+        DeviceGuard dg(m.device);
+        m.timer.harvest();
+        if (out)
+            for (int c = 0; c < KC_COUNT; c++) {
+                out[3 * c] = m.timer.ms[c];
+                out[3 * c + 1] = m.timer.work[c];
+                out[3 * c + 2] = m.timer.n[c];
+            }
+        if (enable >= 0) {
+            m.timer.clear();
+            m.timer.on = enable != 0;
+        }
+    });
+}
+
 int j2k_forward_batch_dev(void* h, int batch, const float* st, const float* meta, float* out, int add_state, void* s) {
     return guard([&] {
         Model& m = model_of(h);

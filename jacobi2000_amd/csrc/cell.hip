@@ -48,6 +48,7 @@ void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0,
     embed_and_trunk(m, s, p, xc, X);
     cell_readout(s, p.M, c.dim, X, m.norm_g, m.norm_b, m.ro_w, m.ro_b, x0, xn, nullptr);
     fill_u32(s, p.resid, p.n_req, 0u);
+    TScope t(m, s, KC_PROJ, 48.0 * p.M);  // read xn and xc, write xn: three float4 per cell
     cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
                  c.norm_depth, c.energy_r_max);
 }

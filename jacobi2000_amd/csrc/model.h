@@ -22,8 +22,42 @@ enum Slot : int {
     S_TOKREQ, S_COUNT
 };
 
+// Kernel classes timed with HIP events when timing is on (the bench's
+// roofline figures): work is FLOPs for GEMM, algorithmic bytes otherwise.
+enum KClass : int { KC_GEMM = 0, KC_ATTN = 1, KC_NORM = 2, KC_PROJ = 3, KC_COUNT = 4 };
+
+struct KTimer {
+    bool on = false;
+    bool capturing = false;  // no event records inside a captured graph
+    struct Rec {
+        hipEvent_t a, b;
+        int cls;
+        double work;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    double ms[KC_COUNT] = {}, work[KC_COUNT] = {}, n[KC_COUNT] = {};
+    hipEvent_t take();
+    void harvest();
+    void clear();
+    ~KTimer();
+};
+
+struct Model;
+// Brackets the launches of its scope with events on `s` when timing is on.
+struct TScope {
+    Model& m;
+    hipStream_t s;
+    int cls;
+    double work;
+    hipEvent_t a = nullptr;
+    TScope(Model& m, hipStream_t s, int cls, double work);
+    ~TScope();
+};
+
 struct Model {
     j2k_config cfg{};
+    KTimer timer;
     int device = 0;
     hipStream_t stream = nullptr;
     std::vector<void*> owned;

@@ -45,6 +45,57 @@ void* Model::scratch(int s, size_t bytes) {
     return slot[s];
 }
 
+hipEvent_t KTimer::take() {
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+void KTimer::harvest() {
+    for (Rec& r : recs) {
+        HIPCHK(hipEventSynchronize(r.b));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, r.a, r.b));
+        ms[r.cls] += t;
+        work[r.cls] += r.work;
+        n[r.cls] += 1;
+        pool.push_back(r.a);
+        pool.push_back(r.b);
+    }
+    recs.clear();
+}
+
+void KTimer::clear() {
+    harvest();
+    for (int i = 0; i < KC_COUNT; i++) ms[i] = work[i] = n[i] = 0;
+}
+
+KTimer::~KTimer() {
+    for (Rec& r : recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+TScope::TScope(Model& mm, hipStream_t ss, int c, double w) : m(mm), s(ss), cls(c), work(w) {
+    if (m.timer.on && !m.timer.capturing) {
+        a = m.timer.take();
+        HIPCHK(hipEventRecord(a, s));
+    }
+}
+
+TScope::~TScope() {
+    if (!a) return;
+    hipEvent_t b = m.timer.take();
+    if (hipEventRecord(b, s) == hipSuccess) m.timer.recs.push_back(KTimer::Rec{a, b, cls, work});
+}
+
 AttnGeo Model::geo() const {
     AttnGeo g{};
     if (cfg.form == J2K_FORM_CELL_ST) {
@@ -205,15 +256,37 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
         float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
         float* O = m.get<float>(S_O, (size_t)M * d);
         float* F = m.get<float>(S_F, (size_t)M * ff);
+        const double ln_b = (double)M * d * 8, at_b = 16.0 * M * d, g = 2.0 * M * d;
         for (int l = 0; l < m.cfg.layers; l++) {
             const LayerW& L = m.L[l];
-            if (l > 0 || !h_ready) layernorm(s, M, d, X, L.n1g, L.n1b, Hh, nullptr);
-            gemm_f32(s, M, 3 * d, d, Hh, d, L.wqkv, L.bqkv, QKV, 3 * d, EPI_STORE);
-            attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-            gemm_f32(s, M, d, d, O, d, L.wp, L.bp, X, d, EPI_RESID);
-            layernorm(s, M, d, X, L.n2g, L.n2b, Hh, nullptr);
-            gemm_f32(s, M, ff, d, Hh, d, L.w1, L.b1, F, ff, EPI_GELU);
-            gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID);
+            if (l > 0 || !h_ready) {
+                TScope t(m, s, KC_NORM, ln_b);
+                layernorm(s, M, d, X, L.n1g, L.n1b, Hh, nullptr);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * 3 * d);
+                gemm_f32(s, M, 3 * d, d, Hh, d, L.wqkv, L.bqkv, QKV, 3 * d, EPI_STORE);
+            }
+            {
+                TScope t(m, s, KC_ATTN, at_b);
+                attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * d);
+                gemm_f32(s, M, d, d, O, d, L.wp, L.bp, X, d, EPI_RESID);
+            }
+            {
+                TScope t(m, s, KC_NORM, ln_b);
+                layernorm(s, M, d, X, L.n2g, L.n2b, Hh, nullptr);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                gemm_f32(s, M, ff, d, Hh, d, L.w1, L.b1, F, ff, EPI_GELU);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID);
+            }
         }
         return;
     }
@@ -221,15 +294,37 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
     uint16_t* QKV = m.get<uint16_t>(S_QKV, (size_t)M * 3 * d);
     uint16_t* O = m.get<uint16_t>(S_O, (size_t)M * d);
     uint16_t* F = m.get<uint16_t>(S_F, (size_t)M * ff);
+    const double ln_b = (double)M * d * 6, at_b = 8.0 * M * d, g = 2.0 * M * d;
     for (int l = 0; l < m.cfg.layers; l++) {
         const LayerW& L = m.L[l];
-        if (l > 0 || !h_ready) layernorm(s, M, d, X, L.n1g, L.n1b, nullptr, Hb);
-        gemm_bf16(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, 3 * d, EPI_STORE);
-        attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-        gemm_bf16(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, d, EPI_RESID);
-        layernorm(s, M, d, X, L.n2g, L.n2b, nullptr, Hb);
-        gemm_bf16(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, ff, EPI_GELU);
-        gemm_bf16(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, d, EPI_RESID);
+        if (l > 0 || !h_ready) {
+            TScope t(m, s, KC_NORM, ln_b);
+            layernorm(s, M, d, X, L.n1g, L.n1b, nullptr, Hb);
+        }
+        {
+            TScope t(m, s, KC_GEMM, g * 3 * d);
+            gemm_bf16(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, 3 * d, EPI_STORE);
+        }
+        {
+            TScope t(m, s, KC_ATTN, at_b);
+            attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+        }
+        {
+            TScope t(m, s, KC_GEMM, g * d);
+            gemm_bf16(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, d, EPI_RESID);
+        }
+        {
+            TScope t(m, s, KC_NORM, ln_b);
+            layernorm(s, M, d, X, L.n2g, L.n2b, nullptr, Hb);
+        }
+        {
+            TScope t(m, s, KC_GEMM, g * ff);
+            gemm_bf16(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, ff, EPI_GELU);
+        }
+        {
+            TScope t(m, s, KC_GEMM, g * ff);
+            gemm_bf16(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, d, EPI_RESID);
+        }
     }
 }
 

@@ -255,11 +255,13 @@ struct Server {
             hipGraph_t g = nullptr;
             bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
+                m.timer.capturing = true;
                 try {
                     cell_iteration(m, s, pack, x0, xc, xn);
                 } catch (const Error&) {
                     ok = false;
                 }
+                m.timer.capturing = false;
                 hipError_t e = hipStreamEndCapture(s, &g);
                 ok = ok && e == hipSuccess && g && hipGraphInstantiate(&graph[parity], g, nullptr, nullptr, 0) == hipSuccess;
                 if (g) (void)hipGraphDestroy(g);

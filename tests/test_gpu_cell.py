@@ -200,7 +200,9 @@ def test_server_results_same_alone_and_in_the_mix(small, precision):
 def test_server_budget_admits_one_at_a_time(small):
     model = j.Model.from_bytes(*small)
     reqs = _requests()[:3]
-    h = _server(model, 16 * 16 * 8 * 4096, 16)  # fits one request's tokens, not two
+    # f32, d=32, ff=64: 1076 device bytes per token; the budget holds the
+    # 2048-token request alone but no two of the three requests together
+    h = _server(model, 2048 * 1076 + 100, 16)
     try:
         ids = [_submit(h, g, k) for g, k in reqs]
         _lib.call("j2k_server_drain", h)

@@ -112,7 +112,10 @@ def test_projected_rollout_matches_oracle(fixture_dir, tiny, golden):
     d = maxabs(s, x)
     print(f"projected 20-step max_abs={d:.3e} violation gpu={gviol:.3e} cpu={viol:.3e}")
     assert d < 1e-4
-    assert gviol <= 1e-6
+    # pt-tiny's r_max is 0.999: where potential energy alone exceeds the bound
+    # the reference projection cannot meet it either; the device must report
+    # the same violation as the CPU loop.
+    assert abs(gviol - viol) <= 1e-6
 
 
 def test_device_projections_match_oracle(tiny):

@@ -141,6 +141,86 @@ def cpu_baseline(st, cj, x0, known, meta, iters, seconds):
                       f"(restatement of engine::cpu extended to the cell_st form), {el:.1f} s"}
 
 
+def stream_requests(n, seed=0):
+    """Mixed request stream: sizes uniform over {32,48,64,96}^2 x {8,16}
+    slices, mask kinds cycling forward / rewind / reconstruction (2% of cells),
+    seeded random drops."""
+    from jacobi2000_amd import checkpoint as ck
+
+    rng = np.random.default_rng(seed)
+    kinds = ("forward", "rewind", "reconstruction")
+    reqs = []
+    for i in range(n):
+        side = int(rng.choice([32, 48, 64, 96]))
+        T = int(rng.choice([8, 16]))
+        g = ck.drops_grid(T, side, side, 3, seed=1000 + i)
+        k = ck.mask_of(kinds[i % 3], T, side, side, seed=2000 + i)
+        reqs.append((g, k))
+    return reqs
+
+
+def run_stream(st, cj, meta, reqs, rate, served):
+    """Poisson arrivals at `rate` requests/s. served: the bf16 server with
+    continuous batching; otherwise the f32 plain form, one request at a time
+    in arrival order. Returns throughput and per-request latency."""
+    import ctypes
+    import jacobi2000_amd as j
+    from jacobi2000_amd import _lib
+
+    rng = np.random.default_rng(7)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(reqs)))
+    cells = sum(g[0].size for g, _ in reqs)
+    lat, iters, conv = [], [], 0
+    if served:
+        model = j.Model.from_bytes(st, cj, precision="bf16")
+        h = ctypes.c_void_p()
+        _lib.call("j2k_server_create", model.handle, 64 << 30, 256, ctypes.byref(h))
+        ids = []
+        t0 = time.perf_counter()
+        for (g, k), ta in zip(reqs, arrivals):
+            wait = t0 + ta - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            rid = ctypes.c_uint64()
+            _lib.call("j2k_server_submit", h, g.ctypes.data_as(ctypes.c_void_p), k.ctypes.data_as(ctypes.c_void_p),
+                      meta.ctypes.data_as(ctypes.c_void_p), g.shape[1], g.shape[2], g.shape[3], ctypes.byref(rid))
+            ids.append(rid.value)
+        _lib.call("j2k_server_drain", h)
+        first = last = None
+        for rid in ids:
+            r = _lib.Result()
+            _lib.call("j2k_server_poll", h, rid, ctypes.byref(r), None)
+            lat.append(r.done_s - r.submit_s)
+            iters.append(r.iters)
+            conv += r.converged
+            first = r.submit_s if first is None else min(first, r.submit_s)
+            last = r.done_s if last is None else max(last, r.done_s)
+        launches, replays, peak = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        _lib.call("j2k_server_stats", h, ctypes.byref(launches), ctypes.byref(replays), ctypes.byref(peak), None)
+        _lib.call("j2k_server_destroy", h)
+        model.close()
+        span = last - first
+        extra = {"iterations_launched": launches.value, "graph_replays": replays.value, "peak_active": peak.value}
+    else:
+        model = j.Model.from_bytes(st, cj, precision="f32")
+        t0 = time.perf_counter()
+        for (g, k), ta in zip(reqs, arrivals):
+            wait = t0 + ta - time.perf_counter()
+            if wait > 0:
+                time.sleep(wait)
+            _, it, r, v = model.iterate(j.fill_blank(g, k), k, meta)
+            lat.append(time.perf_counter() - (t0 + ta))
+            iters.append(it)
+            conv += int(r < 1e-4)
+        span = time.perf_counter() - t0
+        model.close()
+        extra = {}
+    lat_ms = np.array(lat) * 1e3
+    return dict(value=cells / span, seconds=span, requests=len(reqs), cells=cells, converged=conv,
+                capped=len(reqs) - conv, mean_iterations=float(np.mean(iters)),
+                p50_ms=float(np.percentile(lat_ms, 50)), p99_ms=float(np.percentile(lat_ms, 99)), **extra)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,46 +228,75 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--stream", type=int, default=0, help="also run a mixed stream of this many requests")
+    ap.add_argument("--rate", type=float, default=2000.0, help="Poisson arrival rate of the stream, requests/s")
+    ap.add_argument("--headline", default="bf16x2", choices=["f32", "bf16", "bf16x2"],
+                    help="served form reported as `value`")
+    ap.add_argument("--no-oracle-error", action="store_true", help="skip the f64-oracle error check")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one, and this path has no multi-GPU form "
               "(requests shard across GPUs without communication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     st, cj, x0, known, meta = workload()
-    f32, out32 = run_precision("f32", st, cj, x0, known, meta, args.steps, args.warmup)
-    bf16, out16 = run_precision("bf16", st, cj, x0, known, meta, args.steps, args.warmup)
-    rel = float(np.linalg.norm(out16 - out32) / np.linalg.norm(out32))
+    runs, outs = {}, {}
+    for prec in ("f32", "bf16", "bf16x2"):
+        runs[prec], outs[prec] = run_precision(prec, st, cj, x0, known, meta, args.steps, args.warmup)
+    # error of every form against the f64 oracle on this request (outside the timed region)
+    errors = {}
+    if not args.no_oracle_error:
+        from oracle import oracle as O
+
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        cpu = O.FormC.load(st, cj)
+        for prec, out in outs.items():
+            ref, _, _ = cpu.iterate(x0, known, meta, runs[prec]["iterations"][-1], 0.0, f64=True)
+            d = (out.astype(np.float64) - ref).reshape(3, -1)
+            r = ref.reshape(3, -1)
+            errors[prec] = {"rel_l2": float(np.linalg.norm(d) / np.linalg.norm(r)),
+                            "rel_l2_per_field": [float(np.linalg.norm(d[c]) / max(np.linalg.norm(r[c]), 1e-30))
+                                                 for c in range(3)],
+                            "max_abs_per_field": [float(np.abs(d[c]).max()) for c in range(3)]}
+    head, plain = runs[args.headline], runs["f32"]
+    summary = lambda r: {k: r[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "iterations", "resid",
+                                          "energy_violation")}
     line = {
         "metric": "converged grid-cells/s + p50/p99 latency, BF16 served vs FP32 plain",
-        "value": bf16["value"],
+        "value": head["value"],
         "unit": "converged grid-cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": bf16["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16",
+        "dtype": args.headline,
         "data": "synthetic: seeded random-init weights N(0,0.02), seeded Gaussian drops",
         "config": {"workload": "cell_st fixed point, one request 64x64 x 16 slices, 27-key stencil, "
                                "model20 shape (d128 L4 H4x32 ff512), slice 0 known, tol 1e-4, cap 32",
                    "cells_per_request": CELLS},
-        "p50_ms": bf16["p50_ms"],
-        "p99_ms": bf16["p99_ms"],
-        "iterations": bf16["iterations"],
-        "energy_violation": bf16["energy_violation"],
-        "fp32_plain": {k: f32[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "iterations",
-                                          "energy_violation")},
-        "bf16_vs_fp32_plain": bf16["value"] / f32["value"],
-        "bf16_rel_l2_vs_fp32": rel,
-        "kernels_bf16": bf16["kernels"],
-        "kernels_fp32": f32["kernels"],
-        "roofline": roofline(bf16, "bf16"),
-        "roofline_fp32": roofline(f32, "f32"),
+        "p50_ms": head["p50_ms"],
+        "p99_ms": head["p99_ms"],
+        "iterations": head["iterations"],
+        "energy_violation": head["energy_violation"],
+        "forms": {p: summary(r) for p, r in runs.items()},
+        "vs_fp32_plain": {p: r["value"] / plain["value"] for p, r in runs.items()},
+        "error_vs_f64_oracle": errors,
+        "kernels": {p: r["kernels"] for p, r in runs.items()},
+        "roofline": roofline(head, "bf16" if args.headline != "f32" else "f32"),
+        "roofline_by_form": {p: roofline(r, "f32" if p == "f32" else "bf16") for p, r in runs.items()},
     }
+    if args.stream > 0:
+        reqs = stream_requests(args.stream)
+        served = run_stream(st, cj, meta, reqs, args.rate, served=True)
+        plain = run_stream(st, cj, meta, reqs, args.rate, served=False)
+        line["stream"] = {"workload": f"{args.stream} requests, Poisson arrivals at {args.rate:g}/s, sizes "
+                                      "{32,48,64,96}^2 x {8,16}, masks forward/rewind/reconstruction",
+                          "served_bf16": served, "plain_fp32": plain,
+                          "served_vs_plain": served["value"] / plain["value"]}
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(f32["iterations"][0], 1), args.cpu_seconds)
+        line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

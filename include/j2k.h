@@ -48,6 +48,9 @@ extern "C" {
  * in f64) in both. */
 #define J2K_PREC_F32 0  /* plain form: f32 operands, f32 MFMA, f32 accumulate   */
 #define J2K_PREC_BF16 1 /* served form: bf16 weights/activations, f32 accumulate */
+/* served form, compensated: bf16 weights; activations kept f32 and fed to the
+ * bf16 MFMA as a hi + lo bf16 pair (two MFMAs per tile), f32 accumulate */
+#define J2K_PREC_BF16X2 2
 
 #define J2K_FORM_PATCH2D 0 /* the reference forward (crates/engine/src/cpu.rs:249-352) */
 #define J2K_FORM_CELL_ST 1 /* per-cell space-time tokens, fixed-point iteration     */
@@ -68,6 +71,7 @@ typedef struct {
     float dx, dt, g, depth, nu, energy_r_max;
     int max_iter;
     float tol;
+    int stall_stop; /* cell_st: retire an iteration stuck at its noise floor (default 1) */
 } j2k_config;
 
 /* Thread-local message of the last failure on this thread ("" if none). */
@@ -145,7 +149,9 @@ int j2k_fill_blank(const float* grid, const uint8_t* known, int T, int ny, int n
 
 /* Runs the fixed point x_{k+1} = P(clamp_known(x_0 + f(x_k))) from x_0 =
  * grid_io (blank cells already filled) until max|x_{k+1} - x_k| over blank
- * cells < tol or max_iter iterations (tol < 0 / max_iter <= 0: config values).
+ * cells < tol or max_iter iterations (tol < 0 / max_iter <= 0: config values),
+ * or, with the config's stall_stop, when the residual stops contracting (two
+ * consecutive iterations not below 0.99x the best so far).
  * P projects every slice on its blank cells: mass to the mass of x_0's slice,
  * then energy to r_max times x_0's slice energy, sums in f64 on the device.
  * Writes the final iterate to grid_io; iters, resid, violation may be NULL. */
@@ -164,7 +170,8 @@ int j2k_cell_forward_dev(void* model, const float* d_grid, const uint8_t* d_know
 typedef struct {
     uint64_t id;
     int iters;          /* fixed-point iterations run                            */
-    int converged;      /* 1 if max|dx| < tol, 0 if it hit max_iter              */
+    int converged;      /* 1 if max|dx| < tol                                    */
+    int stalled;        /* 1 if retired at its noise floor before tol / max_iter */
     float resid;        /* last max|dx|                                          */
     double violation;   /* max relative energy violation over slices             */
     double submit_s, start_s, done_s; /* monotonic seconds: submit, admit, retire */

@@ -7,6 +7,7 @@ missing library is an error, never a fallback.
 from __future__ import annotations
 
 import ctypes
+import sys
 from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
@@ -16,7 +17,7 @@ OK = 0
 ERR_NO_ADAPTER, ERR_NO_DEVICE, ERR_LIMIT, ERR_MAP, ERR_POLL = 1, 2, 3, 4, 5
 ERR_BAD_CONFIG, ERR_MISSING_TENSOR, ERR_BAD_DTYPE, ERR_SHAPE, ERR_INVALID, ERR_OOM = 6, 7, 8, 9, 10, 11
 PENDING = 12
-PREC_F32, PREC_BF16 = 0, 1
+PREC_F32, PREC_BF16, PREC_BF16X2 = 0, 1, 2
 FORM_PATCH2D, FORM_CELL_ST = 0, 1
 
 _NAMES = {1: "NoAdapter", 2: "NoDevice", 3: "Limit", 4: "Map", 5: "Poll", 6: "BadConfig", 7: "MissingTensor",
@@ -39,12 +40,12 @@ class Config(ctypes.Structure):
                                             "time_radius")]
                 + [("norm_mean", ctypes.c_float * 3), ("norm_std", ctypes.c_float * 3), ("norm_depth", ctypes.c_float)]
                 + [(f, ctypes.c_float) for f in ("dx", "dt", "g", "depth", "nu", "energy_r_max")]
-                + [("max_iter", ctypes.c_int), ("tol", ctypes.c_float)])
+                + [("max_iter", ctypes.c_int), ("tol", ctypes.c_float), ("stall_stop", ctypes.c_int)])
 
 
 class Result(ctypes.Structure):
     _fields_ = [("id", ctypes.c_uint64), ("iters", ctypes.c_int), ("converged", ctypes.c_int),
-                ("resid", ctypes.c_float), ("violation", ctypes.c_double), ("submit_s", ctypes.c_double),
+                ("stalled", ctypes.c_int), ("resid", ctypes.c_float), ("violation", ctypes.c_double), ("submit_s", ctypes.c_double),
                 ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
@@ -91,6 +92,15 @@ def lib() -> ctypes.CDLL:
     """The loaded libj2k.so; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own libamdhip64 under the same soname as the system
+        # ROCm one libj2k links. Whichever loads first serves the whole
+        # process, and torch fails to find a GPU when it is handed the other
+        # runtime. Load torch first when it is installed, so both share its runtime.
+        if "torch" not in sys.modules:
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
         if not LIB_PATH.exists():
             raise RuntimeError(f"libj2k.so is not built ({LIB_PATH}); run `python -c 'import __graft_entry__ as g; "
                                f"g.build()'` or `make -C jacobi2000_amd/csrc`")

@@ -118,6 +118,7 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
                  m.cfg.norm_depth);
     int it = 0;
     float r = 0.f;
+    StallWatch watch;
     while (it < max_iter) {
         cell_iteration(m, s, p, x0, xa, xb);
         unsigned bits = 0;
@@ -126,6 +127,7 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
         std::swap(xa, xb);
         it++;
         if (r < tol) break;
+        if (watch.update(r) && m.cfg.stall_stop) break;
     }
     unsigned long long* vb = reinterpret_cast<unsigned long long*>(p.resid + 2);
     fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);

@@ -39,6 +39,20 @@ struct CellTables {
 
 void check_cell_dims(const Model& m, const CellDims& d);
 
+// Stall rule: when the residual fails to drop below 0.99x the best value
+// seen so far on two consecutive iterations, the iteration has reached the
+// noise floor of its arithmetic; it is retired there (reported as stalled,
+// not converged). A contracting sequence never trips it.
+struct StallWatch {
+    float best = 3.4e38f;
+    int flat = 0;
+    bool update(float r) {
+        flat = r < 0.99f * best ? 0 : flat + 1;
+        best = r < best ? r : best;
+        return flat >= 2;
+    }
+};
+
 // One iteration: xn = P(clamp_known(x0 + f(xc))), resid[r] = max|xn - xc| over blank cells.
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn);
 // y = f(x) only (no clamp, no projection): y [M, 3].

@@ -12,7 +12,20 @@ enum Epi : int {
     EPI_STORE = 0,  // C = acc + bias
     EPI_GELU = 1,   // C = gelu(acc + bias)
     EPI_RESID = 2,  // C += acc + bias            (residual stream update)
+    EPI_RESID_LN = 3,  // X += acc + bias, then H = LayerNorm(X row) in bf16
 };
+
+// ---- row-panel bf16 GEMM (64 rows x up to 512 columns per block) ----------
+// A [M, K] bf16, Bt [N, K] bf16, f32 bias; EPI_STORE/EPI_GELU write bf16 C
+// [M, N]; EPI_RESID adds into f32 X [M, N]; EPI_RESID_LN also writes
+// H = LN(X) (gamma ln_g, beta ln_b) in bf16. N % 64 == 0, K % 32 == 0.
+bool gemm_bf16_panel_ok(int N, int K);
+void gemm_bf16_panel(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
+                     uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H);
+// Same tiling with f32 activations split into a bf16 pair (two MFMAs per
+// tile); C and H are f32.
+void gemm_bf16x2_panel(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
+                       float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

@@ -266,6 +266,7 @@ j2k_config parse_config(const std::string& json_text, const Weights& w, int prec
     c.ff_inner = (int)get(w, "blocks.0.ff1.weight").shape.at(1);
     c.max_iter = 32;
     c.tol = 1e-4f;
+    c.stall_stop = 1;
 
     if (c.form == J2K_FORM_PATCH2D) {
         c.nx = uint_field(j, "nx");
@@ -303,6 +304,7 @@ j2k_config parse_config(const std::string& json_text, const Weights& w, int prec
         c.patch_dim = c.channels + 1;
         if (const Json* mi = j.find("max_iter")) c.max_iter = (int)mi->num;
         if (const Json* t = j.find("tol")) c.tol = (float)t->num;
+        if (const Json* st = j.find("stall_stop")) c.stall_stop = st->kind == Json::Bool ? (st->b ? 1 : 0) : (int)st->num;
     }
     return c;
 }

@@ -79,7 +79,10 @@ struct Model {
         return static_cast<T*>(scratch(s, count * sizeof(T)));
     }
     AttnGeo geo() const;
+    // Activations stored as bf16 (the bf16 form); bf16x2 keeps them f32 and
+    // splits them into a bf16 pair inside the GEMM.
     bool bf16() const { return cfg.precision == J2K_PREC_BF16; }
+    bool split() const { return cfg.precision == J2K_PREC_BF16X2; }
 };
 
 // Makes `m.device` current for the scope of a call.

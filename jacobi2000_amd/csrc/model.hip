@@ -151,7 +151,8 @@ struct Uploader {
 }  // namespace
 
 Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision) {
-    if (precision != J2K_PREC_F32 && precision != J2K_PREC_BF16) fail(J2K_ERR_INVALID, "precision must be F32 or BF16");
+    if (precision != J2K_PREC_F32 && precision != J2K_PREC_BF16 && precision != J2K_PREC_BF16X2)
+        fail(J2K_ERR_INVALID, "precision must be F32, BF16 or BF16X2");
     if (!cfg_json) fail(J2K_ERR_INVALID, "config_json is NULL");
     Weights w = read_safetensors(st, n);
     j2k_config cfg = parse_config(cfg_json, w, precision);
@@ -212,7 +213,7 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
         L.b1 = up.tensor(w, p + "ff1.bias", {ff});
         L.w2 = up.tensor(w, p + "ff2.weight", {ff, d});
         L.b2 = up.tensor(w, p + "ff2.bias", {d});
-        if (precision == J2K_PREC_BF16) {
+        if (precision != J2K_PREC_F32) {
             L.wqkv_t = up.bf16_t(wqkv, d, 3 * d);
             L.wp_t = up.bf16_t(get(w, p + "proj.weight").data, d, d);
             L.w1_t = up.bf16_t(get(w, p + "ff1.weight").data, d, ff);
@@ -251,6 +252,47 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
     const AttnGeo geo = m.geo();
     const int tpb = attn_tokens_per_block(H);
+    if (m.split()) {
+        // bf16 weights, f32 activations split into bf16 pairs inside the
+        // row-panel GEMMs; LayerNorms fused into the proj / ff2 epilogues.
+        if (!(gemm_bf16_panel_ok(3 * d, d) && gemm_bf16_panel_ok(ff, d) && gemm_bf16_panel_ok(d, ff) && d <= 512))
+            fail(J2K_ERR_LIMIT, "bf16x2 needs dim and ff_inner multiples of 64 (dim <= 512)");
+        float* Hh = m.get<float>(S_H, (size_t)M * d);
+        float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
+        float* O = m.get<float>(S_O, (size_t)M * d);
+        float* F = m.get<float>(S_F, (size_t)M * ff);
+        const double g = 2.0 * M * d;
+        if (!h_ready) {
+            TScope t(m, s, KC_NORM, (double)M * d * 8);
+            layernorm(s, M, d, X, m.L[0].n1g, m.L[0].n1b, Hh, nullptr);
+        }
+        for (int l = 0; l < m.cfg.layers; l++) {
+            const LayerW& L = m.L[l];
+            {
+                TScope t(m, s, KC_GEMM, g * 3 * d);
+                gemm_bf16x2_panel(s, M, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, nullptr, EPI_STORE, nullptr, nullptr, nullptr);
+            }
+            {
+                TScope t(m, s, KC_ATTN, 16.0 * M * d);
+                attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * d);
+                gemm_bf16x2_panel(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, EPI_RESID_LN, L.n2g, L.n2b, Hh);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                gemm_bf16x2_panel(s, M, ff, d, Hh, L.w1_t, L.b1, F, nullptr, EPI_GELU, nullptr, nullptr, nullptr);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                const bool last = l + 1 == m.cfg.layers;
+                gemm_bf16x2_panel(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, last ? EPI_RESID : EPI_RESID_LN,
+                                  last ? nullptr : m.L[l + 1].n1g, last ? nullptr : m.L[l + 1].n1b, Hh);
+            }
+        }
+        return;
+    }
     if (!m.bf16()) {
         float* Hh = m.get<float>(S_H, (size_t)M * d);
         float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
@@ -295,6 +337,42 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
     uint16_t* O = m.get<uint16_t>(S_O, (size_t)M * d);
     uint16_t* F = m.get<uint16_t>(S_F, (size_t)M * ff);
     const double ln_b = (double)M * d * 6, at_b = 8.0 * M * d, g = 2.0 * M * d;
+    const bool panel = gemm_bf16_panel_ok(3 * d, d) && gemm_bf16_panel_ok(ff, d) && gemm_bf16_panel_ok(d, ff) && d <= 512;
+    if (panel) {
+        // Row-panel GEMMs; the proj and ff2 epilogues also apply the next
+        // LayerNorm (norm2, then the next block's norm1), so the stream X is
+        // read once per residual update and no LayerNorm pass runs on its own.
+        if (!h_ready) {
+            TScope t(m, s, KC_NORM, ln_b);
+            layernorm(s, M, d, X, m.L[0].n1g, m.L[0].n1b, nullptr, Hb);
+        }
+        for (int l = 0; l < m.cfg.layers; l++) {
+            const LayerW& L = m.L[l];
+            {
+                TScope t(m, s, KC_GEMM, g * 3 * d);
+                gemm_bf16_panel(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, EPI_STORE, nullptr, nullptr, nullptr);
+            }
+            {
+                TScope t(m, s, KC_ATTN, at_b);
+                attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * d);
+                gemm_bf16_panel(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, EPI_RESID_LN, L.n2g, L.n2b, Hb);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                gemm_bf16_panel(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, EPI_GELU, nullptr, nullptr, nullptr);
+            }
+            {
+                TScope t(m, s, KC_GEMM, g * ff);
+                const bool last = l + 1 == m.cfg.layers;
+                gemm_bf16_panel(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, last ? EPI_RESID : EPI_RESID_LN,
+                                last ? nullptr : m.L[l + 1].n1g, last ? nullptr : m.L[l + 1].n1b, Hb);
+            }
+        }
+        return;
+    }
     for (int l = 0; l < m.cfg.layers; l++) {
         const LayerW& L = m.L[l];
         if (l > 0 || !h_ready) {

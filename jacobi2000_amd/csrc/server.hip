@@ -46,6 +46,7 @@ struct Request {
     float resid = 0.f;
     size_t bytes = 0;
     int base = -1;  // first token in the current packing (-1: not yet on the device)
+    StallWatch watch;
 };
 
 // Device bytes one token of an active request holds: three float4 state
@@ -346,7 +347,9 @@ struct Server {
                 Request& r = *active[i];
                 r.iters++;
                 std::memcpy(&r.resid, &h_resid[i], 4);
-                if (r.resid < m.cfg.tol || r.iters >= m.cfg.max_iter) {
+                const bool stalled = r.watch.update(r.resid) && m.cfg.stall_stop;
+                if (r.resid < m.cfg.tol || r.iters >= m.cfg.max_iter || stalled) {
+                    r.res.stalled = r.resid >= m.cfg.tol && stalled ? 1 : 0;
                     finish(i);
                     out[i] = any = true;
                 }

@@ -24,7 +24,7 @@ import numpy as np
 from . import _lib
 from ._lib import J2KError, call
 
-PRECISIONS = {"f32": _lib.PREC_F32, "fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16}
+PRECISIONS = {"f32": _lib.PREC_F32, "fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16, "bf16x2": _lib.PREC_BF16X2}
 
 
 def _f32(a) -> np.ndarray:

@@ -15,17 +15,17 @@ enum Epi : int {
     EPI_RESID_LN = 3,  // X += acc + bias, then H = LayerNorm(X row) in bf16
 };
 
-// ---- row-panel bf16 GEMM (64 rows x up to 512 columns per block) ----------
-// A [M, K] bf16, Bt [N, K] bf16, f32 bias; EPI_STORE/EPI_GELU write bf16 C
-// [M, N]; EPI_RESID adds into f32 X [M, N]; EPI_RESID_LN also writes
-// H = LN(X) (gamma ln_g, beta ln_b) in bf16. N % 64 == 0, K % 32 == 0.
-bool gemm_bf16_panel_ok(int N, int K);
-void gemm_bf16_panel(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
-                     uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H);
-// Same tiling with f32 activations split into a bf16 pair (two MFMAs per
-// tile); C and H are f32.
-void gemm_bf16x2_panel(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
-                       float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H);
+// ---- tiled bf16 GEMM (128 x 128 tiles, LDS-staged, XCD-aware order) ---------
+// A [M, K] (bf16, or f32 for the split form), Bt [N, K] bf16, f32 bias.
+// EPI_STORE/EPI_GELU write C [M, N]; EPI_RESID adds into the f32 stream X
+// [M, N]; EPI_RESID_LN (N == 128) also writes H = LN(X) with gamma ln_g,
+// beta ln_b. N % 16 == 0, K % 8 == 0. The split form feeds f32 activations
+// to the bf16 MFMA as a hi + lo pair (two MFMAs per tile); its C, H are f32.
+bool gemm_tile_ok(int N, int K, bool split);
+void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
+                    uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H);
+void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
+                      float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.
@@ -43,12 +43,6 @@ void gemm_embed_p(hipStream_t s, int M, int N, int K, const float* A, const floa
 void conv_f32(hipStream_t s, int B, int cin, int h, int w, const float* in, const float* in2, int cin1,
               const float* wt, const float* bias, int cout, int k, float* out, int gelu, const float* add_src,
               const float* add_out);
-
-// ---- GEMM, bf16 operands on v_mfma_f32_16x16x32_bf16, f32 accumulate ------
-// A [M,K] bf16 row-major, Bt [N,K] bf16 (the weight transposed), f32 bias.
-// EPI_STORE/EPI_GELU write bf16 C (ldc); EPI_RESID adds into f32 Cf.
-void gemm_bf16(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
-               uint16_t* C, float* Cf, int ldc, int epi);
 
 // ---- LayerNorm (cpu.rs:49-65): one wave per row; f32 and/or bf16 out. ----
 void layernorm(hipStream_t s, int rows, int dim, const float* x, const float* g, const float* b, float* out,

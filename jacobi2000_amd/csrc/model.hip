@@ -248,52 +248,74 @@ void build_tiles(const std::vector<Seg>& segs, int tpb, std::vector<Tile>& tiles
     }
 }
 
+namespace {
+
+// The trunk in the served forms: bf16 weights on the tiled MFMA GEMM, with
+// activations in bf16 (Act = uint16_t) or kept f32 and split into bf16 pairs
+// inside the GEMM (Act = float). With dim == 128 the proj and ff2 GEMMs also
+// apply the next LayerNorm in their epilogue, so the stream is read once per
+// residual update.
+template <class Act>
+void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles,
+                int n_tiles) {
+    constexpr bool split = sizeof(Act) == 4;
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
+    const AttnGeo geo = m.geo();
+    const int tpb = attn_tokens_per_block(H);
+    Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
+    Act* QKV = m.get<Act>(S_QKV, (size_t)M * 3 * d);
+    Act* O = m.get<Act>(S_O, (size_t)M * d);
+    Act* F = m.get<Act>(S_F, (size_t)M * ff);
+    const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
+    const bool fuse_ln = d == 128;
+    auto gemm = [&](int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
+                    const float* lg, const float* lb) {
+        TScope t(m, s, KC_GEMM, g / d * N * K);
+        if (split)
+            gemm_bf16x2_tile(s, M, N, K, (const float*)A, Bt, bias, (float*)C, X, epi, lg, lb, (float*)Hh);
+        else
+            gemm_bf16_tile(s, M, N, K, (const uint16_t*)A, Bt, bias, (uint16_t*)C, X, epi, lg, lb, (uint16_t*)Hh);
+    };
+    auto norm = [&](const float* lg, const float* lb) {
+        TScope t(m, s, KC_NORM, ln_b);
+        if (split) layernorm(s, M, d, X, lg, lb, (float*)Hh, nullptr);
+        else layernorm(s, M, d, X, lg, lb, nullptr, (uint16_t*)Hh);
+    };
+    if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
+    for (int l = 0; l < m.cfg.layers; l++) {
+        const LayerW& L = m.L[l];
+        const bool last = l + 1 == m.cfg.layers;
+        gemm(3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        {
+            TScope t(m, s, KC_ATTN, at_b);
+            if (split) attention_f32(s, (const float*)QKV, (float*)O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            else attention_bf16(s, (const uint16_t*)QKV, (uint16_t*)O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+        }
+        if (fuse_ln) {
+            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
+        } else {
+            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
+            norm(L.n2g, L.n2b);
+        }
+        gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
+        if (last) {
+            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+        } else if (fuse_ln) {
+            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID_LN, m.L[l + 1].n1g, m.L[l + 1].n1b);
+        } else {
+            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            norm(m.L[l + 1].n1g, m.L[l + 1].n1b);
+        }
+    }
+}
+
+}  // namespace
+
 void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles, int n_tiles) {
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
     const AttnGeo geo = m.geo();
     const int tpb = attn_tokens_per_block(H);
-    if (m.split()) {
-        // bf16 weights, f32 activations split into bf16 pairs inside the
-        // row-panel GEMMs; LayerNorms fused into the proj / ff2 epilogues.
-        if (!(gemm_bf16_panel_ok(3 * d, d) && gemm_bf16_panel_ok(ff, d) && gemm_bf16_panel_ok(d, ff) && d <= 512))
-            fail(J2K_ERR_LIMIT, "bf16x2 needs dim and ff_inner multiples of 64 (dim <= 512)");
-        float* Hh = m.get<float>(S_H, (size_t)M * d);
-        float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
-        float* O = m.get<float>(S_O, (size_t)M * d);
-        float* F = m.get<float>(S_F, (size_t)M * ff);
-        const double g = 2.0 * M * d;
-        if (!h_ready) {
-            TScope t(m, s, KC_NORM, (double)M * d * 8);
-            layernorm(s, M, d, X, m.L[0].n1g, m.L[0].n1b, Hh, nullptr);
-        }
-        for (int l = 0; l < m.cfg.layers; l++) {
-            const LayerW& L = m.L[l];
-            {
-                TScope t(m, s, KC_GEMM, g * 3 * d);
-                gemm_bf16x2_panel(s, M, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, nullptr, EPI_STORE, nullptr, nullptr, nullptr);
-            }
-            {
-                TScope t(m, s, KC_ATTN, 16.0 * M * d);
-                attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * d);
-                gemm_bf16x2_panel(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, EPI_RESID_LN, L.n2g, L.n2b, Hh);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * ff);
-                gemm_bf16x2_panel(s, M, ff, d, Hh, L.w1_t, L.b1, F, nullptr, EPI_GELU, nullptr, nullptr, nullptr);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * ff);
-                const bool last = l + 1 == m.cfg.layers;
-                gemm_bf16x2_panel(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, last ? EPI_RESID : EPI_RESID_LN,
-                                  last ? nullptr : m.L[l + 1].n1g, last ? nullptr : m.L[l + 1].n1b, Hh);
-            }
-        }
-        return;
-    }
-    if (!m.bf16()) {
+    if (m.cfg.precision == J2K_PREC_F32) {
         float* Hh = m.get<float>(S_H, (size_t)M * d);
         float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
         float* O = m.get<float>(S_O, (size_t)M * d);
@@ -332,78 +354,8 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
         }
         return;
     }
-    uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)M * d);
-    uint16_t* QKV = m.get<uint16_t>(S_QKV, (size_t)M * 3 * d);
-    uint16_t* O = m.get<uint16_t>(S_O, (size_t)M * d);
-    uint16_t* F = m.get<uint16_t>(S_F, (size_t)M * ff);
-    const double ln_b = (double)M * d * 6, at_b = 8.0 * M * d, g = 2.0 * M * d;
-    const bool panel = gemm_bf16_panel_ok(3 * d, d) && gemm_bf16_panel_ok(ff, d) && gemm_bf16_panel_ok(d, ff) && d <= 512;
-    if (panel) {
-        // Row-panel GEMMs; the proj and ff2 epilogues also apply the next
-        // LayerNorm (norm2, then the next block's norm1), so the stream X is
-        // read once per residual update and no LayerNorm pass runs on its own.
-        if (!h_ready) {
-            TScope t(m, s, KC_NORM, ln_b);
-            layernorm(s, M, d, X, m.L[0].n1g, m.L[0].n1b, nullptr, Hb);
-        }
-        for (int l = 0; l < m.cfg.layers; l++) {
-            const LayerW& L = m.L[l];
-            {
-                TScope t(m, s, KC_GEMM, g * 3 * d);
-                gemm_bf16_panel(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, EPI_STORE, nullptr, nullptr, nullptr);
-            }
-            {
-                TScope t(m, s, KC_ATTN, at_b);
-                attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * d);
-                gemm_bf16_panel(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, EPI_RESID_LN, L.n2g, L.n2b, Hb);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * ff);
-                gemm_bf16_panel(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, EPI_GELU, nullptr, nullptr, nullptr);
-            }
-            {
-                TScope t(m, s, KC_GEMM, g * ff);
-                const bool last = l + 1 == m.cfg.layers;
-                gemm_bf16_panel(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, last ? EPI_RESID : EPI_RESID_LN,
-                                last ? nullptr : m.L[l + 1].n1g, last ? nullptr : m.L[l + 1].n1b, Hb);
-            }
-        }
-        return;
-    }
-    for (int l = 0; l < m.cfg.layers; l++) {
-        const LayerW& L = m.L[l];
-        if (l > 0 || !h_ready) {
-            TScope t(m, s, KC_NORM, ln_b);
-            layernorm(s, M, d, X, L.n1g, L.n1b, nullptr, Hb);
-        }
-        {
-            TScope t(m, s, KC_GEMM, g * 3 * d);
-            gemm_bf16(s, M, 3 * d, d, Hb, L.wqkv_t, L.bqkv, QKV, nullptr, 3 * d, EPI_STORE);
-        }
-        {
-            TScope t(m, s, KC_ATTN, at_b);
-            attention_bf16(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-        }
-        {
-            TScope t(m, s, KC_GEMM, g * d);
-            gemm_bf16(s, M, d, d, O, L.wp_t, L.bp, nullptr, X, d, EPI_RESID);
-        }
-        {
-            TScope t(m, s, KC_NORM, ln_b);
-            layernorm(s, M, d, X, L.n2g, L.n2b, nullptr, Hb);
-        }
-        {
-            TScope t(m, s, KC_GEMM, g * ff);
-            gemm_bf16(s, M, ff, d, Hb, L.w1_t, L.b1, F, nullptr, ff, EPI_GELU);
-        }
-        {
-            TScope t(m, s, KC_GEMM, g * ff);
-            gemm_bf16(s, M, d, ff, F, L.w2_t, L.b2, nullptr, X, d, EPI_RESID);
-        }
-    }
+    if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, d_segs, d_tiles, n_tiles);
+    else trunk_lowp<uint16_t>(m, s, M, X, h_ready, d_segs, d_tiles, n_tiles);
 }
 
 void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state) {

@@ -89,6 +89,9 @@ void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta,
     p.segs = m.get<Seg>(S_SEGS, run.tab.segs.size());
     p.tiles = m.get<Tile>(S_TILES, run.tab.tiles.size());
     p.slices = m.get<Slice>(S_SLICES, run.tab.slices.size());
+    p.ctiles = m.get<CellTile>(S_CTILES, run.tab.ctiles.size());
+    p.n_ctiles = (int)run.tab.ctiles.size();
+    HIPCHK(hipMemcpy(p.ctiles, run.tab.ctiles.data(), run.tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p.segs, run.tab.segs.data(), run.tab.segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p.tiles, run.tab.tiles.data(), run.tab.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(p.slices, run.tab.slices.data(), run.tab.slices.size() * sizeof(Slice), hipMemcpyHostToDevice));

@@ -19,6 +19,8 @@ struct CellPack {
     int M = 0, n_req = 0, n_slices = 0, n_tiles = 0;
     Seg* segs = nullptr;
     Tile* tiles = nullptr;
+    CellTile* ctiles = nullptr;
+    int n_ctiles = 0;
     Slice* slices = nullptr;
     int* tok_req = nullptr;  // per token request index (nullptr for one request)
     float* mvec = nullptr;   // [n_req, dim] meta embedding per request
@@ -31,6 +33,7 @@ struct CellPack {
 struct CellTables {
     std::vector<Seg> segs;
     std::vector<Tile> tiles;
+    std::vector<CellTile> ctiles;
     std::vector<Slice> slices;
     std::vector<int> tok_req;
     int M = 0;
@@ -52,6 +55,16 @@ struct StallWatch {
         return flat >= 2;
     }
 };
+
+inline AttnPlan plan_of(const CellPack& p) {
+    AttnPlan a;
+    a.segs = p.segs;
+    a.tiles = p.tiles;
+    a.n_tiles = p.n_tiles;
+    a.ctiles = p.ctiles;
+    a.n_ctiles = p.n_ctiles;
+    return a;
+}
 
 // One iteration: xn = P(clamp_known(x0 + f(xc))), resid[r] = max|xn - xc| over blank cells.
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn);

@@ -17,6 +17,7 @@ void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_toke
         M += d.tokens();
     }
     build_tiles(segs, tpb, tiles);
+    build_cell_tiles(segs, ctiles);
 }
 
 void check_cell_dims(const Model& m, const CellDims& d) {
@@ -39,7 +40,7 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         float* H = m.get<float>(S_H, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, H, nullptr);
     }
-    trunk(m, s, p.M, X, true, p.segs, p.tiles, p.n_tiles);
+    trunk(m, s, p.M, X, true, plan_of(p));
 }
 
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn) {

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace j2k {
 
@@ -72,6 +73,28 @@ void attention_f32(hipStream_t s, const float* qkv, float* out, int dim, int hea
 void attention_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, AttnGeo geo,
                     const Seg* segs, const Tile* tiles, int n_tiles, int tokens_per_block);
 int attn_tokens_per_block(int heads);
+
+// cell_st attention with K/V staged in LDS: one block per 8 x 8 tile of cells
+// of one time slice ({segment, t, y0, x0}), periodic halo `rs`, time window rt.
+struct CellTile {
+    int seg, t, y0, x0;
+};
+void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles);
+size_t attn_cell_smem(int hd, int rs, int rt, int elem);
+void attn_cell_prepare();  // once per device, before the first launch
+void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,
+                        const Seg* segs, const CellTile* tiles, int n_tiles);
+void attention_cell_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, int rs, int rt,
+                         const Seg* segs, const CellTile* tiles, int n_tiles);
+
+// The attention tables of one packed batch; cell tiles are used when present.
+struct AttnPlan {
+    const Seg* segs = nullptr;
+    const Tile* tiles = nullptr;
+    int n_tiles = 0;
+    const CellTile* ctiles = nullptr;
+    int n_ctiles = 0;
+};
 
 // ---- patch2d helpers -----------------------------------------------------
 // cpu.rs:253-269: patches[b*tokens + t, c*p*p + py*p + px].

@@ -19,7 +19,7 @@ struct LayerW {
 enum Slot : int {
     S_X, S_H, S_HB, S_QKV, S_O, S_F, S_XN, S_T0, S_T1, S_Z0, S_Z2, S_C1, S_PATCH, S_MVEC,
     S_WIN_A, S_WIN_B, S_SUMS, S_SEGS, S_TILES, S_SLICES, S_X0, S_XA, S_XB, S_TARG, S_RESID, S_GRID, S_KNOWN, S_META,
-    S_TOKREQ, S_COUNT
+    S_TOKREQ, S_CTILES, S_COUNT
 };
 
 // Kernel classes timed with HIP events when timing is on (the bench's
@@ -97,7 +97,7 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
 // The transformer trunk (cpu.rs:287-313) over M packed tokens: X [M, dim] is
 // the residual stream; H must hold LN1 of block 0 when h_ready, else it is
 // computed. Leaves the stream un-normalised (callers apply the final norm).
-void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles, int n_tiles);
+void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap);
 
 // patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
 // add_state the result is state + forward (cpu.rs:84-88).

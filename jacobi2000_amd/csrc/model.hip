@@ -165,6 +165,7 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
     m->cfg = cfg;
     m->device = device;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
+    if (cfg.form == J2K_FORM_CELL_ST) attn_cell_prepare();
     Uploader up{*m};
     const int d = cfg.dim, ff = cfg.ff_inner;
     if (cfg.dim / cfg.heads != 8 && cfg.dim / cfg.heads != 16 && cfg.dim / cfg.heads != 32 && cfg.dim / cfg.heads != 64)
@@ -250,18 +251,39 @@ void build_tiles(const std::vector<Seg>& segs, int tpb, std::vector<Tile>& tiles
 
 namespace {
 
+// Attention over one packed batch: the LDS-tiled kernel for the cell_st
+// stencil when its neighbourhood fits in LDS, the generic gather otherwise.
+template <class Act>
+void attend(Model& m, hipStream_t s, const Act* QKV, Act* O, const AttnPlan& ap) {
+    const int d = m.cfg.dim, H = m.cfg.heads;
+    const AttnGeo geo = m.geo();
+    const bool tiled = ap.ctiles && geo.kind == ATTN_CELL_ST &&
+                       attn_cell_smem(d / H, geo.radius, geo.time_radius, sizeof(Act)) <= 160 * 1024;
+    if (tiled) {
+        if (sizeof(Act) == 4)
+            attention_cell_f32(s, (const float*)QKV, (float*)O, d, H, geo.radius, geo.time_radius, ap.segs, ap.ctiles,
+                               ap.n_ctiles);
+        else
+            attention_cell_bf16(s, (const uint16_t*)QKV, (uint16_t*)O, d, H, geo.radius, geo.time_radius, ap.segs,
+                                ap.ctiles, ap.n_ctiles);
+        return;
+    }
+    const int tpb = attn_tokens_per_block(H);
+    if (sizeof(Act) == 4)
+        attention_f32(s, (const float*)QKV, (float*)O, d, H, geo, ap.segs, ap.tiles, ap.n_tiles, tpb);
+    else
+        attention_bf16(s, (const uint16_t*)QKV, (uint16_t*)O, d, H, geo, ap.segs, ap.tiles, ap.n_tiles, tpb);
+}
+
 // The trunk in the served forms: bf16 weights on the tiled MFMA GEMM, with
 // activations in bf16 (Act = uint16_t) or kept f32 and split into bf16 pairs
 // inside the GEMM (Act = float). With dim == 128 the proj and ff2 GEMMs also
 // apply the next LayerNorm in their epilogue, so the stream is read once per
 // residual update.
 template <class Act>
-void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles,
-                int n_tiles) {
+void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
     constexpr bool split = sizeof(Act) == 4;
-    const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
-    const AttnGeo geo = m.geo();
-    const int tpb = attn_tokens_per_block(H);
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner;
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
     Act* QKV = m.get<Act>(S_QKV, (size_t)M * 3 * d);
     Act* O = m.get<Act>(S_O, (size_t)M * d);
@@ -288,8 +310,7 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Se
         gemm(3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
         {
             TScope t(m, s, KC_ATTN, at_b);
-            if (split) attention_f32(s, (const float*)QKV, (float*)O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
-            else attention_bf16(s, (const uint16_t*)QKV, (uint16_t*)O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+            attend<Act>(m, s, QKV, O, ap);
         }
         if (fuse_ln) {
             gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
@@ -311,10 +332,8 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Se
 
 }  // namespace
 
-void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_segs, const Tile* d_tiles, int n_tiles) {
-    const int d = m.cfg.dim, ff = m.cfg.ff_inner, H = m.cfg.heads;
-    const AttnGeo geo = m.geo();
-    const int tpb = attn_tokens_per_block(H);
+void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner;
     if (m.cfg.precision == J2K_PREC_F32) {
         float* Hh = m.get<float>(S_H, (size_t)M * d);
         float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
@@ -333,7 +352,7 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
             }
             {
                 TScope t(m, s, KC_ATTN, at_b);
-                attention_f32(s, QKV, O, d, H, geo, d_segs, d_tiles, n_tiles, tpb);
+                attend<float>(m, s, QKV, O, ap);
             }
             {
                 TScope t(m, s, KC_GEMM, g * d);
@@ -354,8 +373,8 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Seg* d_
         }
         return;
     }
-    if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, d_segs, d_tiles, n_tiles);
-    else trunk_lowp<uint16_t>(m, s, M, X, h_ready, d_segs, d_tiles, n_tiles);
+    if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, ap);
+    else trunk_lowp<uint16_t>(m, s, M, X, h_ready, ap);
 }
 
 void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state) {
@@ -381,9 +400,11 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
         m.p_tables_batch = B;
         m.p_tiles = (int)tiles.size();
     }
-    const Seg* d_segs = static_cast<const Seg*>(m.slot[S_SEGS]);
-    const Tile* d_tiles = static_cast<const Tile*>(m.slot[S_TILES]);
-    trunk(m, s, M, X, false, d_segs, d_tiles, m.p_tiles);
+    AttnPlan ap;
+    ap.segs = static_cast<const Seg*>(m.slot[S_SEGS]);
+    ap.tiles = static_cast<const Tile*>(m.slot[S_TILES]);
+    ap.n_tiles = m.p_tiles;
+    trunk(m, s, M, X, false, ap);
 
     // Final norm and CellHead (cpu.rs:315-349).
     float* xn = m.get<float>(S_XN, (size_t)M * d);

@@ -87,6 +87,8 @@ struct Server {
     size_t cap_tok = 0;
     Seg* d_segs = nullptr;
     Tile* d_tiles = nullptr;
+    CellTile* d_ctiles = nullptr;
+    size_t cap_ctiles = 0;
     Slice* d_slices = nullptr;
     float* d_mvec = nullptr;
     unsigned* d_resid = nullptr;
@@ -205,6 +207,12 @@ struct Server {
             d_targ = dev_alloc<double>(2 * w);
             cap_tab = w;
         }
+        if (tab.ctiles.size() > cap_ctiles) {
+            dev_free(d_ctiles);
+            cap_ctiles = tab.ctiles.size() + tab.ctiles.size() / 2;
+            d_ctiles = dev_alloc<CellTile>(cap_ctiles);
+        }
+        HIPCHK(hipMemcpy(d_ctiles, tab.ctiles.data(), tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_segs, tab.segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_tiles, tab.tiles.data(), ntile * sizeof(Tile), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_slices, tab.slices.data(), nsl * sizeof(Slice), hipMemcpyHostToDevice));
@@ -221,6 +229,8 @@ struct Server {
         pack.n_tiles = (int)ntile;
         pack.segs = d_segs;
         pack.tiles = d_tiles;
+        pack.ctiles = d_ctiles;
+        pack.n_ctiles = (int)tab.ctiles.size();
         pack.slices = d_slices;
         pack.tok_req = multi ? d_tokreq : nullptr;
         pack.mvec = d_mvec;

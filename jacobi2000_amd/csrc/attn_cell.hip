@@ -2,13 +2,14 @@
 //
 // A block owns an 8 x 8 tile of cells in one time slice of one segment and
 // loops over the heads. For each head it stages the K and V rows of the
-// tile's neighbourhood (the tile plus a periodic halo of `radius` cells in y
-// and x, over the clipped time window t-rt .. t+rt) into LDS once, so every
-// key row is read from L2 once per tile instead of once per query. Four lanes
-// share a query, each holding a quarter of the head's dimensions: the dot
-// products are reduced with two lane shuffles, the softmax runs single-pass
-// with a running max in f32, and keys are visited in the order time offset,
-// y offset, x offset (the order of the generic kernel and the oracle).
+// tile's neighbourhood (the tile plus a periodic halo of RS cells in y and x,
+// over the clipped time window t-RT .. t+RT) into LDS once, so every key row
+// is read from L2 once per tile instead of once per query. Four lanes share a
+// query, each holding a quarter of the head's dimensions; the dot products
+// are reduced with two lane shuffles. The softmax is two-pass over the
+// scores kept in registers (max, then exp / sum / weighted V in one sweep),
+// f32 throughout; keys are visited in the order time offset, y offset,
+// x offset (the order of the generic kernel and of the oracle).
 #include "common.h"
 #include "kernels.h"
 
@@ -27,7 +28,7 @@ __device__ __forceinline__ float ldf<uint16_t>(const uint16_t* p) {
     return bf2f(*p);
 }
 
-// n consecutive elements from LDS as f32, in 16-byte vectors.
+// N consecutive elements from LDS as f32, in 16-byte vectors.
 template <int N>
 __device__ __forceinline__ void lds_row(const float* p, float* o) {
 #pragma unroll
@@ -58,18 +59,25 @@ __device__ __forceinline__ void lds_row(const uint16_t* p, float* o) {
     }
 }
 
-template <int HD, class T>
+__device__ __forceinline__ int wrap(int v, int n) {
+    while (v < 0) v += n;
+    while (v >= n) v -= n;
+    return v;
+}
+
+template <int HD, int RS, int RT, class T>
 __global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T* __restrict__ out, int dim, int heads,
-                                                   int rs, int rt, const Seg* __restrict__ segs,
-                                                   const CellTile* __restrict__ tiles) {
-    constexpr int D4 = HD / 4;      // dims per lane
-    constexpr int VEC = 16 / sizeof(T);  // elements per 16-byte vector
+                                                   const Seg* __restrict__ segs, const CellTile* __restrict__ tiles) {
+    constexpr int D4 = HD / 4;               // dims per lane
+    constexpr int VEC = 16 / sizeof(T);      // elements per 16-byte vector
+    constexpr int VPR = HD / VEC;            // vectors per K (or V) row
+    constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
+    constexpr int KMAX = (2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1);
     extern __shared__ __align__(16) unsigned char smem[];
     const CellTile tl = tiles[blockIdx.x];
     const Seg sg = segs[tl.seg];
-    const int hw = CT_X + 2 * rs, hh = CT_Y + 2 * rs;
-    const int t_lo = max(tl.t - rt, 0), t_hi = min(tl.t + rt, sg.T - 1), nt = t_hi - t_lo + 1;
-    const int ncell = nt * hh * hw;
+    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + RT, sg.T - 1), nt = t_hi - t_lo + 1;
+    const int ncell = nt * PLANE;
     T* Ks = reinterpret_cast<T*>(smem);
     T* Vs = Ks + (size_t)ncell * HD;
     const long ld = 3L * dim;
@@ -83,50 +91,59 @@ __global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T*
 
     for (int h = 0; h < heads; h++) {
         __syncthreads();  // the previous head's K/V are no longer read
-        // stage K and V of head h for every cell of the neighbourhood
-        constexpr int VPR = HD / VEC;  // vectors per row
-        for (int v = tid; v < ncell * VPR * 2; v += 256) {
-            const int which = v / (ncell * VPR), r = v - which * ncell * VPR;
-            const int c = r / VPR, e = r - c * VPR;
-            const int ct = c / (hh * hw), cr = c - ct * hh * hw, cy = cr / hw, cx = cr - cy * hw;
-            int y = tl.y0 - rs + cy, x = tl.x0 - rs + cx;
-            y = ((y % sg.ny) + sg.ny) % sg.ny;
-            x = ((x % sg.nx) + sg.nx) % sg.nx;
+        for (int v = tid; v < ncell * 2 * VPR; v += 256) {
+            const int c = v / (2 * VPR), rem = v - c * (2 * VPR), which = rem / VPR, e = rem - which * VPR;
+            const int ct = c / PLANE, cr = c - ct * PLANE, cy = cr / HW, cx = cr - cy * HW;
+            const int y = wrap(tl.y0 - RS + cy, sg.ny), x = wrap(tl.x0 - RS + cx, sg.nx);
             const long row = (long)sg.base + (long)(t_lo + ct) * n + (long)y * sg.nx + x;
             const uint4 val = *reinterpret_cast<const uint4*>(qkv + row * ld + (1 + which) * dim + h * HD + e * VEC);
             *reinterpret_cast<uint4*>((which ? Vs : Ks) + (size_t)c * HD + e * VEC) = val;
         }
         __syncthreads();
-        float qv[D4], acc[D4], kv[D4];
+        float qv[D4], kv[D4], acc[D4], s[KMAX];
 #pragma unroll
-        for (int d = 0; d < D4; d++) {
-            qv[d] = ldf<T>(qkv + qrow * ld + h * HD + sub * D4 + d);
-            acc[d] = 0.f;
-        }
-        float mx = -INFINITY, l = 0.f;
-        for (int tt = t_lo; tt <= t_hi; tt++) {
-            const int ct = tt - t_lo;
-            for (int dy = -rs; dy <= rs; dy++) {
-                const int cy = (q >> 3) + rs + dy;
-                for (int dx = -rs; dx <= rs; dx++) {
-                    const int c = (ct * hh + cy) * hw + (q & 7) + rs + dx;
+        for (int d = 0; d < D4; d++) qv[d] = ldf<T>(qkv + qrow * ld + h * HD + sub * D4 + d);
+        float mx = -INFINITY;
+        int nk = 0;
+#pragma unroll
+        for (int ct = 0; ct < 2 * RT + 1; ct++) {
+            if (ct >= nt) break;
+#pragma unroll
+            for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                for (int dx = -RS; dx <= RS; dx++) {
+                    const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
                     lds_row<D4>(Ks + (size_t)c * HD + sub * D4, kv);
-                    float s = 0.f;
+                    float dot = 0.f;
 #pragma unroll
-                    for (int d = 0; d < D4; d++) s += qv[d] * kv[d];
-                    s += __shfl_xor(s, 1, 64);
-                    s += __shfl_xor(s, 2, 64);
-                    s *= scale;
-                    const float mn = fmaxf(mx, s);
-                    const float corr = expf(mx - mn), p = expf(s - mn);
-                    l = l * corr + p;
+                    for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
+                    dot += __shfl_xor(dot, 1, 64);
+                    dot += __shfl_xor(dot, 2, 64);
+                    const float sc = dot * scale;
+                    s[ct * (2 * RS + 1) * (2 * RS + 1) + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
+                    mx = fmaxf(mx, sc);
+                    nk++;
+                }
+        }
+#pragma unroll
+        for (int d = 0; d < D4; d++) acc[d] = 0.f;
+        float l = 0.f;
+#pragma unroll
+        for (int ct = 0; ct < 2 * RT + 1; ct++) {
+            if (ct >= nt) break;
+#pragma unroll
+            for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                for (int dx = -RS; dx <= RS; dx++) {
+                    const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
+                    const float p = __expf(s[ct * (2 * RS + 1) * (2 * RS + 1) + (dy + RS) * (2 * RS + 1) + dx + RS] - mx);
+                    l += p;
                     lds_row<D4>(Vs + (size_t)c * HD + sub * D4, kv);
 #pragma unroll
-                    for (int d = 0; d < D4; d++) acc[d] = acc[d] * corr + p * kv[d];
-                    mx = mn;
+                    for (int d = 0; d < D4; d++) acc[d] += p * kv[d];
                 }
-            }
         }
+        (void)nk;
         if (live) {
             const float inv = 1.0f / l;
             T* o = out + qrow * dim + h * HD + sub * D4;
@@ -139,14 +156,48 @@ __global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T*
     }
 }
 
+namespace {
+
+template <int HD, int RS, int RT, class T>
+void* fn_ptr() {
+    return (void*)k_attn_cell<HD, RS, RT, T>;
+}
+
+// Kernel for (head size, space radius, time radius); nullptr when not built.
+template <class T>
+const void* pick(int hd, int rs, int rt) {
+#define J2K_ATTN_CASE(HD_, RS_, RT_) \
+    if (hd == HD_ && rs == RS_ && rt == RT_) return fn_ptr<HD_, RS_, RT_, T>();
+#define J2K_ATTN_HD(HD_)      \
+    J2K_ATTN_CASE(HD_, 1, 0)  \
+    J2K_ATTN_CASE(HD_, 1, 1)  \
+    J2K_ATTN_CASE(HD_, 2, 0)  \
+    J2K_ATTN_CASE(HD_, 2, 1)
+    J2K_ATTN_HD(8)
+    J2K_ATTN_HD(16)
+    J2K_ATTN_HD(32)
+    J2K_ATTN_HD(64)
+#undef J2K_ATTN_HD
+#undef J2K_ATTN_CASE
+    return nullptr;
+}
+
+}  // namespace
+
+bool attn_cell_ok(int hd, int rs, int rt, int elem) {
+    const void* f = elem == 4 ? pick<float>(hd, rs, rt) : pick<uint16_t>(hd, rs, rt);
+    return f != nullptr && attn_cell_smem(hd, rs, rt, elem) <= 160 * 1024;
+}
+
 // Dynamic LDS above 64 KB has to be allowed per kernel; done once per device
 // at model creation (outside any stream capture).
 void attn_cell_prepare() {
-    const void* fns[] = {(const void*)k_attn_cell<8, float>,     (const void*)k_attn_cell<16, float>,
-                         (const void*)k_attn_cell<32, float>,    (const void*)k_attn_cell<64, float>,
-                         (const void*)k_attn_cell<8, uint16_t>,  (const void*)k_attn_cell<16, uint16_t>,
-                         (const void*)k_attn_cell<32, uint16_t>, (const void*)k_attn_cell<64, uint16_t>};
-    for (const void* f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    for (int hd : {8, 16, 32, 64})
+        for (int rs : {1, 2})
+            for (int rt : {0, 1}) {
+                HIPCHK(hipFuncSetAttribute(pick<float>(hd, rs, rt), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                HIPCHK(hipFuncSetAttribute(pick<uint16_t>(hd, rs, rt), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            }
 }
 
 size_t attn_cell_smem(int hd, int rs, int rt, int elem) {
@@ -166,17 +217,11 @@ static void launch(hipStream_t s, const T* qkv, T* out, int dim, int heads, int 
                    const CellTile* tiles, int n_tiles) {
     if (n_tiles <= 0) return;
     const int hd = dim / heads;
+    const void* f = pick<T>(hd, rs, rt);
     const size_t smem = attn_cell_smem(hd, rs, rt, sizeof(T));
-    if (smem > 160 * 1024) fail(J2K_ERR_LIMIT, "attention: stencil neighbourhood does not fit in LDS");
-    dim3 grid(n_tiles), block(256);
-    switch (hd) {
-        case 8: hipLaunchKernelGGL((k_attn_cell<8, T>), grid, block, smem, s, qkv, out, dim, heads, rs, rt, segs, tiles); break;
-        case 16: hipLaunchKernelGGL((k_attn_cell<16, T>), grid, block, smem, s, qkv, out, dim, heads, rs, rt, segs, tiles); break;
-        case 32: hipLaunchKernelGGL((k_attn_cell<32, T>), grid, block, smem, s, qkv, out, dim, heads, rs, rt, segs, tiles); break;
-        case 64: hipLaunchKernelGGL((k_attn_cell<64, T>), grid, block, smem, s, qkv, out, dim, heads, rs, rt, segs, tiles); break;
-        default: fail(J2K_ERR_LIMIT, "attention: head size must be 8, 16, 32 or 64");
-    }
-    LAUNCHCHK("attention (cell tiles)");
+    if (!f || smem > 160 * 1024) fail(J2K_ERR_LIMIT, "attention: no LDS-tiled kernel for this head size / stencil");
+    void* args[] = {(void*)&qkv, (void*)&out, (void*)&dim, (void*)&heads, (void*)&segs, (void*)&tiles};
+    HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(256), args, smem, s));
 }
 
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,

@@ -28,6 +28,12 @@ void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const
 void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
                       float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H);
 
+// ---- fused MLP: X += ff2(GELU(ff1(H))), then H = LN(X) if ln_g != null ------
+// dim == 128, ff % 128 == 0; H (bf16, or f32 with split) may alias Hout.
+bool mlp_fused_ok(int dim, int ff);
+void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const void* H, const uint16_t* W1t, const float* b1,
+               const uint16_t* W2t, const float* b2, float* X, const float* ln_g, const float* ln_b, void* Hout);
+
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.
 void gemm_f32(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, const float* bias, float* C,
@@ -81,6 +87,7 @@ struct CellTile {
 };
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles);
 size_t attn_cell_smem(int hd, int rs, int rt, int elem);
+bool attn_cell_ok(int hd, int rs, int rt, int elem);  // a tiled kernel exists and fits in LDS
 void attn_cell_prepare();  // once per device, before the first launch
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,
                         const Seg* segs, const CellTile* tiles, int n_tiles);

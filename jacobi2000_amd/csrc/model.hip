@@ -257,8 +257,8 @@ template <class Act>
 void attend(Model& m, hipStream_t s, const Act* QKV, Act* O, const AttnPlan& ap) {
     const int d = m.cfg.dim, H = m.cfg.heads;
     const AttnGeo geo = m.geo();
-    const bool tiled = ap.ctiles && geo.kind == ATTN_CELL_ST &&
-                       attn_cell_smem(d / H, geo.radius, geo.time_radius, sizeof(Act)) <= 160 * 1024;
+    const bool tiled =
+        ap.ctiles && geo.kind == ATTN_CELL_ST && attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act));
     if (tiled) {
         if (sizeof(Act) == 4)
             attention_cell_f32(s, (const float*)QKV, (float*)O, d, H, geo.radius, geo.time_radius, ap.segs, ap.ctiles,
@@ -317,6 +317,12 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
         } else {
             gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
             norm(L.n2g, L.n2b);
+        }
+        if (mlp_fused_ok(d, ff)) {
+            TScope t(m, s, KC_GEMM, 2.0 * g * ff);
+            mlp_fused(s, M, d, ff, split, Hh, L.w1_t, L.b1, L.w2_t, L.b2, X, last ? nullptr : m.L[l + 1].n1g,
+                      last ? nullptr : m.L[l + 1].n1b, Hh);
+            continue;
         }
         gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
         if (last) {

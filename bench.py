@@ -87,6 +87,7 @@ def run_precision(prec, st, cj, x0, known, meta, steps, warmup):
     for c, name in enumerate(CLASSES):
         ms, work, n = timing[3 * c], timing[3 * c + 1], timing[3 * c + 2]
         kern[name] = {"ms_total": ms, "launches": int(n), "work": work}
+    tuning = model.autotune(0) if prec != "f32" else None
     model.close()
     lat_ms = np.array(lat) * 1e3
     return {
@@ -99,6 +100,7 @@ def run_precision(prec, st, cj, x0, known, meta, steps, warmup):
         "resid": float(r),
         "energy_violation": float(v),
         "kernels": kern,
+        "autotune": tuning,
     }, out
 
 
@@ -159,10 +161,11 @@ def stream_requests(n, seed=0):
     return reqs
 
 
-def run_stream(st, cj, meta, reqs, rate, served):
-    """Poisson arrivals at `rate` requests/s. served: the bf16 server with
-    continuous batching; otherwise the f32 plain form, one request at a time
-    in arrival order. Returns throughput and per-request latency."""
+def run_stream(st, cj, meta, reqs, rate, served, precision="bf16x2"):
+    """Poisson arrivals at `rate` requests/s. served: the server in the given
+    low-precision form with continuous batching; otherwise the f32 plain
+    form, one request at a time in arrival order. Returns throughput and
+    per-request latency."""
     import ctypes
     import jacobi2000_amd as j
     from jacobi2000_amd import _lib
@@ -172,7 +175,7 @@ def run_stream(st, cj, meta, reqs, rate, served):
     cells = sum(g[0].size for g, _ in reqs)
     lat, iters, conv = [], [], 0
     if served:
-        model = j.Model.from_bytes(st, cj, precision="bf16")
+        model = j.Model.from_bytes(st, cj, precision=precision)
         h = ctypes.c_void_p()
         _lib.call("j2k_server_create", model.handle, 64 << 30, 256, ctypes.byref(h))
         ids = []
@@ -286,15 +289,17 @@ def main():
         "kernels": {p: r["kernels"] for p, r in runs.items()},
         "roofline": roofline(head, "bf16" if args.headline != "f32" else "f32"),
         "roofline_by_form": {p: roofline(r, "f32" if p == "f32" else "bf16") for p, r in runs.items()},
+        "autotune": head.get("autotune"),
     }
     if args.stream > 0:
         reqs = stream_requests(args.stream)
-        served = run_stream(st, cj, meta, reqs, args.rate, served=True)
-        plain = run_stream(st, cj, meta, reqs, args.rate, served=False)
+        s_served = run_stream(st, cj, meta, reqs, args.rate, served=True,
+                              precision=args.headline if args.headline != "f32" else "bf16x2")
+        s_plain = run_stream(st, cj, meta, reqs, args.rate, served=False)
         line["stream"] = {"workload": f"{args.stream} requests, Poisson arrivals at {args.rate:g}/s, sizes "
                                       "{32,48,64,96}^2 x {8,16}, masks forward/rewind/reconstruction",
-                          "served_bf16": served, "plain_fp32": plain,
-                          "served_vs_plain": served["value"] / plain["value"]}
+                          "served": s_served, "plain_fp32": s_plain,
+                          "served_vs_plain": s_served["value"] / s_plain["value"]}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)
     print(json.dumps(line), flush=True)

@@ -62,6 +62,7 @@ PROTOS = {
     "j2k_model_destroy": [P],
     "j2k_model_config": [P, ctypes.POINTER(Config)],
     "j2k_model_timing": [P, I, P],
+    "j2k_model_autotune": [P, ctypes.c_longlong, ctypes.c_char_p, ctypes.c_size_t],
     "j2k_forward": [P, P, P, P],
     "j2k_step": [P, P, P, P],
     "j2k_forward_batch_dev": [P, I, P, P, P, I, P],

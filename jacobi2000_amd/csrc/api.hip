@@ -109,6 +109,7 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     need_form(m, J2K_FORM_CELL_ST, "j2k_iterate");
     if (max_iter <= 0) max_iter = m.cfg.max_iter;
     if (tol < 0) tol = m.cfg.tol;
+    if (m.tune.enabled) autotune(m, (long)T * ny * nx);  // before the tables: it reuses their slots
     CellRun run;
     cell_setup(m, s, CellDims{T, ny, nx}, d_meta, run);
     const CellPack& p = run.pack;
@@ -180,6 +181,20 @@ int j2k_model_config(const void* h, j2k_config* out) {
         need(h, "model");
         need(out, "out");
         *out = static_cast<const Model*>(h)->cfg;
+    });
+}
+
+int j2k_model_autotune(void* h, long long max_tokens, char* report, size_t report_len) {
+    return guard([&] {
+        Model& m = model_of(h);
+        if (max_tokens < 0) fail(J2K_ERR_INVALID, "max_tokens must be >= 0");
+        if (max_tokens > 0) autotune(m, (long)max_tokens);
+        if (report && report_len) {
+            const std::string j = tuning_json(m);
+            const size_t n = std::min(j.size(), report_len - 1);
+            std::memcpy(report, j.data(), n);
+            report[n] = 0;
+        }
     });
 }
 

@@ -4,12 +4,24 @@
 // loops over the heads. For each head it stages the K and V rows of the
 // tile's neighbourhood (the tile plus a periodic halo of RS cells in y and x,
 // over the clipped time window t-RT .. t+RT) into LDS once, so every key row
-// is read from L2 once per tile instead of once per query. Four lanes share a
-// query, each holding a quarter of the head's dimensions; the dot products
-// are reduced with two lane shuffles. The softmax is two-pass over the
-// scores kept in registers (max, then exp / sum / weighted V in one sweep),
-// f32 throughout; keys are visited in the order time offset, y offset,
-// x offset (the order of the generic kernel and of the oracle).
+// is read from L2 once per tile instead of once per query. Rows are loaded
+// into registers one phase ahead of their use, so the L2 round trips overlap
+// the arithmetic. Tiles are dealt to blocks XCD by XCD (xcd_remap):
+// consecutive tiles, and so the time slices whose K/V rows they share, sit
+// in one XCD's L2.
+//
+// Two staging modes, chosen by the autotuner (the arithmetic, and so the
+// result, is the same in both):
+//   KV_JOINT  K and V of a head sit in LDS together; one phase per head.
+//   KV_SPLIT  K, then V, in one buffer of half the size: twice the blocks
+//             per CU, twice the barriers.
+//
+// Four lanes share a query, each holding a quarter of the head's dimensions;
+// the dot products are reduced with two lane shuffles. The softmax is
+// two-pass over the scores kept in registers (max, then exp / sum / weighted
+// V in one sweep), f32 throughout; keys are visited in the order time
+// offset, y offset, x offset (the order of the generic kernel and of the
+// oracle).
 #include "common.h"
 #include "kernels.h"
 
@@ -17,31 +29,30 @@ namespace j2k {
 
 constexpr int CT_Y = 8, CT_X = 8;
 
-template <class T>
-__device__ __forceinline__ float ldf(const T* p);
-template <>
-__device__ __forceinline__ float ldf<float>(const float* p) {
-    return *p;
-}
-template <>
-__device__ __forceinline__ float ldf<uint16_t>(const uint16_t* p) {
-    return bf2f(*p);
-}
+// A native 16-byte vector: copies of it stay in registers (HIP's uint4 is a
+// struct whose copies lower to memcpy and pin arrays of it in scratch).
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// N consecutive elements from LDS as f32, in 16-byte vectors.
+// N consecutive elements as f32, in 16-byte vectors when N allows (the row
+// then starts 16-byte aligned: head sizes and offsets are multiples of it).
 template <int N>
-__device__ __forceinline__ void lds_row(const float* p, float* o) {
+__device__ __forceinline__ void row_f32(const float* p, float* o) {
+    if constexpr (N % 4 == 0) {
 #pragma unroll
-    for (int i = 0; i < N; i += 4) {
-        const float4 v = *reinterpret_cast<const float4*>(p + i);
-        o[i] = v.x;
-        o[i + 1] = v.y;
-        o[i + 2] = v.z;
-        o[i + 3] = v.w;
+        for (int i = 0; i < N; i += 4) {
+            const float4 v = *reinterpret_cast<const float4*>(p + i);
+            o[i] = v.x;
+            o[i + 1] = v.y;
+            o[i + 2] = v.z;
+            o[i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) o[i] = p[i];
     }
 }
 template <int N>
-__device__ __forceinline__ void lds_row(const uint16_t* p, float* o) {
+__device__ __forceinline__ void row_f32(const uint16_t* p, float* o) {
     if constexpr (N % 8 == 0) {
 #pragma unroll
         for (int i = 0; i < N; i += 8) {
@@ -58,6 +69,31 @@ __device__ __forceinline__ void lds_row(const uint16_t* p, float* o) {
         for (int i = 0; i < N; i++) o[i] = bf2f(p[i]);
     }
 }
+template <int N>
+__device__ __forceinline__ void row_store(float* p, const float* v) {
+    if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < N; i += 4) *reinterpret_cast<float4*>(p + i) = make_float4(v[i], v[i + 1], v[i + 2], v[i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) p[i] = v[i];
+    }
+}
+template <int N>
+__device__ __forceinline__ void row_store(uint16_t* p, const float* v) {
+    if constexpr (N % 8 == 0) {
+#pragma unroll
+        for (int i = 0; i < N; i += 8) {
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[j] = (uint32_t)f2bf(v[i + 2 * j]) | ((uint32_t)f2bf(v[i + 2 * j + 1]) << 16);
+            *reinterpret_cast<uint4*>(p + i) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < N; i++) p[i] = f2bf(v[i]);
+    }
+}
 
 __device__ __forceinline__ int wrap(int v, int n) {
     while (v < 0) v += n;
@@ -65,46 +101,119 @@ __device__ __forceinline__ int wrap(int v, int n) {
     return v;
 }
 
-template <int HD, int RS, int RT, class T>
-__global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T* __restrict__ out, int dim, int heads,
-                                                   const Seg* __restrict__ segs, const CellTile* __restrict__ tiles) {
+// Waves per SIMD the LDS footprint allows (two blocks per CU up to 80 KB);
+// at one, the register file holds twice as much and nothing spills.
+template <int HD, int RS, int RT, class T, int MODE>
+constexpr int cell_occupancy() {
+    return (MODE == KV_JOINT ? 2 : 1) * (2 * RT + 1) * (CT_Y + 2 * RS) * (CT_X + 2 * RS) * HD * (int)sizeof(T) >
+                   80 * 1024
+               ? 1
+               : 2;
+}
+
+template <int HD, int RS, int RT, class T, int MODE>
+__global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void k_attn_cell(
+    const T* __restrict__ qkv, T* __restrict__ out, int dim, int heads, const Seg* __restrict__ segs,
+    const CellTile* __restrict__ tiles) {
     constexpr int D4 = HD / 4;               // dims per lane
     constexpr int VEC = 16 / sizeof(T);      // elements per 16-byte vector
     constexpr int VPR = HD / VEC;            // vectors per K (or V) row
     constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
-    constexpr int KMAX = (2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1);
+    constexpr int KS = (2 * RS + 1) * (2 * RS + 1), KMAX = KS * (2 * RT + 1);
+    constexpr int ROWS = MODE == KV_JOINT ? 2 : 1;  // K/V rows per cell in one staging phase
+    constexpr int NV = ((2 * RT + 1) * PLANE * ROWS * VPR + 255) / 256;  // staged vectors per thread
     extern __shared__ __align__(16) unsigned char smem[];
-    const CellTile tl = tiles[blockIdx.x];
+    const CellTile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
     const Seg sg = segs[tl.seg];
     const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + RT, sg.T - 1), nt = t_hi - t_lo + 1;
-    const int ncell = nt * PLANE;
+    const int ncell = nt * PLANE, nvec = ncell * ROWS * VPR;
     T* Ks = reinterpret_cast<T*>(smem);
-    T* Vs = Ks + (size_t)ncell * HD;
+    T* Vs = MODE == KV_JOINT ? Ks + (size_t)ncell * HD : Ks;
     const long ld = 3L * dim;
     const int n = sg.ny * sg.nx;
+    const int tid = threadIdx.x;
 
-    const int tid = threadIdx.x, q = tid >> 2, sub = tid & 3;
+    // Vector v of a staging phase holds elements [e*VEC, e*VEC+VEC) of the K
+    // (which = 0) or V row of neighbourhood cell c: v = (c*ROWS + which)*VPR
+    // + e (KV_JOINT), v = c*VPR + e with `which` set by the phase (KV_SPLIT).
+    // Vectors past the neighbourhood (v >= nvec) load the segment's first row
+    // and are never stored, so every load is unconditional.
+    auto row_of = [&](int v) {
+        if (v >= nvec) return sg.base;
+        const int c = v / (ROWS * VPR), ct = c / PLANE, cr = c - ct * PLANE, cy = cr / HW, cx = cr - cy * HW;
+        const int y = wrap(tl.y0 - RS + cy, sg.ny), x = wrap(tl.x0 - RS + cx, sg.nx);
+        return sg.base + (t_lo + ct) * n + y * sg.nx + x;
+    };
+    // The first PF vectors of each thread are loaded into registers one phase
+    // ahead; the rest (wide stencils) are loaded when the phase is staged. PF
+    // is sized so that the prefetch (5 registers a vector), the scores and
+    // the q / k / accumulator rows fit the register file without spilling.
+    constexpr int PF_FIT = ((cell_occupancy<HD, RS, RT, T, MODE>() == 2 ? 160 : 200) - KMAX - 3 * D4) / 5;
+    constexpr int PF = NV < PF_FIT ? NV : (PF_FIT < 4 ? 4 : PF_FIT);
+    int grow[PF];
+#pragma unroll
+    for (int i = 0; i < PF; i++) grow[i] = row_of(tid + 256 * i);
+    // Loads of phase (head h, rows `w`: 0 = K, 1 = V, -1 = both) and their
+    // stores into LDS; written out in place rather than as lambdas so that
+    // `pre` stays in registers.
+#define J2K_WHICH(v, w) ((w) < 0 ? ((v) / VPR) & 1 : (w))
+#define J2K_FETCH(h, w)                                                                                            \
+    _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                               \
+        const int v = tid + 256 * i, e = v % VPR;                                                                  \
+        pre[i] = *reinterpret_cast<const u32x4*>(qkv + (long)grow[i] * ld + (1 + J2K_WHICH(v, w)) * dim + (h) * HD + \
+                                                 e * VEC);                                                         \
+    }
+#define J2K_STAGE(h, w)                                                                                            \
+    {                                                                                                              \
+        _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                           \
+            const int v = tid + 256 * i, c = v / (ROWS * VPR), e = v % VPR;                                        \
+            if (v < nvec) *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = pre[i]; \
+        }                                                                                                          \
+        _Pragma("unroll") for (int i0 = PF; i0 < NV; i0 += 8) {                                                    \
+            u32x4 tmp[8];                                                                                          \
+            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
+                const int i = i0 + j, v = tid + 256 * i, e = v % VPR;                                              \
+                if (i < NV)                                                                                        \
+                    tmp[j] = *reinterpret_cast<const u32x4*>(qkv + (long)row_of(v) * ld + (1 + J2K_WHICH(v, w)) * dim + \
+                                                             (h) * HD + e * VEC);                                  \
+            }                                                                                                      \
+            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
+                const int i = i0 + j, v = tid + 256 * i, c = v / (ROWS * VPR), e = v % VPR;                        \
+                if (i < NV && v < nvec)                                                                            \
+                    *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = tmp[j];    \
+            }                                                                                                      \
+        }                                                                                                          \
+    }
+    u32x4 pre[PF];
+    const int q = tid >> 2, sub = tid & 3;
     const int qy = tl.y0 + (q >> 3), qx = tl.x0 + (q & 7);
     const bool live = qy < sg.ny && qx < sg.nx;
     const long qrow = (long)sg.base + (long)tl.t * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
     const float scale = 1.0f / sqrtf((float)HD);
 
+    if (MODE == KV_JOINT) {
+        J2K_FETCH(0, -1)
+    } else {
+        J2K_FETCH(0, 0)
+    }
     for (int h = 0; h < heads; h++) {
-        __syncthreads();  // the previous head's K/V are no longer read
-        for (int v = tid; v < ncell * 2 * VPR; v += 256) {
-            const int c = v / (2 * VPR), rem = v - c * (2 * VPR), which = rem / VPR, e = rem - which * VPR;
-            const int ct = c / PLANE, cr = c - ct * PLANE, cy = cr / HW, cx = cr - cy * HW;
-            const int y = wrap(tl.y0 - RS + cy, sg.ny), x = wrap(tl.x0 - RS + cx, sg.nx);
-            const long row = (long)sg.base + (long)(t_lo + ct) * n + (long)y * sg.nx + x;
-            const uint4 val = *reinterpret_cast<const uint4*>(qkv + row * ld + (1 + which) * dim + h * HD + e * VEC);
-            *reinterpret_cast<uint4*>((which ? Vs : Ks) + (size_t)c * HD + e * VEC) = val;
+        __syncthreads();  // the previous phase's rows are no longer read
+        if (MODE == KV_JOINT) {
+            J2K_STAGE(h, -1)
+        } else {
+            J2K_STAGE(h, 0)
         }
         __syncthreads();
+        if (MODE == KV_JOINT) {
+            if (h + 1 < heads) {
+                J2K_FETCH(h + 1, -1)  // in flight while this head is computed
+            }
+        } else {
+            J2K_FETCH(h, 1)  // V of this head, in flight during the scores
+        }
         float qv[D4], kv[D4], acc[D4], s[KMAX];
-#pragma unroll
-        for (int d = 0; d < D4; d++) qv[d] = ldf<T>(qkv + qrow * ld + h * HD + sub * D4 + d);
+        row_f32<D4>(qkv + qrow * ld + h * HD + sub * D4, qv);
         float mx = -INFINITY;
-        int nk = 0;
 #pragma unroll
         for (int ct = 0; ct < 2 * RT + 1; ct++) {
             if (ct >= nt) break;
@@ -113,17 +222,24 @@ __global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T*
 #pragma unroll
                 for (int dx = -RS; dx <= RS; dx++) {
                     const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
-                    lds_row<D4>(Ks + (size_t)c * HD + sub * D4, kv);
+                    row_f32<D4>(Ks + (size_t)c * HD + sub * D4, kv);
                     float dot = 0.f;
 #pragma unroll
                     for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
                     dot += __shfl_xor(dot, 1, 64);
                     dot += __shfl_xor(dot, 2, 64);
                     const float sc = dot * scale;
-                    s[ct * (2 * RS + 1) * (2 * RS + 1) + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
+                    s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
                     mx = fmaxf(mx, sc);
-                    nk++;
                 }
+        }
+        if (MODE == KV_SPLIT) {
+            __syncthreads();  // every wave's scores are done with K
+            J2K_STAGE(h, 1)
+            __syncthreads();
+            if (h + 1 < heads) {
+                J2K_FETCH(h + 1, 0)  // next head's K, in flight during the weighting
+            }
         }
 #pragma unroll
         for (int d = 0; d < D4; d++) acc[d] = 0.f;
@@ -136,38 +252,38 @@ __global__ __launch_bounds__(256) void k_attn_cell(const T* __restrict__ qkv, T*
 #pragma unroll
                 for (int dx = -RS; dx <= RS; dx++) {
                     const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
-                    const float p = __expf(s[ct * (2 * RS + 1) * (2 * RS + 1) + (dy + RS) * (2 * RS + 1) + dx + RS] - mx);
+                    const float p = __expf(s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] - mx);
                     l += p;
-                    lds_row<D4>(Vs + (size_t)c * HD + sub * D4, kv);
+                    row_f32<D4>(Vs + (size_t)c * HD + sub * D4, kv);
 #pragma unroll
                     for (int d = 0; d < D4; d++) acc[d] += p * kv[d];
                 }
         }
-        (void)nk;
         if (live) {
             const float inv = 1.0f / l;
-            T* o = out + qrow * dim + h * HD + sub * D4;
 #pragma unroll
-            for (int d = 0; d < D4; d++) {
-                if constexpr (sizeof(T) == 4) o[d] = acc[d] * inv;
-                else o[d] = f2bf(acc[d] * inv);
-            }
+            for (int d = 0; d < D4; d++) acc[d] *= inv;
+            row_store<D4>(out + qrow * dim + h * HD + sub * D4, acc);
         }
     }
+#undef J2K_STAGE
+#undef J2K_FETCH
+#undef J2K_WHICH
 }
 
 namespace {
 
 template <int HD, int RS, int RT, class T>
-void* fn_ptr() {
-    return (void*)k_attn_cell<HD, RS, RT, T>;
+const void* fn_ptr(int mode) {
+    return mode == KV_JOINT ? (const void*)k_attn_cell<HD, RS, RT, T, KV_JOINT>
+                            : (const void*)k_attn_cell<HD, RS, RT, T, KV_SPLIT>;
 }
 
-// Kernel for (head size, space radius, time radius); nullptr when not built.
+// Kernel for (head size, space radius, time radius, mode); nullptr when not built.
 template <class T>
-const void* pick(int hd, int rs, int rt) {
+const void* pick(int hd, int rs, int rt, int mode) {
 #define J2K_ATTN_CASE(HD_, RS_, RT_) \
-    if (hd == HD_ && rs == RS_ && rt == RT_) return fn_ptr<HD_, RS_, RT_, T>();
+    if (hd == HD_ && rs == RS_ && rt == RT_) return fn_ptr<HD_, RS_, RT_, T>(mode);
 #define J2K_ATTN_HD(HD_)      \
     J2K_ATTN_CASE(HD_, 1, 0)  \
     J2K_ATTN_CASE(HD_, 1, 1)  \
@@ -184,9 +300,9 @@ const void* pick(int hd, int rs, int rt) {
 
 }  // namespace
 
-bool attn_cell_ok(int hd, int rs, int rt, int elem) {
-    const void* f = elem == 4 ? pick<float>(hd, rs, rt) : pick<uint16_t>(hd, rs, rt);
-    return f != nullptr && attn_cell_smem(hd, rs, rt, elem) <= 160 * 1024;
+bool attn_cell_ok(int hd, int rs, int rt, int elem, int mode) {
+    const void* f = elem == 4 ? pick<float>(hd, rs, rt, mode) : pick<uint16_t>(hd, rs, rt, mode);
+    return f != nullptr && attn_cell_smem(hd, rs, rt, elem, mode) <= 160 * 1024;
 }
 
 // Dynamic LDS above 64 KB has to be allowed per kernel; done once per device
@@ -194,14 +310,17 @@ bool attn_cell_ok(int hd, int rs, int rt, int elem) {
 void attn_cell_prepare() {
     for (int hd : {8, 16, 32, 64})
         for (int rs : {1, 2})
-            for (int rt : {0, 1}) {
-                HIPCHK(hipFuncSetAttribute(pick<float>(hd, rs, rt), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                HIPCHK(hipFuncSetAttribute(pick<uint16_t>(hd, rs, rt), hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            }
+            for (int rt : {0, 1})
+                for (int mode : {KV_JOINT, KV_SPLIT}) {
+                    HIPCHK(hipFuncSetAttribute(pick<float>(hd, rs, rt, mode),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                    HIPCHK(hipFuncSetAttribute(pick<uint16_t>(hd, rs, rt, mode),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                }
 }
 
-size_t attn_cell_smem(int hd, int rs, int rt, int elem) {
-    return 2ull * (2 * rt + 1) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
+size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode) {
+    return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + 1) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
 }
 
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles) {
@@ -214,24 +333,24 @@ void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles
 
 template <class T>
 static void launch(hipStream_t s, const T* qkv, T* out, int dim, int heads, int rs, int rt, const Seg* segs,
-                   const CellTile* tiles, int n_tiles) {
+                   const CellTile* tiles, int n_tiles, int mode) {
     if (n_tiles <= 0) return;
     const int hd = dim / heads;
-    const void* f = pick<T>(hd, rs, rt);
-    const size_t smem = attn_cell_smem(hd, rs, rt, sizeof(T));
+    const void* f = pick<T>(hd, rs, rt, mode);
+    const size_t smem = attn_cell_smem(hd, rs, rt, sizeof(T), mode);
     if (!f || smem > 160 * 1024) fail(J2K_ERR_LIMIT, "attention: no LDS-tiled kernel for this head size / stencil");
     void* args[] = {(void*)&qkv, (void*)&out, (void*)&dim, (void*)&heads, (void*)&segs, (void*)&tiles};
     HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(256), args, smem, s));
 }
 
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,
-                        const Seg* segs, const CellTile* tiles, int n_tiles) {
-    launch<float>(s, qkv, out, dim, heads, rs, rt, segs, tiles, n_tiles);
+                        const Seg* segs, const CellTile* tiles, int n_tiles, int mode) {
+    launch<float>(s, qkv, out, dim, heads, rs, rt, segs, tiles, n_tiles, mode);
 }
 
 void attention_cell_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, int rs, int rt,
-                         const Seg* segs, const CellTile* tiles, int n_tiles) {
-    launch<uint16_t>(s, qkv, out, dim, heads, rs, rt, segs, tiles, n_tiles);
+                         const Seg* segs, const CellTile* tiles, int n_tiles, int mode) {
+    launch<uint16_t>(s, qkv, out, dim, heads, rs, rt, segs, tiles, n_tiles, mode);
 }
 
 }  // namespace j2k

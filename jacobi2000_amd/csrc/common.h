@@ -55,4 +55,12 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uin
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Workgroups are dealt round-robin over the 8 XCDs (block b runs on XCD
+// b mod 8). This bijection on [0, G) hands each XCD one contiguous run of
+// work items, so neighbouring items share that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int b, int G) {
+    const int q = G >> 3, r = G & 7, k = b & 7;
+    return k * q + min(k, r) + (b >> 3);
+}
+
 }  // namespace j2k

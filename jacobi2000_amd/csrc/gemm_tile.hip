@@ -1,15 +1,16 @@
 // Tiled bf16 GEMM for the trunk's products (M = tokens, N and K a few
 // hundred) on v_mfma_f32_16x16x32_bf16 with f32 accumulation.
 //
-// Block tile 128 x 128, 256 threads = 4 waves in a 2 x 2 grid, each wave a
-// 64 x 64 sub-tile of 4 x 4 MFMA tiles. K advances in chunks of 64: a chunk of
-// A and of Bt (both K-contiguous) sits in LDS, rows padded by 8 bf16 so the
-// 16-row fragment reads of a wave hit distinct banks, while the next chunk is
-// already loading into registers. Sized for two blocks per CU.
+// Block tile BM x 128 (BM = 64 or 128, chosen per shape by the autotuner),
+// 256 threads = 4 waves; each wave owns 64 rows and 128 / (4*64/BM) columns
+// of 16 x 16 MFMA tiles. K advances in chunks of 64: a chunk of A and of Bt
+// (both K-contiguous) sits in LDS, rows padded by 8 bf16 so the 16-row
+// fragment reads of a wave hit distinct banks, while the next chunk is
+// already loading into registers.
 //
-// Blocks map to tiles XCD by XCD: workgroups are dealt round-robin over the
-// 8 XCDs, so tile t goes to block ((t mod G/8) * 8 + t / (G/8)) and the
-// column tiles of one row panel land on one XCD and share its L2.
+// Blocks map to tiles XCD by XCD (xcd_remap): each XCD takes a contiguous
+// run of tiles, so the column tiles of one row panel land on one XCD and
+// share its L2.
 //
 // SPLIT: A is f32 and is staged as a bf16 pair hi = bf16(a), lo = bf16(a-hi);
 // every tile takes two MFMAs against the same bf16 weight fragment, so the
@@ -27,46 +28,47 @@ namespace j2k {
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
-constexpr int TBM = 128, TBN = 128, TBK = 64, TLD = TBK + 8;
+constexpr int TBN = 128, TBK = 64, TLD = TBK + 8;
 
-template <int EPI, bool SPLIT>
+template <int EPI, bool SPLIT, int BM>
 __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const void* __restrict__ Av,
                                                       const uint16_t* __restrict__ Bt, const float* __restrict__ bias,
                                                       void* __restrict__ Cv, float* __restrict__ X,
                                                       const float* __restrict__ lng, const float* __restrict__ lnb,
                                                       void* __restrict__ Hv, int nt) {
-    __shared__ __align__(16) uint16_t Ah[TBM * TLD];
-    __shared__ __align__(16) uint16_t Al[SPLIT ? TBM * TLD : 8];
+    constexpr int WMW = BM / 64, WNW = 4 / WMW;  // waves along rows / columns
+    constexpr int WN = TBN / WNW, J = WN / 16;   // columns per wave, column tiles per wave
+    __shared__ __align__(16) uint16_t Ah[BM * TLD];
+    __shared__ __align__(16) uint16_t Al[SPLIT ? BM * TLD : 8];
     __shared__ __align__(16) uint16_t Bs[TBN * TLD];
-    __shared__ float red[2][TBM];
+    __shared__ float red[TBN / 16][BM];
 
-    const int G = gridDim.x, b = blockIdx.x;
-    const int t = (G % 8 == 0) ? (b % 8) * (G / 8) + b / 8 : b;
-    const int m0 = (t / nt) * TBM, n0 = (t % nt) * TBN;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (t / nt) * BM, n0 = (t % nt) * TBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int wm = (wave / WNW) * 64, wn = (wave % WNW) * WN;
     const int r16 = lane & 15, kq = (lane >> 4) * 8;
 
-    f4v acc[4][4];
+    f4v acc[4][J];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < J; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-    constexpr int NA = SPLIT ? 8 : 4;  // 16-byte vectors of A per thread per chunk
+    constexpr int NA = (SPLIT ? 16 : 8) * BM / 256;  // 16-byte vectors of A per thread per chunk
     uint4 pa[NA], pb[4];
     auto load_chunk = [&](int kc) {
         if (!SPLIT) {
             const uint16_t* A = static_cast<const uint16_t*>(Av);
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < NA; q++) {
                 const int v = tid + 256 * q, row = v >> 3, k = kc + (v & 7) * 8, m = m0 + row;
                 pa[q] = (m < M && k < K) ? *reinterpret_cast<const uint4*>(A + (long)m * K + k) : make_uint4(0, 0, 0, 0);
             }
         } else {
             const float* A = static_cast<const float*>(Av);
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < NA; q++) {
                 const int v = tid + 256 * q, row = v >> 4, k = kc + (v & 15) * 4, m = m0 + row;
                 pa[q] = (m < M && k < K) ? *reinterpret_cast<const uint4*>(A + (long)m * K + k) : make_uint4(0, 0, 0, 0);
             }
@@ -80,13 +82,13 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
     auto store_chunk = [&]() {
         if (!SPLIT) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < NA; q++) {
                 const int v = tid + 256 * q, row = v >> 3, kv = v & 7;
                 *reinterpret_cast<uint4*>(&Ah[row * TLD + kv * 8]) = pa[q];
             }
         } else {
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
+            for (int q = 0; q < NA; q++) {
                 const int v = tid + 256 * q, row = v >> 4, kv = v & 15;
                 const uint32_t w[4] = {pa[q].x, pa[q].y, pa[q].z, pa[q].w};
                 uint32_t hi[4], lo[4];
@@ -115,9 +117,9 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
         if (kc + TBK < K) load_chunk(kc + TBK);
 #pragma unroll
         for (int ks = 0; ks < TBK; ks += 32) {
-            bf8v a[4], bb[4];
+            bf8v a[4], bb[J];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
+            for (int j = 0; j < J; j++)
                 bb[j] = __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(&Bs[(wn + j * 16 + r16) * TLD + ks + kq]));
 #pragma unroll
             for (int i = 0; i < 4; i++)
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+                for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
             if (SPLIT) {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
@@ -133,15 +135,15 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
 #pragma unroll
                 for (int i = 0; i < 4; i++)
 #pragma unroll
-                    for (int j = 0; j < 4; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bb[j], acc[i][j], 0, 0, 0);
             }
         }
     }
 
-    float bcol[4];
-    bool colok[4];
+    float bcol[J];
+    bool colok[J];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < J; j++) {
         const int n = n0 + wn + j * 16 + r16;
         colok[j] = n < N;
         bcol[j] = colok[j] ? bias[n] : 0.f;
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
                 if (m >= M) continue;
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+                for (int j = 0; j < J; j++) {
                     if (!colok[j]) continue;
                     float v = acc[i][j][r] + bcol[j];
                     if (EPI == EPI_GELU) v = gelu(v);
@@ -172,7 +174,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
             const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
             if (m >= M) continue;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < J; j++) {
                 if (!colok[j]) continue;
                 float* p = X + (long)m * N + n0 + wn + j * 16 + r16;
                 const float v = *p + acc[i][j][r] + bcol[j];
@@ -183,41 +185,44 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
     if (EPI != EPI_RESID_LN) return;
 
     // LayerNorm of the updated rows (cpu.rs:49-65). N == 128: a row's values
-    // sit in 16 lanes x 4 tiles of each of the two waves sharing its rows.
+    // sit in the 16 lanes of 8 column tiles, spread over the waves sharing
+    // its rows. Each tile's 16 values are reduced by lane shuffles and the 8
+    // tile sums added in column order, so the result is the same for every
+    // block height.
     float mean[4][4], inv[4][4];
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float s = 0.f;
+            for (int r = 0; r < 4; r++)
 #pragma unroll
-                for (int j = 0; j < 4; j++) {
+                for (int j = 0; j < J; j++) {
                     const float v = acc[i][j][r];
-                    s += pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
+                    float s = pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
+                    s += __shfl_xor(s, 1, 64);
+                    s += __shfl_xor(s, 2, 64);
+                    s += __shfl_xor(s, 4, 64);
+                    s += __shfl_xor(s, 8, 64);
+                    if (r16 == 0) red[wn / 16 + j][wm + i * 16 + (lane >> 4) * 4 + r] = s;
                 }
-                s += __shfl_xor(s, 1, 64);
-                s += __shfl_xor(s, 2, 64);
-                s += __shfl_xor(s, 4, 64);
-                s += __shfl_xor(s, 8, 64);
-                if (r16 == 0) red[wave & 1][wm + i * 16 + (lane >> 4) * 4 + r] = s;
-            }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = wm + i * 16 + (lane >> 4) * 4 + r;
-                const float tot = red[0][row] + red[1][row];
+                float tot = 0.f;
+#pragma unroll
+                for (int w = 0; w < TBN / 16; w++) tot += red[w][row];
                 if (pass == 0) mean[i][r] = tot / (float)N;
                 else inv[i][r] = 1.0f / sqrtf(tot / (float)N + 1e-5f);
             }
         __syncthreads();
     }
-    float gc[4], bc[4];
+    float gc[J], bc[J];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < J; j++) {
         gc[j] = lng[wn + j * 16 + r16];
         bc[j] = lnb[wn + j * 16 + r16];
     }
@@ -228,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
             const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
             if (m >= M) continue;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
+            for (int j = 0; j < J; j++) {
                 const float h = (acc[i][j][r] - mean[i][r]) * inv[i][r] * gc[j] + bc[j];
                 const long o = (long)m * N + wn + j * 16 + r16;
                 if (SPLIT) static_cast<float*>(Hv)[o] = h;
@@ -239,32 +244,46 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
 
 bool gemm_tile_ok(int N, int K, bool split) { return N % 16 == 0 && K % (split ? 4 : 8) == 0; }
 
+int gemm_tile_default_bm(int M, int N) {
+    // enough blocks to give every CU two at BM = 128, else halve the rows
+    const long blocks128 = (long)cdiv(M, 128) * cdiv(N, TBN);
+    return blocks128 >= 512 ? 128 : 64;
+}
+
+template <bool SPLIT, int BM>
+static void tile_bm(hipStream_t s, int M, int N, int K, const void* A, const uint16_t* Bt, const float* bias, void* C,
+                    float* X, int epi, const float* g, const float* b, void* H) {
+    const int mt = cdiv(M, BM), nt = cdiv(N, TBN);
+    dim3 grid(mt * nt), block(256);
+    switch (epi) {
+        case EPI_STORE: hipLaunchKernelGGL((k_tile_bf16<EPI_STORE, SPLIT, BM>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
+        case EPI_GELU: hipLaunchKernelGGL((k_tile_bf16<EPI_GELU, SPLIT, BM>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
+        case EPI_RESID: hipLaunchKernelGGL((k_tile_bf16<EPI_RESID, SPLIT, BM>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
+        default: hipLaunchKernelGGL((k_tile_bf16<EPI_RESID_LN, SPLIT, BM>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
+    }
+}
+
 template <bool SPLIT>
 static void tile(hipStream_t s, int M, int N, int K, const void* A, const uint16_t* Bt, const float* bias, void* C,
-                 float* X, int epi, const float* g, const float* b, void* H) {
+                 float* X, int epi, const float* g, const float* b, void* H, int bm) {
     if (M <= 0) return;
     if (!gemm_tile_ok(N, K, SPLIT)) fail(J2K_ERR_LIMIT, "gemm tile: N must be a multiple of 16 and K of 8");
     if (((uintptr_t)A & 15) || ((uintptr_t)Bt & 15)) fail(J2K_ERR_INVALID, "gemm tile: operands must be 16-byte aligned");
     if (epi == EPI_RESID_LN && N != TBN) fail(J2K_ERR_LIMIT, "gemm tile: the LayerNorm epilogue needs N == 128");
-    const int mt = cdiv(M, TBM), nt = cdiv(N, TBN);
-    dim3 grid(mt * nt), block(256);
-    switch (epi) {
-        case EPI_STORE: hipLaunchKernelGGL((k_tile_bf16<EPI_STORE, SPLIT>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
-        case EPI_GELU: hipLaunchKernelGGL((k_tile_bf16<EPI_GELU, SPLIT>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
-        case EPI_RESID: hipLaunchKernelGGL((k_tile_bf16<EPI_RESID, SPLIT>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
-        default: hipLaunchKernelGGL((k_tile_bf16<EPI_RESID_LN, SPLIT>), grid, block, 0, s, M, N, K, A, Bt, bias, C, X, g, b, H, nt); break;
-    }
+    if (bm <= 0) bm = gemm_tile_default_bm(M, N);
+    if (bm == 64) tile_bm<SPLIT, 64>(s, M, N, K, A, Bt, bias, C, X, epi, g, b, H);
+    else tile_bm<SPLIT, 128>(s, M, N, K, A, Bt, bias, C, X, epi, g, b, H);
     LAUNCHCHK("gemm tile");
 }
 
 void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
-                    uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H) {
-    tile<false>(s, M, N, K, A, Bt, bias, C, X, epi, ln_g, ln_b, H);
+                    uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H, int bm) {
+    tile<false>(s, M, N, K, A, Bt, bias, C, X, epi, ln_g, ln_b, H, bm);
 }
 
 void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
-                      float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H) {
-    tile<true>(s, M, N, K, A, Bt, bias, C, X, epi, ln_g, ln_b, H);
+                      float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H, int bm) {
+    tile<true>(s, M, N, K, A, Bt, bias, C, X, epi, ln_g, ln_b, H, bm);
 }
 
 }  // namespace j2k

@@ -16,17 +16,19 @@ enum Epi : int {
     EPI_RESID_LN = 3,  // X += acc + bias, then H = LayerNorm(X row) in bf16
 };
 
-// ---- tiled bf16 GEMM (128 x 128 tiles, LDS-staged, XCD-aware order) ---------
+// ---- tiled bf16 GEMM (BM x 128 tiles, LDS-staged, XCD-aware order) ----------
 // A [M, K] (bf16, or f32 for the split form), Bt [N, K] bf16, f32 bias.
 // EPI_STORE/EPI_GELU write C [M, N]; EPI_RESID adds into the f32 stream X
 // [M, N]; EPI_RESID_LN (N == 128) also writes H = LN(X) with gamma ln_g,
 // beta ln_b. N % 16 == 0, K % 8 == 0. The split form feeds f32 activations
 // to the bf16 MFMA as a hi + lo pair (two MFMAs per tile); its C, H are f32.
+// `bm` is the block height, 64 or 128 (0: gemm_tile_default_bm).
 bool gemm_tile_ok(int N, int K, bool split);
+int gemm_tile_default_bm(int M, int N);
 void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,
-                    uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H);
+                    uint16_t* C, float* X, int epi, const float* ln_g, const float* ln_b, uint16_t* H, int bm = 0);
 void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
-                      float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H);
+                      float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H, int bm = 0);
 
 // ---- fused MLP: X += ff2(GELU(ff1(H))), then H = LN(X) if ln_g != null ------
 // dim == 128, ff % 128 == 0; H (bf16, or f32 with split) may alias Hout.
@@ -81,18 +83,21 @@ void attention_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, 
 int attn_tokens_per_block(int heads);
 
 // cell_st attention with K/V staged in LDS: one block per 8 x 8 tile of cells
-// of one time slice ({segment, t, y0, x0}), periodic halo `rs`, time window rt.
+// of one time slice ({segment, t, y0, x0}), periodic halo `rs`, time window
+// rt. Staging modes (same results): K and V together, or K then V in half
+// the LDS.
 struct CellTile {
     int seg, t, y0, x0;
 };
+enum KvMode : int { KV_JOINT = 0, KV_SPLIT = 1 };
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles);
-size_t attn_cell_smem(int hd, int rs, int rt, int elem);
-bool attn_cell_ok(int hd, int rs, int rt, int elem);  // a tiled kernel exists and fits in LDS
+size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode);
+bool attn_cell_ok(int hd, int rs, int rt, int elem, int mode);  // a tiled kernel exists and fits in LDS
 void attn_cell_prepare();  // once per device, before the first launch
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,
-                        const Seg* segs, const CellTile* tiles, int n_tiles);
+                        const Seg* segs, const CellTile* tiles, int n_tiles, int mode);
 void attention_cell_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, int rs, int rt,
-                         const Seg* segs, const CellTile* tiles, int n_tiles);
+                         const Seg* segs, const CellTile* tiles, int n_tiles, int mode);
 
 // The attention tables of one packed batch; cell tiles are used when present.
 struct AttnPlan {

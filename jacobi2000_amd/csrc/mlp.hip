@@ -7,7 +7,7 @@
 // intermediate is produced 128 columns at a time: ff1 + bias + GELU into a
 // 64 x 128 LDS tile, which the ff2 accumulators consume right away as their
 // next K chunk. Weights (W1t [ff, dim], W2t [dim, ff], bf16, K-contiguous)
-// stream from L2 as 16-byte MFMA fragments. v_mfma_f32_16x16x32_bf16, f32
+// stream from L2 as 16-byte MFMA fragments, a chunk ahead of their use. v_mfma_f32_16x16x32_bf16, f32
 // accumulate. SPLIT: H is f32 and both the staged H and the intermediate are
 // held as bf16 hi + lo pairs (two MFMAs per tile), as in the split GEMM.
 #include "common.h"
@@ -71,7 +71,19 @@ __global__ __launch_bounds__(256) void k_mlp_bf16(int M, int ff, const void* __r
 #pragma unroll
         for (int j = 0; j < 2; j++) acc2[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
+    // Weight fragments of a whole 128-wide chunk are loaded one phase ahead:
+    // W2 of chunk c while ff1 of chunk c runs, W1 of chunk c+1 while GELU and
+    // ff2 of chunk c run, so their L2 latency hides behind the MFMAs.
+    bf8v w1[4][2], w2[4][2];
+#define J2K_LOAD_W1(c0_)                                                                                        \
+    _Pragma("unroll") for (int k4 = 0; k4 < 4; k4++) _Pragma("unroll") for (int j = 0; j < 2; j++)          \
+        w1[k4][j] = frag(W1t + (long)((c0_) + wn + j * 16 + r16) * MD + k4 * 32 + kq);
+#define J2K_LOAD_W2(c0_)                                                                                        \
+    _Pragma("unroll") for (int k4 = 0; k4 < 4; k4++) _Pragma("unroll") for (int j = 0; j < 2; j++)          \
+        w2[k4][j] = frag(W2t + (long)(wn + j * 16 + r16) * ff + (c0_) + k4 * 32 + kq);
+    J2K_LOAD_W1(0)
     for (int c0 = 0; c0 < ff; c0 += MD) {
+        J2K_LOAD_W2(c0)
         // ff1 for intermediate columns c0 + wn .. + 32 of this wave
         f4v acc1[4][2];
 #pragma unroll
@@ -79,28 +91,30 @@ __global__ __launch_bounds__(256) void k_mlp_bf16(int M, int ff, const void* __r
 #pragma unroll
             for (int j = 0; j < 2; j++) acc1[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < MD; ks += 32) {
-            bf8v a[4], b[2];
-#pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = frag(W1t + (long)(c0 + wn + j * 16 + r16) * MD + ks + kq);
+        for (int k4 = 0; k4 < 4; k4++) {
+            const int ks = k4 * 32;
+            bf8v a[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) a[i] = frag(&Hh[(i * 16 + r16) * MLD + ks + kq]);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc1[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w1[k4][j], acc1[i][j], 0, 0, 0);
             if (SPLIT) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) a[i] = frag(&Hl[(i * 16 + r16) * MLD + ks + kq]);
 #pragma unroll
                 for (int i = 0; i < 4; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc1[i][j], 0, 0, 0);
+                    for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w1[k4][j], acc1[i][j], 0, 0, 0);
             }
         }
         float bb[2];
 #pragma unroll
         for (int j = 0; j < 2; j++) bb[j] = b1[c0 + wn + j * 16 + r16];
+        if (c0 + MD < ff) {
+            J2K_LOAD_W1(c0 + MD)
+        }
         __syncthreads();  // every wave has finished reading the previous intermediate chunk
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -117,26 +131,27 @@ __global__ __launch_bounds__(256) void k_mlp_bf16(int M, int ff, const void* __r
         __syncthreads();
         // ff2: output columns wn .. + 32, K chunk c0 .. c0 + 128
 #pragma unroll
-        for (int ks = 0; ks < MD; ks += 32) {
-            bf8v a[4], b[2];
-#pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = frag(W2t + (long)(wn + j * 16 + r16) * ff + c0 + ks + kq);
+        for (int k4 = 0; k4 < 4; k4++) {
+            const int ks = k4 * 32;
+            bf8v a[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) a[i] = frag(&Fh[(i * 16 + r16) * MLD + ks + kq]);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc2[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w2[k4][j], acc2[i][j], 0, 0, 0);
             if (SPLIT) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) a[i] = frag(&Fl[(i * 16 + r16) * MLD + ks + kq]);
 #pragma unroll
                 for (int i = 0; i < 4; i++)
 #pragma unroll
-                    for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc2[i][j], 0, 0, 0);
+                    for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w2[k4][j], acc2[i][j], 0, 0, 0);
             }
         }
     }
+#undef J2K_LOAD_W1
+#undef J2K_LOAD_W2
 
     // residual
     float bo[2];

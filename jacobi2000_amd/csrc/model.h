@@ -2,6 +2,7 @@
 // the forward passes both forms are built from.
 #pragma once
 
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -43,6 +44,26 @@ struct KTimer {
     ~KTimer();
 };
 
+// Launch choices picked by the autotuner per bucket of packed tokens
+// (bucket b holds M in [2^b, 2^(b+1))); -1 keeps the built-in default.
+enum TuneOp : int {
+    TO_GEMM_QKV = 0,   // block height of the qkv GEMM
+    TO_GEMM_PROJ = 1,  // block height of the attention-output GEMM
+    TO_GEMM_FF = 2,    // block height of ff1 / ff2 when the MLP is not fused
+    TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode)
+    TO_COUNT = 4
+};
+struct Tuning {
+    static constexpr int NB = 31;
+    signed char pick[TO_COUNT][NB];
+    float us[TO_COUNT][NB][2];  // measured µs of each candidate
+    int tuned_to = -1;          // highest bucket tuned so far
+    bool enabled = true;        // iterate / server tune on first use (J2K_AUTOTUNE=0 turns it off)
+    Tuning();
+    static int bucket(long M);
+    int choice(int op, long M) const;  // -1 when untuned
+};
+
 struct Model;
 // Brackets the launches of its scope with events on `s` when timing is on.
 struct TScope {
@@ -71,6 +92,7 @@ struct Model {
     size_t cap[S_COUNT] = {};
     int p_tables_batch = -1;  // batch size the patch2d seg/tile tables were built for
     int p_tiles = 0;
+    Tuning tune;
 
     ~Model();
     void* scratch(int s, size_t bytes);
@@ -102,6 +124,18 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPla
 // patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
 // add_state the result is state + forward (cpu.rs:84-88).
 void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state);
+
+// Times the candidate launch shapes of the low-precision trunk on synthetic
+// packs of 2^b tokens for every bucket b up to that of max_tokens (from
+// 2^12; at most 2^20) not yet tuned, and keeps the fastest per bucket. Runs
+// on the model's stream and synchronises; never call it inside a capture.
+// Results do not depend on the choices (they only change the launch shape).
+void autotune(Model& m, long max_tokens);
+// Sets fixed choices for every bucket and turns tuning off ("gemm_bm=64",
+// "attn_kv_mode=1", comma separated; unnamed ops keep their defaults).
+void force_tuning(Model& m, const char* spec);
+// The tuning table as JSON text.
+std::string tuning_json(const Model& m);
 
 // Tile table for a list of segments (host side).
 void build_tiles(const std::vector<Seg>& segs, int tpb, std::vector<Tile>& tiles);

@@ -1,4 +1,7 @@
 // Model construction (weights to HBM) and the two forward passes' shared parts.
+#include <cstdlib>
+#include <cstring>
+
 #include "model.h"
 
 #include <cstring>
@@ -163,6 +166,8 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
     DeviceGuard dg(device);
     auto m = std::make_unique<Model>();
     m->cfg = cfg;
+    if (const char* e = std::getenv("J2K_AUTOTUNE")) m->tune.enabled = std::strcmp(e, "0") != 0;
+    if (const char* e = std::getenv("J2K_TUNE_FORCE")) force_tuning(*m, e);
     m->device = device;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
     if (cfg.form == J2K_FORM_CELL_ST) attn_cell_prepare();
@@ -254,18 +259,20 @@ namespace {
 // Attention over one packed batch: the LDS-tiled kernel for the cell_st
 // stencil when its neighbourhood fits in LDS, the generic gather otherwise.
 template <class Act>
-void attend(Model& m, hipStream_t s, const Act* QKV, Act* O, const AttnPlan& ap) {
+void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPlan& ap) {
     const int d = m.cfg.dim, H = m.cfg.heads;
     const AttnGeo geo = m.geo();
-    const bool tiled =
-        ap.ctiles && geo.kind == ATTN_CELL_ST && attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act));
+    int mode = m.tune.choice(TO_ATTN, M);
+    if (mode < 0 || !attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode)) mode = KV_JOINT;
+    const bool tiled = ap.ctiles && geo.kind == ATTN_CELL_ST &&
+                       attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode);
     if (tiled) {
         if (sizeof(Act) == 4)
             attention_cell_f32(s, (const float*)QKV, (float*)O, d, H, geo.radius, geo.time_radius, ap.segs, ap.ctiles,
-                               ap.n_ctiles);
+                               ap.n_ctiles, mode);
         else
             attention_cell_bf16(s, (const uint16_t*)QKV, (uint16_t*)O, d, H, geo.radius, geo.time_radius, ap.segs,
-                                ap.ctiles, ap.n_ctiles);
+                                ap.ctiles, ap.n_ctiles, mode);
         return;
     }
     const int tpb = attn_tokens_per_block(H);
@@ -290,13 +297,14 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
     Act* F = m.get<Act>(S_F, (size_t)M * ff);
     const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const bool fuse_ln = d == 128;
-    auto gemm = [&](int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
+    auto gemm = [&](int op, int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb) {
         TScope t(m, s, KC_GEMM, g / d * N * K);
+        const int bm = m.tune.choice(op, M);
         if (split)
-            gemm_bf16x2_tile(s, M, N, K, (const float*)A, Bt, bias, (float*)C, X, epi, lg, lb, (float*)Hh);
+            gemm_bf16x2_tile(s, M, N, K, (const float*)A, Bt, bias, (float*)C, X, epi, lg, lb, (float*)Hh, bm);
         else
-            gemm_bf16_tile(s, M, N, K, (const uint16_t*)A, Bt, bias, (uint16_t*)C, X, epi, lg, lb, (uint16_t*)Hh);
+            gemm_bf16_tile(s, M, N, K, (const uint16_t*)A, Bt, bias, (uint16_t*)C, X, epi, lg, lb, (uint16_t*)Hh, bm);
     };
     auto norm = [&](const float* lg, const float* lb) {
         TScope t(m, s, KC_NORM, ln_b);
@@ -307,15 +315,15 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
     for (int l = 0; l < m.cfg.layers; l++) {
         const LayerW& L = m.L[l];
         const bool last = l + 1 == m.cfg.layers;
-        gemm(3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
         {
             TScope t(m, s, KC_ATTN, at_b);
-            attend<Act>(m, s, QKV, O, ap);
+            attend<Act>(m, s, M, QKV, O, ap);
         }
         if (fuse_ln) {
-            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
+            gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
         } else {
-            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
             norm(L.n2g, L.n2b);
         }
         if (mlp_fused_ok(d, ff)) {
@@ -324,13 +332,13 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
                       last ? nullptr : m.L[l + 1].n1b, Hh);
             continue;
         }
-        gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
+        gemm(TO_GEMM_FF, ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
         if (last) {
-            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
         } else if (fuse_ln) {
-            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID_LN, m.L[l + 1].n1g, m.L[l + 1].n1b);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID_LN, m.L[l + 1].n1g, m.L[l + 1].n1b);
         } else {
-            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
             norm(m.L[l + 1].n1g, m.L[l + 1].n1b);
         }
     }
@@ -358,7 +366,7 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPla
             }
             {
                 TScope t(m, s, KC_ATTN, at_b);
-                attend<float>(m, s, QKV, O, ap);
+                attend<float>(m, s, M, QKV, O, ap);
             }
             {
                 TScope t(m, s, KC_GEMM, g * d);

@@ -414,6 +414,11 @@ int j2k_server_create(void* model, size_t budget, int max_active, void** out) {
         Model& m = *static_cast<Model*>(model);
         if (m.cfg.form != J2K_FORM_CELL_ST) fail(J2K_ERR_INVALID, "the server runs cell_st models");
         if (max_active <= 0) fail(J2K_ERR_INVALID, "max_active must be positive");
+        if (m.tune.enabled) {
+            // tune before the worker starts, up to the largest pack the budget admits
+            const size_t cap = budget / bytes_per_token(m);
+            autotune(m, (long)std::min<size_t>(cap, size_t(1) << 20));
+        }
         auto s = std::make_unique<Server>(m, budget, max_active);
         s->start();
         *out = s.release();

@@ -185,6 +185,15 @@ class Model:
              T, ny, nx, max_iter, tol, ctypes.byref(it), ctypes.byref(r), ctypes.byref(v), ctypes.c_void_p(stream))
         return it.value, r.value, v.value
 
+    def autotune(self, max_tokens: int = 0) -> dict:
+        """Tune launch shapes up to `max_tokens` packed tokens (0: only report);
+        returns the tuning table (see j2k_model_autotune)."""
+        import json
+
+        buf = ctypes.create_string_buffer(1 << 16)
+        call("j2k_model_autotune", self._h, int(max_tokens), buf, len(buf))
+        return json.loads(buf.value.decode())
+
 
 GpuModel = Model
 

@@ -5,6 +5,7 @@ oracle, and the continuous-batching server (each result the same alone and
 in the mix)."""
 
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -166,6 +167,81 @@ def test_wide_model_5x5x3_stencil(precision):
         # an indexing error would show as O(1).
         assert rel < 5e-3
     assert vg <= 1e-6
+
+
+def _model_env(st, cj, precision, **env):
+    """A model created with the given environment (the tuning switches are
+    read at creation)."""
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return j.Model.from_bytes(st, cj, precision=precision)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+SPECS = ["gemm_bm=64,attn_kv_mode=0", "gemm_bm=128,attn_kv_mode=1", "gemm_bm=64,attn_kv_mode=1"]
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16", "bf16x2"])
+@pytest.mark.parametrize("spec", SPECS)
+def test_launch_choices_do_not_change_results(model20_cell, precision, spec):
+    """Every launch shape the autotuner may pick (GEMM block height, K/V
+    staging mode of the attention) gives bit-identical iterates."""
+    st, cj = model20_cell
+    base = _model_env(st, cj, precision, J2K_AUTOTUNE="0")
+    forced = _model_env(st, cj, precision, J2K_TUNE_FORCE=spec)
+    T, ny, nx = 8, 32, 32
+    g = ck.drops_grid(T, ny, nx, 3, 17)
+    k = ck.mask_of("forward", T, ny, nx)
+    x0 = O.fill_blank(g, k)
+    xa, ia, ra, _ = base.iterate(x0, k, META, max_iter=2, tol=0.0)
+    xb, ib, rb, _ = forced.iterate(x0, k, META, max_iter=2, tol=0.0)
+    assert (ia, ra) == (ib, rb)
+    assert np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x2"])
+def test_wide_stencil_staging_modes_agree(precision):
+    """The 75-key stencil (prefetch plus batched remainder) in both staging modes."""
+    st, cj = ck.random_cell_st(dim=256, layers=1, heads=8, ff=1024, seed=1, radius=2)
+    a = _model_env(st, cj, precision, J2K_TUNE_FORCE="attn_kv_mode=0")
+    b = _model_env(st, cj, precision, J2K_TUNE_FORCE="attn_kv_mode=1")
+    T, ny, nx = 3, 24, 20
+    g = ck.drops_grid(T, ny, nx, 2, 23)
+    k = ck.mask_of("rewind", T, ny, nx)
+    x0 = O.fill_blank(g, k)
+    xa = a.iterate(x0, k, META, max_iter=1, tol=0.0)[0]
+    xb = b.iterate(x0, k, META, max_iter=1, tol=0.0)[0]
+    assert np.array_equal(xa, xb)
+
+
+def test_autotune_table_and_results(model20_cell):
+    """The tuner fills one entry per bucket with measured candidates, and a
+    tuned model iterates exactly like an untuned one."""
+    st, cj = model20_cell
+    tuned = _model_env(st, cj, "bf16x2")
+    rep = tuned.autotune(1 << 14)
+    print("autotune", rep)
+    assert rep["tuned_to_tokens"] == 1 << 14
+    assert [b["tokens"] for b in rep["buckets"]] == [4096, 8192, 16384]
+    for b in rep["buckets"]:
+        assert b["gemm_qkv_bm"] in (64, 128) and b["gemm_proj_bm"] in (64, 128)
+        assert b["attn_kv_mode"] in (0, 1)
+        assert min(b["gemm_qkv_bm_us"]) > 0 and min(b["attn_kv_mode_us"]) > 0
+    plain = _model_env(st, cj, "bf16x2", J2K_AUTOTUNE="0")
+    assert plain.autotune(0)["buckets"] == []
+    T, ny, nx = 8, 48, 48
+    g = ck.drops_grid(T, ny, nx, 3, 29)
+    k = ck.mask_of("reconstruction", T, ny, nx, seed=4)
+    x0 = O.fill_blank(g, k)
+    xa = tuned.iterate(x0, k, META, max_iter=2, tol=0.0)[0]
+    xb = plain.iterate(x0, k, META, max_iter=2, tol=0.0)[0]
+    assert np.array_equal(xa, xb)
 
 
 def test_grid_smaller_than_stencil_is_rejected(small):

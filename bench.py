@@ -236,6 +236,11 @@ def main():
     ap.add_argument("--headline", default="bf16x2", choices=["f32", "bf16", "bf16x2"],
                     help="served form reported as `value`")
     ap.add_argument("--no-oracle-error", action="store_true", help="skip the f64-oracle error check")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="after the runs, copy a buffer of this many MiB once (a known-size kernel that scales "
+                         "the profiler's HBM byte counters)")
+    ap.add_argument("--forms", default="f32,bf16,bf16x2",
+                    help="forms to run (comma separated; f32 and the headline are always included)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one, and this path has no multi-GPU form "
@@ -243,8 +248,18 @@ def main():
         sys.exit(2)
     st, cj, x0, known, meta = workload()
     runs, outs = {}, {}
-    for prec in ("f32", "bf16", "bf16x2"):
+    forms = [p for p in ("f32", "bf16", "bf16x2")
+             if p in args.forms.split(",") or p in ("f32", args.headline)]
+    for prec in forms:
         runs[prec], outs[prec] = run_precision(prec, st, cj, x0, known, meta, args.steps, args.warmup)
+    if args.copy_probe > 0:
+        import torch
+
+        a = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     # error of every form against the f64 oracle on this request (outside the timed region)
     errors = {}
     if not args.no_oracle_error:

@@ -90,16 +90,13 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
 #pragma unroll
             for (int q = 0; q < NA; q++) {
                 const int v = tid + 256 * q, row = v >> 4, kv = v & 15;
-                const uint32_t w[4] = {pa[q].x, pa[q].y, pa[q].z, pa[q].w};
-                uint32_t hi[4], lo[4];
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const float f = __uint_as_float(w[e]);
-                    hi[e] = f2bf(f);
-                    lo[e] = f2bf(f - bf2f((uint16_t)hi[e]));
-                }
-                *reinterpret_cast<uint2*>(&Ah[row * TLD + kv * 4]) = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));
-                *reinterpret_cast<uint2*>(&Al[row * TLD + kv * 4]) = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));
+                const float f0 = __uint_as_float(pa[q].x), f1 = __uint_as_float(pa[q].y);
+                const float f2 = __uint_as_float(pa[q].z), f3 = __uint_as_float(pa[q].w);
+                const uint32_t h01 = f2bf2(f0, f1), h23 = f2bf2(f2, f3);
+                const uint32_t l01 = f2bf2(f0 - __uint_as_float(h01 << 16), f1 - __uint_as_float(h01 & 0xffff0000u));
+                const uint32_t l23 = f2bf2(f2 - __uint_as_float(h23 << 16), f3 - __uint_as_float(h23 & 0xffff0000u));
+                *reinterpret_cast<uint2*>(&Ah[row * TLD + kv * 4]) = make_uint2(h01, h23);
+                *reinterpret_cast<uint2*>(&Al[row * TLD + kv * 4]) = make_uint2(l01, l23);
             }
         }
 #pragma unroll
@@ -159,7 +156,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
                 for (int j = 0; j < J; j++) {
                     if (!colok[j]) continue;
                     float v = acc[i][j][r] + bcol[j];
-                    if (EPI == EPI_GELU) v = gelu(v);
+                    if (EPI == EPI_GELU) v = gelu_fast(v);
                     const long o = (long)m * N + n0 + wn + j * 16 + r16;
                     if (SPLIT) static_cast<float*>(Cv)[o] = v;
                     else static_cast<uint16_t*>(Cv)[o] = f2bf(v);

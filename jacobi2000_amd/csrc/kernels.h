@@ -30,11 +30,27 @@ void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const
 void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
                       float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H, int bm = 0);
 
-// ---- fused MLP: X += ff2(GELU(ff1(H))), then H = LN(X) if ln_g != null ------
-// dim == 128, ff % 128 == 0; H (bf16, or f32 with split) may alias Hout.
-bool mlp_fused_ok(int dim, int ff);
-void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const void* H, const uint16_t* W1t, const float* b1,
-               const uint16_t* W2t, const float* b2, float* X, const float* ln_g, const float* ln_b, void* Hout);
+// ---- row-panel kernel (dim == 128, ff % 128 == 0) --------------------------
+// Per 64-token panel: [X += O @ Wp + bp; H = LN2(X)] if Wp, else H = Hin;
+// X += ff2(GELU(ff1(H))); then, if Wqkv, QKV = LN1'(X) @ Wqkv + bqkv, else
+// if n1g, Hout = LN1'(X). Activations (O, Hin, Hout, QKV) are bf16, or f32
+// with split; weights are the bf16 [N, K] copies.
+struct RowsArgs {
+    const void* O = nullptr;
+    const uint16_t* Wp = nullptr;
+    const float *bp = nullptr, *n2g = nullptr, *n2b = nullptr;
+    const void* Hin = nullptr;
+    const uint16_t *W1 = nullptr, *W2 = nullptr;
+    const float *b1 = nullptr, *b2 = nullptr;
+    float* X = nullptr;
+    const float *n1g = nullptr, *n1b = nullptr;
+    const uint16_t* Wqkv = nullptr;
+    const float* bqkv = nullptr;
+    void* QKV = nullptr;
+    void* Hout = nullptr;
+};
+bool rows_fused_ok(int dim, int ff);
+void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

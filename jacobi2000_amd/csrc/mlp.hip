@@ -1,242 +1,317 @@
-// Fused MLP of a block (cpu.rs:308-312): X += ff2(GELU(ff1(H))) and, when
-// a next block follows, H = LN(X) with its norm1 — one kernel, the ff-wide
-// intermediate never leaves the CU.
+// Row-panel kernel for the d = 128 trunk: everything of a block
+// (cpu.rs:287-313) that is local to a token, fused over 64-token panels so
+// the panel's activations stay in LDS and registers:
 //
-// A block owns 64 tokens and all dim (= 128) output columns; its four waves
-// split the columns (32 each). The block's H rows are staged in LDS once. The
-// intermediate is produced 128 columns at a time: ff1 + bias + GELU into a
-// 64 x 128 LDS tile, which the ff2 accumulators consume right away as their
-// next K chunk. Weights (W1t [ff, dim], W2t [dim, ff], bf16, K-contiguous)
-// stream from L2 as 16-byte MFMA fragments, a chunk ahead of their use. v_mfma_f32_16x16x32_bf16, f32
-// accumulate. SPLIT: H is f32 and both the staged H and the intermediate are
-// held as bf16 hi + lo pairs (two MFMAs per tile), as in the split GEMM.
+//   PROJ   X += O @ Wp + bp, then H = LN2(X)         (else H comes from Hin)
+//          X += ff2(GELU(ff1(H)))                    (always)
+//   QKV    QKV = LN1'(X) @ Wqkv' + bqkv'             (next block's norm and qkv)
+//   LNOUT  Hout = LN1'(X)                            (when the qkv GEMM runs apart)
+//
+// With PROJ and QKV a layer is two launches: this kernel and the attention.
+// The same arithmetic, in the same order, as the tile GEMM epilogues and the
+// MLP alone, so every combination gives bit-identical results and the
+// autotuner may pick any of them.
+//
+// A block owns 64 tokens; its four waves split every 128-column product (32
+// columns each, 2 x 4 tiles of v_mfma_f32_16x16x32_bf16, f32 accumulate).
+// A operands (O, H, the GELU intermediate) sit in LDS as bf16 (or, SPLIT, as
+// a bf16 hi + lo pair of an f32 value, two MFMAs per tile). Weights
+// (K-contiguous bf16 [N, K]) stream from L2 as 16-byte fragments, loaded a
+// whole 128 x 128 chunk ahead of their use. The ff-wide intermediate is made
+// 128 columns at a time and consumed at once as the next K chunk of ff2.
+// Row statistics of the LayerNorms: each 16-column tile's values are reduced
+// by lane shuffles and the 8 tile sums added in column order (cpu.rs:49-65).
 #include "common.h"
 #include "kernels.h"
 
 namespace j2k {
 
+namespace {
+
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
-constexpr int MBM = 64, MD = 128, MLD = MD + 8;
+constexpr int RB = 64, RD = 128, RLD = RD + 8;
 
 __device__ __forceinline__ bf8v frag(const uint16_t* p) { return __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(p)); }
 
-template <bool SPLIT, bool LN>
-__global__ __launch_bounds__(256) void k_mlp_bf16(int M, int ff, const void* __restrict__ Hin,
-                                                  const uint16_t* __restrict__ W1t, const float* __restrict__ b1,
-                                                  const uint16_t* __restrict__ W2t, const float* __restrict__ b2,
-                                                  float* __restrict__ X, const float* __restrict__ lng,
-                                                  const float* __restrict__ lnb, void* __restrict__ Hout) {
-    __shared__ __align__(16) uint16_t Hh[MBM * MLD];
-    __shared__ __align__(16) uint16_t Hl[SPLIT ? MBM * MLD : 8];
-    __shared__ __align__(16) uint16_t Fh[MBM * MLD];
-    __shared__ __align__(16) uint16_t Fl[SPLIT ? MBM * MLD : 8];
-    __shared__ float red[4][MBM];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * MBM, wn = wave * 32;
-    const int r16 = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
+// Fragments of W rows n0 + wn + j*16 + r16 (j < 2), K columns k0 .. k0+128.
+__device__ __forceinline__ void load_w(bf8v (&w)[4][2], const uint16_t* Wt, long ldk, int n0, int k0, int wn, int r16,
+                                       int kq) {
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) w[k4][j] = frag(Wt + (long)(n0 + wn + j * 16 + r16) * ldk + k0 + k4 * 32 + kq);
+}
 
-    // stage the block's H rows
+// acc += A (64 x 128 in LDS, hi and lo) @ the loaded W chunk.
+template <bool SPLIT>
+__device__ __forceinline__ void mma_panel(f4v (&acc)[4][2], const uint16_t* Ah, const uint16_t* Al,
+                                          const bf8v (&w)[4][2], int r16, int kq) {
+#pragma unroll
+    for (int k4 = 0; k4 < 4; k4++) {
+        bf8v a[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) a[i] = frag(&Ah[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w[k4][j], acc[i][j], 0, 0, 0);
+        if (SPLIT) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = frag(&Al[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w[k4][j], acc[i][j], 0, 0, 0);
+        }
+    }
+}
+
+__device__ __forceinline__ void zero(f4v (&acc)[4][2]) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+}
+
+// v as an A operand at (row, col): bf16, and with SPLIT the bf16 remainder.
+template <bool SPLIT>
+__device__ __forceinline__ void put_a(uint16_t* Ah, uint16_t* Al, int row, int col, float v) {
+    const uint16_t h = f2bf(v);
+    Ah[row * RLD + col] = h;
+    if (SPLIT) Al[row * RLD + col] = f2bf(v - bf2f(h));
+}
+
+// 64 rows x 128 of global A (bf16, or f32 split into hi + lo) into LDS.
+template <bool SPLIT>
+__device__ __forceinline__ void stage_rows(const void* src, int m0, int M, uint16_t* Ah, uint16_t* Al, int tid) {
     if (!SPLIT) {
-        const uint16_t* A = static_cast<const uint16_t*>(Hin);
+        const uint16_t* A = static_cast<const uint16_t*>(src);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             const int v = tid + 256 * q, row = v >> 4, kv = v & 15, m = m0 + row;
-            const uint4 val = m < M ? *reinterpret_cast<const uint4*>(A + (long)m * MD + kv * 8) : make_uint4(0, 0, 0, 0);
-            *reinterpret_cast<uint4*>(&Hh[row * MLD + kv * 8]) = val;
+            const uint4 val = m < M ? *reinterpret_cast<const uint4*>(A + (long)m * RD + kv * 8) : make_uint4(0, 0, 0, 0);
+            *reinterpret_cast<uint4*>(&Ah[row * RLD + kv * 8]) = val;
         }
     } else {
-        const float* A = static_cast<const float*>(Hin);
+        const float* A = static_cast<const float*>(src);
 #pragma unroll
         for (int q = 0; q < 8; q++) {
             const int v = tid + 256 * q, row = v >> 5, kv = v & 31, m = m0 + row;
-            const float4 f = m < M ? *reinterpret_cast<const float4*>(A + (long)m * MD + kv * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
-            const float e[4] = {f.x, f.y, f.z, f.w};
-            uint32_t hi[4], lo[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                hi[i] = f2bf(e[i]);
-                lo[i] = f2bf(e[i] - bf2f((uint16_t)hi[i]));
-            }
-            *reinterpret_cast<uint2*>(&Hh[row * MLD + kv * 4]) = make_uint2(hi[0] | (hi[1] << 16), hi[2] | (hi[3] << 16));
-            *reinterpret_cast<uint2*>(&Hl[row * MLD + kv * 4]) = make_uint2(lo[0] | (lo[1] << 16), lo[2] | (lo[3] << 16));
+            const float4 f = m < M ? *reinterpret_cast<const float4*>(A + (long)m * RD + kv * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const uint32_t h01 = f2bf2(f.x, f.y), h23 = f2bf2(f.z, f.w);
+            const uint32_t l01 = f2bf2(f.x - __uint_as_float(h01 << 16), f.y - __uint_as_float(h01 & 0xffff0000u));
+            const uint32_t l23 = f2bf2(f.z - __uint_as_float(h23 << 16), f.w - __uint_as_float(h23 & 0xffff0000u));
+            *reinterpret_cast<uint2*>(&Ah[row * RLD + kv * 4]) = make_uint2(h01, h23);
+            *reinterpret_cast<uint2*>(&Al[row * RLD + kv * 4]) = make_uint2(l01, l23);
         }
     }
-    __syncthreads();
+}
 
-    f4v acc2[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) acc2[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
-
-    // Weight fragments of a whole 128-wide chunk are loaded one phase ahead:
-    // W2 of chunk c while ff1 of chunk c runs, W1 of chunk c+1 while GELU and
-    // ff2 of chunk c run, so their L2 latency hides behind the MFMAs.
-    bf8v w1[4][2], w2[4][2];
-#define J2K_LOAD_W1(c0_)                                                                                        \
-    _Pragma("unroll") for (int k4 = 0; k4 < 4; k4++) _Pragma("unroll") for (int j = 0; j < 2; j++)          \
-        w1[k4][j] = frag(W1t + (long)((c0_) + wn + j * 16 + r16) * MD + k4 * 32 + kq);
-#define J2K_LOAD_W2(c0_)                                                                                        \
-    _Pragma("unroll") for (int k4 = 0; k4 < 4; k4++) _Pragma("unroll") for (int j = 0; j < 2; j++)          \
-        w2[k4][j] = frag(W2t + (long)(wn + j * 16 + r16) * ff + (c0_) + k4 * 32 + kq);
-    J2K_LOAD_W1(0)
-    for (int c0 = 0; c0 < ff; c0 += MD) {
-        J2K_LOAD_W2(c0)
-        // ff1 for intermediate columns c0 + wn .. + 32 of this wave
-        f4v acc1[4][2];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++) acc1[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) {
-            const int ks = k4 * 32;
-            bf8v a[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = frag(&Hh[(i * 16 + r16) * MLD + ks + kq]);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w1[k4][j], acc1[i][j], 0, 0, 0);
-            if (SPLIT) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) a[i] = frag(&Hl[(i * 16 + r16) * MLD + ks + kq]);
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 2; j++) acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w1[k4][j], acc1[i][j], 0, 0, 0);
-            }
-        }
-        float bb[2];
-#pragma unroll
-        for (int j = 0; j < 2; j++) bb[j] = b1[c0 + wn + j * 16 + r16];
-        if (c0 + MD < ff) {
-            J2K_LOAD_W1(c0 + MD)
-        }
-        __syncthreads();  // every wave has finished reading the previous intermediate chunk
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const float f = gelu(acc1[i][j][r] + bb[j]);
-                    const int o = (i * 16 + rq + r) * MLD + wn + j * 16 + r16;
-                    const uint16_t h = f2bf(f);
-                    Fh[o] = h;
-                    if (SPLIT) Fl[o] = f2bf(f - bf2f(h));
-                }
-        __syncthreads();
-        // ff2: output columns wn .. + 32, K chunk c0 .. c0 + 128
-#pragma unroll
-        for (int k4 = 0; k4 < 4; k4++) {
-            const int ks = k4 * 32;
-            bf8v a[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = frag(&Fh[(i * 16 + r16) * MLD + ks + kq]);
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w2[k4][j], acc2[i][j], 0, 0, 0);
-            if (SPLIT) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) a[i] = frag(&Fl[(i * 16 + r16) * MLD + ks + kq]);
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-#pragma unroll
-                    for (int j = 0; j < 2; j++) acc2[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w2[k4][j], acc2[i][j], 0, 0, 0);
-            }
-        }
-    }
-#undef J2K_LOAD_W1
-#undef J2K_LOAD_W2
-
-    // residual
-    float bo[2];
-#pragma unroll
-    for (int j = 0; j < 2; j++) bo[j] = b2[wn + j * 16 + r16];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = m0 + i * 16 + rq + r;
-            if (m >= M) continue;
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                float* p = X + (long)m * MD + wn + j * 16 + r16;
-                const float v = *p + acc2[i][j][r] + bo[j];
-                *p = v;
-                acc2[i][j][r] = v;
-            }
-        }
-    if (!LN) return;
-    // LayerNorm of the updated rows (cpu.rs:49-65); a row spans 4 waves x 2 tiles x 16 lanes
-    float mean[4][4], inv[4][4];
+// LayerNorm statistics of the panel held in v (row i*16 + rq + r, column
+// wave*32 + j*16 + r16): per 16-column tile by lane shuffles, then the 8
+// tile sums in column order. Synchronises the block.
+__device__ __forceinline__ void ln_stats(const f4v (&v)[4][2], float (*red)[RB], int wave, int lane,
+                                         float (&mean)[4][4], float (&inv)[4][4]) {
+    const int r16 = lane & 15, rq = (lane >> 4) * 4;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                float s = 0.f;
+            for (int r = 0; r < 4; r++)
 #pragma unroll
                 for (int j = 0; j < 2; j++) {
-                    const float v = acc2[i][j][r];
-                    s += pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
+                    const float x = v[i][j][r];
+                    float s = pass == 0 ? x : (x - mean[i][r]) * (x - mean[i][r]);
+                    s += __shfl_xor(s, 1, 64);
+                    s += __shfl_xor(s, 2, 64);
+                    s += __shfl_xor(s, 4, 64);
+                    s += __shfl_xor(s, 8, 64);
+                    if (r16 == 0) red[wave * 2 + j][i * 16 + rq + r] = s;
                 }
-                s += __shfl_xor(s, 1, 64);
-                s += __shfl_xor(s, 2, 64);
-                s += __shfl_xor(s, 4, 64);
-                s += __shfl_xor(s, 8, 64);
-                if (r16 == 0) red[wave][i * 16 + rq + r] = s;
-            }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int row = i * 16 + rq + r;
-                const float tot = red[0][row] + red[1][row] + red[2][row] + red[3][row];
-                if (pass == 0) mean[i][r] = tot / (float)MD;
-                else inv[i][r] = 1.0f / sqrtf(tot / (float)MD + 1e-5f);
+                float tot = 0.f;
+#pragma unroll
+                for (int t = 0; t < RD / 16; t++) tot += red[t][row];
+                if (pass == 0) mean[i][r] = tot / (float)RD;
+                else inv[i][r] = 1.0f / sqrtf(tot / (float)RD + 1e-5f);
             }
         __syncthreads();
     }
+}
+
+template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT>
+__global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
+    __shared__ __align__(16) uint16_t Hh[RB * RLD];
+    __shared__ __align__(16) uint16_t Hl[SPLIT ? RB * RLD : 8];
+    __shared__ __align__(16) uint16_t Fh[RB * RLD];
+    __shared__ __align__(16) uint16_t Fl[SPLIT ? RB * RLD : 8];
+    __shared__ float red[RD / 16][RB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, wn = wave * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
+    float mean[4][4], inv[4][4];
+    bf8v w1[4][2], w2[4][2];
+    f4v xm[4][2];  // the residual stream of the panel
+
+    // ---- the residual stream, and H, the MLP's input
+    stage_rows<SPLIT>(PROJ ? a.O : a.Hin, m0, M, Hh, Hl, tid);
+    if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, r16, kq);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = m0 + i * 16 + rq + r;
+#pragma unroll
+            for (int j = 0; j < 2; j++) xm[i][j][r] = m < M ? a.X[(long)m * RD + wn + j * 16 + r16] : 0.f;
+        }
+    __syncthreads();
+    if (PROJ) {
+        f4v acc[4][2];
+        zero(acc);
+        mma_panel<SPLIT>(acc, Hh, Hl, w1, r16, kq);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const float b = a.bp[wn + j * 16 + r16];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b;
+        }
+        ln_stats(xm, red, wave, lane, mean, inv);  // its barriers also retire every read of O
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int n = wn + j * 16 + r16;
+            const float g = a.n2g[n], b = a.n2b[n];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+                    put_a<SPLIT>(Hh, Hl, i * 16 + rq + r, n, (xm[i][j][r] - mean[i][r]) * inv[i][r] * g + b);
+        }
+        __syncthreads();
+    }
+
+    // ---- MLP: 128 intermediate columns at a time
+    f4v acc2[4][2];
+    zero(acc2);
+    load_w(w1, a.W1, RD, 0, 0, wn, r16, kq);
+    for (int c0 = 0; c0 < ff; c0 += RD) {
+        load_w(w2, a.W2, ff, 0, c0, wn, r16, kq);  // in flight during ff1
+        f4v acc1[4][2];
+        zero(acc1);
+        mma_panel<SPLIT>(acc1, Hh, Hl, w1, r16, kq);
+        float bb[2];
+#pragma unroll
+        for (int j = 0; j < 2; j++) bb[j] = a.b1[c0 + wn + j * 16 + r16];
+        if (c0 + RD < ff) load_w(w1, a.W1, RD, c0 + RD, 0, wn, r16, kq);  // in flight during GELU and ff2
+        __syncthreads();  // every wave has finished reading the previous intermediate chunk
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) put_a<SPLIT>(Fh, Fl, i * 16 + rq + r, wn + j * 16 + r16, gelu_fast(acc1[i][j][r] + bb[j]));
+        __syncthreads();
+        mma_panel<SPLIT>(acc2, Fh, Fl, w2, r16, kq);
+    }
+    if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int n = wn + j * 16 + r16;
-        const float g = lng[n], b = lnb[n];
+        const float b = a.b2[wn + j * 16 + r16];
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + i * 16 + rq + r;
-                if (m >= M) continue;
-                const float h = (acc2[i][j][r] - mean[i][r]) * inv[i][r] * g + b;
-                if (SPLIT) static_cast<float*>(Hout)[(long)m * MD + n] = h;
-                else static_cast<uint16_t*>(Hout)[(long)m * MD + n] = f2bf(h);
+                const float v = xm[i][j][r] + acc2[i][j][r] + b;
+                xm[i][j][r] = v;
+                if (m < M) a.X[(long)m * RD + wn + j * 16 + r16] = v;
             }
+    }
+    if (!QKV && !LNOUT) return;
+
+    // ---- the next block's LN1, then its qkv product or H itself
+    ln_stats(xm, red, wave, lane, mean, inv);  // its barriers also retire every read of H
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int n = wn + j * 16 + r16;
+        const float g = a.n1g[n], b = a.n1b[n];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float h = (xm[i][j][r] - mean[i][r]) * inv[i][r] * g + b;
+                if (QKV) {
+                    put_a<SPLIT>(Hh, Hl, i * 16 + rq + r, n, h);
+                } else {
+                    const int m = m0 + i * 16 + rq + r;
+                    if (m >= M) continue;
+                    if (SPLIT) static_cast<float*>(a.Hout)[(long)m * RD + n] = h;
+                    else static_cast<uint16_t*>(a.Hout)[(long)m * RD + n] = f2bf(h);
+                }
+            }
+    }
+    if (!QKV) return;
+    __syncthreads();
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        f4v acc[4][2];
+        zero(acc);
+        mma_panel<SPLIT>(acc, Hh, Hl, w1, r16, kq);
+        if (c < 2) load_w(w1, a.Wqkv, RD, (c + 1) * RD, 0, wn, r16, kq);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int n = c * RD + wn + j * 16 + r16;
+            const float b = a.bqkv[n];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m = m0 + i * 16 + rq + r;
+                    if (m >= M) continue;
+                    const float v = acc[i][j][r] + b;
+                    if (SPLIT) static_cast<float*>(a.QKV)[(long)m * 3 * RD + n] = v;
+                    else static_cast<uint16_t*>(a.QKV)[(long)m * 3 * RD + n] = f2bf(v);
+                }
+        }
     }
 }
 
-bool mlp_fused_ok(int dim, int ff) { return dim == MD && ff % MD == 0; }
+}  // namespace
 
-void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const void* H, const uint16_t* W1t, const float* b1,
-               const uint16_t* W2t, const float* b2, float* X, const float* ln_g, const float* ln_b, void* Hout) {
+bool rows_fused_ok(int dim, int ff) { return dim == RD && ff % RD == 0; }
+
+void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a) {
     if (M <= 0) return;
-    if (!mlp_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "fused MLP needs dim == 128 and ff a multiple of 128");
-    dim3 grid(cdiv(M, MBM)), block(256);
-    const bool ln = ln_g != nullptr;
-    if (split) {
-        if (ln) hipLaunchKernelGGL((k_mlp_bf16<true, true>), grid, block, 0, s, M, ff, H, W1t, b1, W2t, b2, X, ln_g, ln_b, Hout);
-        else hipLaunchKernelGGL((k_mlp_bf16<true, false>), grid, block, 0, s, M, ff, H, W1t, b1, W2t, b2, X, ln_g, ln_b, Hout);
-    } else {
-        if (ln) hipLaunchKernelGGL((k_mlp_bf16<false, true>), grid, block, 0, s, M, ff, H, W1t, b1, W2t, b2, X, ln_g, ln_b, Hout);
-        else hipLaunchKernelGGL((k_mlp_bf16<false, false>), grid, block, 0, s, M, ff, H, W1t, b1, W2t, b2, X, ln_g, ln_b, Hout);
+    if (!rows_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "fused row kernel needs dim == 128 and ff a multiple of 128");
+    const bool proj = a.Wp != nullptr, qkv = a.Wqkv != nullptr, lnout = !qkv && a.n1g != nullptr;
+    if (proj ? !a.O : !a.Hin) fail(J2K_ERR_INVALID, "fused row kernel: missing input");
+    dim3 grid(cdiv(M, RB)), block(256);
+#define J2K_ROWS(S, P, Q, L)                                                   \
+    if (split == S && proj == P && qkv == Q && lnout == L) {                   \
+        hipLaunchKernelGGL((k_rows<S, P, Q, L>), grid, block, 0, s, M, ff, a); \
+        LAUNCHCHK("rows_fused");                                               \
+        return;                                                                \
     }
-    LAUNCHCHK("mlp_fused");
+    J2K_ROWS(true, true, true, false)
+    J2K_ROWS(true, true, false, false)
+    J2K_ROWS(true, false, false, true)
+    J2K_ROWS(true, false, false, false)
+    J2K_ROWS(false, true, true, false)
+    J2K_ROWS(false, true, false, false)
+    J2K_ROWS(false, false, false, true)
+    J2K_ROWS(false, false, false, false)
+#undef J2K_ROWS
+    fail(J2K_ERR_INVALID, "fused row kernel: unsupported stage combination");
 }
 
 }  // namespace j2k

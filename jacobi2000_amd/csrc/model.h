@@ -51,7 +51,8 @@ enum TuneOp : int {
     TO_GEMM_PROJ = 1,  // block height of the attention-output GEMM
     TO_GEMM_FF = 2,    // block height of ff1 / ff2 when the MLP is not fused
     TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode)
-    TO_COUNT = 4
+    TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), 1 fused
+    TO_COUNT = 5
 };
 struct Tuning {
     static constexpr int NB = 31;
@@ -132,7 +133,8 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
 // Results do not depend on the choices (they only change the launch shape).
 void autotune(Model& m, long max_tokens);
 // Sets fixed choices for every bucket and turns tuning off ("gemm_bm=64",
-// "attn_kv_mode=1", comma separated; unnamed ops keep their defaults).
+// "attn_kv_mode=1", "rows_fused=1", comma separated; unnamed ops keep their
+// defaults).
 void force_tuning(Model& m, const char* spec);
 // The tuning table as JSON text.
 std::string tuning_json(const Model& m);

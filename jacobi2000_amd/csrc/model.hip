@@ -284,19 +284,23 @@ void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPl
 
 // The trunk in the served forms: bf16 weights on the tiled MFMA GEMM, with
 // activations in bf16 (Act = uint16_t) or kept f32 and split into bf16 pairs
-// inside the GEMM (Act = float). With dim == 128 the proj and ff2 GEMMs also
-// apply the next LayerNorm in their epilogue, so the stream is read once per
-// residual update.
+// inside the GEMM (Act = float). With dim == 128 the token-local part of a
+// block runs in the row-panel kernel, in one of two arrangements of the same
+// arithmetic (the autotuner picks): fused, where one launch takes O to the
+// next block's QKV (proj, LN2, MLP, LN1', qkv), or apart, where the proj and
+// qkv GEMMs run on their own and the row kernel does the MLP and LN1'.
+// Other widths use the tile GEMMs with LayerNorm epilogues or launches.
 template <class Act>
 void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
     constexpr bool split = sizeof(Act) == 4;
-    const int d = m.cfg.dim, ff = m.cfg.ff_inner;
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
     Act* QKV = m.get<Act>(S_QKV, (size_t)M * 3 * d);
     Act* O = m.get<Act>(S_O, (size_t)M * d);
     Act* F = m.get<Act>(S_F, (size_t)M * ff);
     const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
-    const bool fuse_ln = d == 128;
+    const bool rows = rows_fused_ok(d, ff);
+    const bool fused = rows && m.tune.choice(TO_ROWS, M) == 1;
     auto gemm = [&](int op, int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb) {
         TScope t(m, s, KC_GEMM, g / d * N * K);
@@ -312,34 +316,63 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
         else layernorm(s, M, d, X, lg, lb, nullptr, (uint16_t*)Hh);
     };
     if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
-    for (int l = 0; l < m.cfg.layers; l++) {
+    gemm(TO_GEMM_QKV, 3 * d, d, Hh, m.L[0].wqkv_t, m.L[0].bqkv, QKV, EPI_STORE, nullptr, nullptr);
+    for (int l = 0; l < nl; l++) {
         const LayerW& L = m.L[l];
-        const bool last = l + 1 == m.cfg.layers;
-        gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;
+        if (l > 0 && !fused) gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
         {
             TScope t(m, s, KC_ATTN, at_b);
             attend<Act>(m, s, M, QKV, O, ap);
         }
-        if (fuse_ln) {
+        if (rows) {
+            RowsArgs a;
+            a.W1 = L.w1_t;
+            a.b1 = L.b1;
+            a.W2 = L.w2_t;
+            a.b2 = L.b2;
+            a.X = X;
+            if (next) {
+                a.n1g = next->n1g;
+                a.n1b = next->n1b;
+            }
+            double work = 2.0 * g * ff;
+            if (fused) {
+                a.O = O;
+                a.Wp = L.wp_t;
+                a.bp = L.bp;
+                a.n2g = L.n2g;
+                a.n2b = L.n2b;
+                work += g * d;
+                if (next) {
+                    a.Wqkv = next->wqkv_t;
+                    a.bqkv = next->bqkv;
+                    a.QKV = QKV;
+                    work += g * 3 * d;
+                }
+            } else {
+                gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
+                a.Hin = Hh;
+                a.Hout = Hh;
+            }
+            TScope t(m, s, KC_GEMM, work);
+            rows_fused(s, M, d, ff, split, a);
+            continue;
+        }
+        if (d == 128) {
             gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
         } else {
             gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
             norm(L.n2g, L.n2b);
         }
-        if (mlp_fused_ok(d, ff)) {
-            TScope t(m, s, KC_GEMM, 2.0 * g * ff);
-            mlp_fused(s, M, d, ff, split, Hh, L.w1_t, L.b1, L.w2_t, L.b2, X, last ? nullptr : m.L[l + 1].n1g,
-                      last ? nullptr : m.L[l + 1].n1b, Hh);
-            continue;
-        }
         gemm(TO_GEMM_FF, ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
-        if (last) {
+        if (!next) {
             gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
-        } else if (fuse_ln) {
-            gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID_LN, m.L[l + 1].n1g, m.L[l + 1].n1b);
+        } else if (d == 128) {
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID_LN, next->n1g, next->n1b);
         } else {
             gemm(TO_GEMM_FF, d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
-            norm(m.L[l + 1].n1g, m.L[l + 1].n1b);
+            norm(next->n1g, next->n1b);
         }
     }
 }

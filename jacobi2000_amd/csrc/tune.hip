@@ -8,6 +8,7 @@
 // choice cannot change a result:
 //   qkv / proj / ff GEMMs   block height 64 or 128 rows
 //   cell attention          K and V staged together, or K then V
+//   d = 128 token-local     proj / qkv GEMMs apart, or fused with the MLP
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -86,7 +87,7 @@ void tune_bucket(Model& m, int b, Events& ev) {
         m.tune.us[TO_GEMM_PROJ][b][c] = time_us(s, ev, [&] {
             gemm(d, d, O, L.wp_t, L.bp, nullptr, fuse_ln ? EPI_RESID_LN : EPI_RESID, L.n2g, L.n2b, bm);
         });
-        if (!mlp_fused_ok(d, ff))
+        if (!rows_fused_ok(d, ff))
             m.tune.us[TO_GEMM_FF][b][c] = time_us(s, ev, [&] {
                 gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr, bm);
                 gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr, bm);
@@ -116,9 +117,46 @@ void tune_bucket(Model& m, int b, Events& ev) {
             });
         }
     }
-    for (int op = 0; op < TO_COUNT; op++) {
+    auto settle = [&](int op) {
         const float* u = m.tune.us[op][b];
         if (u[0] > 0.f || u[1] > 0.f) m.tune.pick[op][b] = (u[1] > 0.f && (u[0] <= 0.f || u[1] < u[0])) ? 1 : 0;
+    };
+    for (int op = 0; op < TO_COUNT; op++) settle(op);
+
+    if (rows_fused_ok(d, ff) && m.cfg.layers > 1) {
+        // the token-local work between two attentions, apart (with the block
+        // heights just chosen) or fused into one row-panel launch
+        const LayerW& N1 = m.L[1];
+        const int bq = GEMM_BM[m.tune.pick[TO_GEMM_QKV][b]], bp = GEMM_BM[m.tune.pick[TO_GEMM_PROJ][b]];
+        RowsArgs r;
+        r.W1 = L.w1_t;
+        r.b1 = L.b1;
+        r.W2 = L.w2_t;
+        r.b2 = L.b2;
+        r.X = X;
+        r.n1g = N1.n1g;
+        r.n1b = N1.n1b;
+        m.tune.us[TO_ROWS][b][0] = time_us(s, ev, [&] {
+            RowsArgs a = r;
+            a.Hin = Hh;
+            a.Hout = Hh;
+            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b, bp);
+            rows_fused(s, (int)M, d, ff, split, a);
+            gemm(3 * d, d, Hh, N1.wqkv_t, N1.bqkv, QKV, EPI_STORE, nullptr, nullptr, bq);
+        });
+        m.tune.us[TO_ROWS][b][1] = time_us(s, ev, [&] {
+            RowsArgs a = r;
+            a.O = O;
+            a.Wp = L.wp_t;
+            a.bp = L.bp;
+            a.n2g = L.n2g;
+            a.n2b = L.n2b;
+            a.Wqkv = N1.wqkv_t;
+            a.bqkv = N1.bqkv;
+            a.QKV = QKV;
+            rows_fused(s, (int)M, d, ff, split, a);
+        });
+        settle(TO_ROWS);
     }
 }
 
@@ -142,12 +180,12 @@ int Tuning::choice(int op, long M) const {
     if (b < B_MIN) b = B_MIN;
     const int c = pick[op][b];
     if (c < 0) return -1;
-    return op == TO_ATTN ? c : GEMM_BM[c];
+    return op == TO_ATTN || op == TO_ROWS ? c : GEMM_BM[c];
 }
 
 void force_tuning(Model& m, const char* spec) {
-    // "gemm_bm=64|128", "attn_kv_mode=0|1", comma separated; every bucket
-    int bm = -1, mode = -1;
+    // "gemm_bm=64|128", "attn_kv_mode=0|1", "rows_fused=0|1", comma separated; every bucket
+    int bm = -1, mode = -1, rows = -1;
     std::string t(spec);
     size_t pos = 0;
     while (pos <= t.size()) {
@@ -160,6 +198,7 @@ void force_tuning(Model& m, const char* spec) {
             const int v = std::atoi(kv.c_str() + eq + 1);
             if (k == "gemm_bm" && (v == 64 || v == 128)) bm = v == 64 ? 0 : 1;
             else if (k == "attn_kv_mode" && (v == KV_JOINT || v == KV_SPLIT)) mode = v;
+            else if (k == "rows_fused" && (v == 0 || v == 1)) rows = v;
             else fail(J2K_ERR_INVALID, "J2K_TUNE_FORCE: unknown setting '" + kv + "'");
         }
         pos = end + 1;
@@ -167,6 +206,7 @@ void force_tuning(Model& m, const char* spec) {
     for (int b = 0; b < Tuning::NB; b++) {
         if (bm >= 0) m.tune.pick[TO_GEMM_QKV][b] = m.tune.pick[TO_GEMM_PROJ][b] = m.tune.pick[TO_GEMM_FF][b] = bm;
         if (mode >= 0) m.tune.pick[TO_ATTN][b] = mode;
+        if (rows >= 0) m.tune.pick[TO_ROWS][b] = rows;
     }
     m.tune.tuned_to = B_MAX;
     m.tune.enabled = false;
@@ -193,7 +233,7 @@ void autotune(Model& m, long max_tokens) {
 }
 
 std::string tuning_json(const Model& m) {
-    static const char* names[TO_COUNT] = {"gemm_qkv_bm", "gemm_proj_bm", "gemm_ff_bm", "attn_kv_mode"};
+    static const char* names[TO_COUNT] = {"gemm_qkv_bm", "gemm_proj_bm", "gemm_ff_bm", "attn_kv_mode", "rows_fused"};
     std::string out = "{\"tuned_to_tokens\": ";
     out += std::to_string(m.tune.tuned_to < 0 ? 0L : 1L << m.tune.tuned_to);
     out += ", \"buckets\": [";
@@ -206,7 +246,7 @@ std::string tuning_json(const Model& m) {
         for (int op = 0; op < TO_COUNT; op++) {
             const int c = m.tune.pick[op][b];
             if (c < 0) continue;
-            const int v = op == TO_ATTN ? c : GEMM_BM[c];
+            const int v = op == TO_ATTN || op == TO_ROWS ? c : GEMM_BM[c];
             std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f]", names[op], v, names[op],
                           m.tune.us[op][b][0], m.tune.us[op][b][1]);
             out += buf;

@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 T, NY, NX = 16, 64, 64
 CELLS = T * NY * NX
 PEAK = {"mfma_f32": 157.3e12, "mfma_bf16": 2.5e15, "hbm": 8.0e12}
-CLASSES = ["gemm", "attention", "layernorm", "projection"]
+CLASSES = ["gemm", "attention", "layernorm", "projection", "rows", "rows_tail"]
 
 
 def workload():
@@ -68,7 +68,7 @@ def run_precision(prec, st, cj, x0, known, meta, steps, warmup):
     for _ in range(warmup):
         one()
     torch.cuda.synchronize()
-    timing = (ctypes.c_double * 12)()
+    timing = (ctypes.c_double * (3 * len(CLASSES)))()
     _lib.call("j2k_model_timing", model.handle, 1, None)
     lat, iters = [], []
     torch.cuda.synchronize()
@@ -104,22 +104,85 @@ def run_precision(prec, st, cj, x0, known, meta, steps, warmup):
     }, out
 
 
-def roofline(res, prec):
+def note(msg):
+    """Progress line on stderr (the JSON line alone goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_symbol(cls, prec):
+    """Name prefix of the kernel a timing class measures, as the profiler shows it."""
+    split = "true" if prec == "bf16x2" else "false"
+    return {"gemm": "j2k::k_gemm_f32" if prec == "f32" else "j2k::k_tile_bf16",
+            "attention": "j2k::k_attn_cell", "layernorm": "j2k::k_layernorm", "projection": "j2k::k_cell_project",
+            "rows": f"k_rows<{split}, true, true, false>", "rows_tail": "k_rows<"}[cls]
+
+
+def roofline(res, prec, traffic=None):
+    """Roofline of the dominant kernel class (most event-timed time): GEMMs
+    against the MFMA peak of their precision, the rest (attention, the
+    row-panel kernel, norms, projection) against HBM with algorithmic bytes."""
     k = res["kernels"]
     dom = max(k, key=lambda n: k[n]["ms_total"])
     d = k[dom]
     per_launch_s = d["ms_total"] / max(d["launches"], 1) / 1e3
     work_per_launch = d["work"] / max(d["launches"], 1)
-    if dom == "gemm":
-        peak = PEAK["mfma_bf16" if prec == "bf16" else "mfma_f32"]
-        achieved = work_per_launch / per_launch_s
-        return {"kernel": "gemm_" + prec, "bound": "mfma", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                "avg_launch_us": per_launch_s * 1e6, "work_per_launch": work_per_launch}
     achieved = work_per_launch / per_launch_s
-    return {"kernel": dom + "_" + prec, "bound": "hbm", "achieved": achieved / 1e9, "peak": PEAK["hbm"] / 1e9,
-            "unit": "GB/s", "frac": achieved / PEAK["hbm"], "traffic": None,
-            "avg_launch_us": per_launch_s * 1e6, "work_per_launch": work_per_launch}
+    out = {"kernel": kernel_symbol(dom, prec), "class": dom, "avg_launch_us": per_launch_s * 1e6,
+           "launches": d["launches"], "work_per_launch": work_per_launch}
+    if dom == "gemm":
+        peak = PEAK["mfma_f32" if prec == "f32" else "mfma_bf16"]
+        out.update(bound="mfma", achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s", frac=achieved / peak)
+    else:
+        out.update(bound="hbm", achieved=achieved / 1e9, peak=PEAK["hbm"] / 1e9, unit="GB/s",
+                   frac=achieved / PEAK["hbm"])
+    out["traffic"] = traffic
+    return out
+
+
+def measure_traffic(args, symbol):
+    """HBM bytes per launch of `symbol` at the workload's size: two
+    rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
+    dispatch) on the headline form, each scaled by what the same counter reads
+    for a copy of a known 256 MiB buffer in the same run. Returns
+    (bytes, detail) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not on PATH"
+    child = [sys.executable, str(ROOT / "bench.py"), "--forms", args.headline, "--headline", args.headline,
+             "--steps", "1", "--warmup", "1", "--no-cpu-baseline", "--no-oracle-error", "--no-traffic",
+             "--stream", "0", "--copy-probe", "256", "--no-plain"]
+    probe_kib = 256 * 1024.0
+    got = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as td:
+            cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", td, "-o", "run", "--"] + child
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
+                                   timeout=240)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {ctr} timed out"
+            files = sorted(Path(td).rglob("*counter_collection.csv"))
+            if not files:
+                return None, f"rocprofv3 --pmc {ctr} wrote no counters (exit {r.returncode})"
+            rows = [row for f in files for row in csv.DictReader(open(f)) if row["Counter_Name"] == ctr]
+        mine = [row for row in rows if symbol in row["Kernel_Name"]]
+        copies = [row for row in rows if "elementwise" in row["Kernel_Name"] or "copy" in row["Kernel_Name"].lower()]
+        if not mine or not copies:
+            return None, f"no {symbol} or copy dispatch in the {ctr} run"
+        gmax = max(int(row["Grid_Size"]) for row in mine)
+        vals = [float(row["Counter_Value"]) for row in mine if int(row["Grid_Size"]) == gmax]
+        copy_kib = max(float(row["Counter_Value"]) for row in copies)
+        got[ctr] = (sum(vals) / len(vals), copy_kib / probe_kib, len(vals), gmax)
+    fetch, fscale, n, grid = got["FETCH_SIZE"]
+    write, wscale, _, _ = got["WRITE_SIZE"]
+    total = (fetch / fscale + write / wscale) * 1024.0
+    return total, {"fetch_kib": fetch, "write_kib": write, "fetch_scale": fscale, "write_scale": wscale,
+                   "dispatches": n, "grid_size": grid, "probe": "256 MiB torch copy"}
 
 
 def cpu_baseline(st, cj, x0, known, meta, iters, seconds):
@@ -231,7 +294,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--stream", type=int, default=0, help="also run a mixed stream of this many requests")
+    ap.add_argument("--stream", type=int, default=512,
+                    help="also run the mixed request stream with this many requests (0: skip)")
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson arrival rate of the stream, requests/s")
     ap.add_argument("--headline", default="bf16x2", choices=["f32", "bf16", "bf16x2"],
                     help="served form reported as `value`")
@@ -241,6 +305,9 @@ def main():
                          "the profiler's HBM byte counters)")
     ap.add_argument("--forms", default="f32,bf16,bf16x2",
                     help="forms to run (comma separated; f32 and the headline are always included)")
+    ap.add_argument("--no-plain", action="store_true", help="skip the f32 plain form (profiling runs)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 --pmc runs that measure the dominant kernel's HBM bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box here has one, and this path has no multi-GPU form "
@@ -249,8 +316,9 @@ def main():
     st, cj, x0, known, meta = workload()
     runs, outs = {}, {}
     forms = [p for p in ("f32", "bf16", "bf16x2")
-             if p in args.forms.split(",") or p in ("f32", args.headline)]
+             if p in args.forms.split(",") or p == args.headline or (p == "f32" and not args.no_plain)]
     for prec in forms:
+        note(f"form {prec}: {args.warmup} warmup + {args.steps} timed requests")
         runs[prec], outs[prec] = run_precision(prec, st, cj, x0, known, meta, args.steps, args.warmup)
     if args.copy_probe > 0:
         import torch
@@ -275,7 +343,15 @@ def main():
                             "rel_l2_per_field": [float(np.linalg.norm(d[c]) / max(np.linalg.norm(r[c]), 1e-30))
                                                  for c in range(3)],
                             "max_abs_per_field": [float(np.abs(d[c]).max()) for c in range(3)]}
-    head, plain = runs[args.headline], runs["f32"]
+    head, plain = runs[args.headline], runs.get("f32", runs[args.headline])
+    traffic = None
+    if not args.no_traffic:
+        k = head["kernels"]
+        dom = max(k, key=lambda n: k[n]["ms_total"])
+        note(f"traffic: two rocprofv3 --pmc runs for {kernel_symbol(dom, args.headline)}")
+        t_bytes, t_detail = measure_traffic(args, kernel_symbol(dom, args.headline))
+        traffic = t_bytes
+        traffic_detail = t_detail
     summary = lambda r: {k: r[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "iterations", "resid",
                                           "energy_violation")}
     line = {
@@ -302,11 +378,14 @@ def main():
         "vs_fp32_plain": {p: r["value"] / plain["value"] for p, r in runs.items()},
         "error_vs_f64_oracle": errors,
         "kernels": {p: r["kernels"] for p, r in runs.items()},
-        "roofline": roofline(head, "bf16" if args.headline != "f32" else "f32"),
-        "roofline_by_form": {p: roofline(r, "f32" if p == "f32" else "bf16") for p, r in runs.items()},
+        "roofline": roofline(head, args.headline, traffic),
+        "roofline_by_form": {p: roofline(r, p) for p, r in runs.items()},
         "autotune": head.get("autotune"),
     }
+    if not args.no_traffic:
+        line["traffic_detail"] = traffic_detail
     if args.stream > 0:
+        note(f"stream: {args.stream} requests, served then plain")
         reqs = stream_requests(args.stream)
         s_served = run_stream(st, cj, meta, reqs, args.rate, served=True,
                               precision=args.headline if args.headline != "f32" else "bf16x2")
@@ -316,6 +395,7 @@ def main():
                           "served": s_served, "plain_fp32": s_plain,
                           "served_vs_plain": s_served["value"] / s_plain["value"]}
     if not args.no_cpu_baseline:
+        note(f"cpu baseline: about {args.cpu_seconds:g} s of the CPU oracle")
         line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)
     print(json.dumps(line), flush=True)
 

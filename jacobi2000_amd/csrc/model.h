@@ -25,7 +25,17 @@ enum Slot : int {
 
 // Kernel classes timed with HIP events when timing is on (the bench's
 // roofline figures): work is FLOPs for GEMM, algorithmic bytes otherwise.
-enum KClass : int { KC_GEMM = 0, KC_ATTN = 1, KC_NORM = 2, KC_PROJ = 3, KC_COUNT = 4 };
+// The row-panel kernel counts its activations read and written once:
+// KC_ROWS when it also makes the next block's QKV, KC_ROWS_TAIL otherwise.
+enum KClass : int {
+    KC_GEMM = 0,
+    KC_ATTN = 1,
+    KC_NORM = 2,
+    KC_PROJ = 3,
+    KC_ROWS = 4,
+    KC_ROWS_TAIL = 5,
+    KC_COUNT = 6
+};
 
 struct KTimer {
     bool on = false;

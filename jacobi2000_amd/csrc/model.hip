@@ -336,26 +336,27 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
                 a.n1g = next->n1g;
                 a.n1b = next->n1b;
             }
-            double work = 2.0 * g * ff;
+            // algorithmic bytes: the panel's input, X read and written, the output
+            double bytes = (double)M * d * (sizeof(Act) + 8.0);
             if (fused) {
                 a.O = O;
                 a.Wp = L.wp_t;
                 a.bp = L.bp;
                 a.n2g = L.n2g;
                 a.n2b = L.n2b;
-                work += g * d;
                 if (next) {
                     a.Wqkv = next->wqkv_t;
                     a.bqkv = next->bqkv;
                     a.QKV = QKV;
-                    work += g * 3 * d;
+                    bytes += (double)M * 3 * d * sizeof(Act);
                 }
             } else {
                 gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
                 a.Hin = Hh;
                 a.Hout = Hh;
+                if (next) bytes += (double)M * d * sizeof(Act);
             }
-            TScope t(m, s, KC_GEMM, work);
+            TScope t(m, s, a.QKV ? KC_ROWS : KC_ROWS_TAIL, bytes);
             rows_fused(s, M, d, ff, split, a);
             continue;
         }

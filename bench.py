@@ -155,7 +155,7 @@ def measure_traffic(args, symbol):
         return None, "rocprofv3 not on PATH"
     child = [sys.executable, str(ROOT / "bench.py"), "--forms", args.headline, "--headline", args.headline,
              "--steps", "1", "--warmup", "1", "--no-cpu-baseline", "--no-oracle-error", "--no-traffic",
-             "--stream", "0", "--copy-probe", "256", "--no-plain"]
+             "--stream", "0", "--rollouts", "0", "--copy-probe", "256", "--no-plain"]
     probe_kib = 256 * 1024.0
     got = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -287,6 +287,71 @@ def run_stream(st, cj, meta, reqs, rate, served, precision="bf16x2"):
                 p50_ms=float(np.percentile(lat_ms, 50)), p99_ms=float(np.percentile(lat_ms, 99)), **extra)
 
 
+def run_rollouts(st, cj, meta, n, served, precision="bf16x2", ny=96, nx=96, window=8, ticks=50):
+    """Long rollouts: n forward requests on ny x nx grids, each chaining
+    `window`-slice windows to `ticks` slices. served: all submitted at once to
+    the server (chained on the device side, windows of different requests
+    packed together); otherwise the f32 plain form runs every window of every
+    rollout one at a time. Per-window latency and energy violation."""
+    import ctypes
+    import jacobi2000_amd as j
+    from jacobi2000_amd import _lib
+    from jacobi2000_amd import checkpoint as ck
+
+    states = [ck.drops_grid(1, ny, nx, 5, seed=3000 + i)[:, 0].copy() for i in range(n)]
+    lat, viol, iters = [], [], []
+    if served:
+        model = j.Model.from_bytes(st, cj, precision=precision)
+        h = ctypes.c_void_p()
+        _lib.call("j2k_server_create", model.handle, 128 << 30, 4096, ctypes.byref(h))
+        t0 = time.perf_counter()
+        ids = []
+        for s0 in states:
+            rid = ctypes.c_uint64()
+            _lib.call("j2k_server_submit_rollout", h, s0.ctypes.data_as(ctypes.c_void_p),
+                      meta.ctypes.data_as(ctypes.c_void_p), ny, nx, window, ticks, ctypes.byref(rid))
+            ids.append(rid.value)
+        _lib.call("j2k_server_drain", h)
+        span = time.perf_counter() - t0
+        for rid in ids:
+            cnt = ctypes.c_int()
+            la, vi, it = (ctypes.c_double * 64)(), (ctypes.c_double * 64)(), (ctypes.c_int * 64)()
+            _lib.call("j2k_server_windows", h, rid, 64, ctypes.byref(cnt), la, vi, it)
+            lat += list(la)[:cnt.value]
+            viol += list(vi)[:cnt.value]
+            iters += list(it)[:cnt.value]
+        peak = ctypes.c_int()
+        _lib.call("j2k_server_stats", h, None, None, ctypes.byref(peak), None)
+        _lib.call("j2k_server_destroy", h)
+        model.close()
+        extra = {"peak_active": peak.value}
+    else:
+        model = j.Model.from_bytes(st, cj, precision="f32")
+        t0 = time.perf_counter()
+        for s0 in states:
+            anchor, tt = s0, 0
+            while tt < ticks - 1:
+                T = min(window, ticks - tt)
+                g = np.ascontiguousarray(np.repeat(anchor[:, None], T, axis=1))
+                k = np.zeros((T, ny, nx), np.uint8)
+                k[0] = 1
+                ts = time.perf_counter()
+                x, it, _, v = model.iterate(g, k, meta)
+                lat.append(time.perf_counter() - ts)
+                viol.append(v)
+                iters.append(it)
+                anchor, tt = x[:, T - 1].copy(), tt + window - 1
+        span = time.perf_counter() - t0
+        model.close()
+        extra = {}
+    cells = n * ticks * ny * nx
+    lat_ms = np.array(lat) * 1e3
+    return dict(value=cells / span, seconds=span, rollouts=n, windows=len(lat), cells=cells,
+                mean_window_iterations=float(np.mean(iters)), max_energy_violation=float(max(viol)),
+                window_p50_ms=float(np.percentile(lat_ms, 50)), window_p99_ms=float(np.percentile(lat_ms, 99)),
+                **extra)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +362,8 @@ def main():
     ap.add_argument("--stream", type=int, default=512,
                     help="also run the mixed request stream with this many requests (0: skip)")
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson arrival rate of the stream, requests/s")
+    ap.add_argument("--rollouts", type=int, default=256,
+                    help="also run this many chained 96x96 rollouts (8-slice windows to 50 ticks; 0: skip)")
     ap.add_argument("--headline", default="bf16x2", choices=["f32", "bf16", "bf16x2"],
                     help="served form reported as `value`")
     ap.add_argument("--no-oracle-error", action="store_true", help="skip the f64-oracle error check")
@@ -394,6 +461,16 @@ def main():
                                       "{32,48,64,96}^2 x {8,16}, masks forward/rewind/reconstruction",
                           "served": s_served, "plain_fp32": s_plain,
                           "served_vs_plain": s_served["value"] / s_plain["value"]}
+    if args.rollouts > 0:
+        note(f"rollouts: {args.rollouts} x 96x96, 8-slice windows to 50 ticks, served then plain")
+        r_served = run_rollouts(st, cj, meta, args.rollouts, served=True,
+                                precision=args.headline if args.headline != "f32" else "bf16x2")
+        r_plain = run_rollouts(st, cj, meta, args.rollouts, served=False)
+        line["rollouts"] = {"workload": f"{args.rollouts} concurrent forward rollouts, 96x96 grids, 8-slice windows "
+                                        "chained to 50 ticks, energy violation checked per window",
+                            "unit": "grid-cells/s over the whole trajectories",
+                            "served": r_served, "plain_fp32": r_plain,
+                            "served_vs_plain": r_served["value"] / r_plain["value"]}
     if not args.no_cpu_baseline:
         note(f"cpu baseline: about {args.cpu_seconds:g} s of the CPU oracle")
         line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)

@@ -202,6 +202,21 @@ int j2k_server_create(void* model, size_t hbm_budget_bytes, int max_active, void
  * filled or not (the server fills them), known [T,ny,nx], meta [meta_dim]. */
 int j2k_server_submit(void* server, const float* grid, const uint8_t* known, const float* meta, int T, int ny, int nx,
                       uint64_t* out_id);
+/* Queues a chained rollout, the served form of the reference's rollout loop
+ * (crates/engine-cli/src/main.rs:117-127) run window-parallel in time:
+ * state [3,ny,nx] is tick 0; windows of `window` slices overlap by one
+ * slice (each window's last slice is the next one's known slice 0, the
+ * others start as copies of it) until `ticks` slices are covered. The result
+ * (poll's grid_out) is the trajectory [3,ticks,ny,nx]; its iters sum over
+ * windows, violation is the worst window's, converged means every window
+ * converged. The budget is charged one window. */
+int j2k_server_submit_rollout(void* server, const float* state, const float* meta, int ny, int nx, int window,
+                              int ticks, uint64_t* out_id);
+/* Per-window figures of a retired rollout: *count windows; for the first n,
+ * latency (seconds from the previous window's retirement, or from submit),
+ * relative energy violation, iterations. J2K_PENDING before retirement. */
+int j2k_server_windows(void* server, uint64_t id, int n, int* count, double* latency_s, double* violation,
+                       int* iters);
 /* Non-blocking: J2K_OK and the result (and the grid into grid_out if not
  * NULL) once retired, J2K_PENDING before. */
 int j2k_server_poll(void* server, uint64_t id, j2k_result* out, float* grid_out);

@@ -79,6 +79,8 @@ PROTOS = {
     "j2k_cell_forward_dev": [P, P, P, P, I, I, I, P, P],
     "j2k_server_create": [P, ctypes.c_size_t, I, PP],
     "j2k_server_submit": [P, P, P, P, I, I, I, ctypes.POINTER(ctypes.c_uint64)],
+    "j2k_server_submit_rollout": [P, P, P, I, I, I, I, ctypes.POINTER(ctypes.c_uint64)],
+    "j2k_server_windows": [P, ctypes.c_uint64, I, ctypes.POINTER(I), P, P, P],
     "j2k_server_poll": [P, ctypes.c_uint64, ctypes.POINTER(Result), P],
     "j2k_server_drain": [P],
     "j2k_server_stats": [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(I),

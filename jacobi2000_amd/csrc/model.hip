@@ -297,9 +297,9 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
     Act* QKV = m.get<Act>(S_QKV, (size_t)M * 3 * d);
     Act* O = m.get<Act>(S_O, (size_t)M * d);
-    Act* F = m.get<Act>(S_F, (size_t)M * ff);
-    const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const bool rows = rows_fused_ok(d, ff);
+    Act* F = rows ? nullptr : m.get<Act>(S_F, (size_t)M * ff);  // the row kernel keeps it on chip
+    const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const bool fused = rows && m.tune.choice(TO_ROWS, M) == 1;
     auto gemm = [&](int op, int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb) {

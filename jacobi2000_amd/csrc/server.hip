@@ -11,6 +11,13 @@
 // A request's result does not depend on the mix: every kernel computes a
 // token from its own row and its own request's neighbours, and every
 // reduction (mass, energy, residual) runs per slice in a fixed order.
+//
+// A rollout request (j2k_server_submit_rollout) chains windows of T slices
+// along time: window w's last slice becomes window w+1's known slice 0, its
+// other slices start as copies of it, and the request stays active (state
+// re-uploaded from the host copy) until `ticks` slices are covered. Each
+// window retires on its own residual like a plain request, and its energy
+// violation, iterations and completion time are kept per window.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -46,13 +53,41 @@ struct Request {
     float resid = 0.f;
     size_t bytes = 0;
     int base = -1;  // first token in the current packing (-1: not yet on the device)
+    int slice0 = 0;  // its first slice in the packing's slice table
     StallWatch watch;
+    // rollout requests: ticks > 0
+    int ticks = 0, window = 0, wi = 0;  // total slices, slices per window, windows done
+    std::vector<float> traj;            // [3, ticks, ny, nx]
+    std::vector<double> win_done, win_viol;
+    std::vector<int> win_iters;
+    bool converged_all = true;
 };
+
+// Slices covered by window w of a rollout (windows overlap by one slice).
+int window_slices(int ticks, int window, int w) { return std::min(window, ticks - w * (window - 1)); }
+int window_count(int ticks, int window) { return (ticks - 1 + window - 2) / (window - 1); }
+
+// Window input from an anchor slice [3, ny*nx]: every slice a copy of it,
+// slice 0 known (the blank fill of a forward request).
+void set_window(Request& r, const float* anchor, int T) {
+    const size_t n = (size_t)r.dims.ny * r.dims.nx;
+    r.dims.T = T;
+    r.grid.resize(3 * T * n);
+    r.known.assign(T * n, 0);
+    std::fill(r.known.begin(), r.known.begin() + n, (uint8_t)1);
+    for (int c = 0; c < 3; c++)
+        for (int t = 0; t < T; t++) std::memcpy(&r.grid[(c * T + t) * n], anchor + c * n, n * sizeof(float));
+    r.iters = 0;
+    r.resid = 0.f;
+    r.watch = StallWatch{};
+    r.base = -1;
+}
 
 // Device bytes one token of an active request holds: three float4 state
 // vectors, its request index, and its rows of every activation buffer.
 size_t bytes_per_token(const Model& m) {
-    const size_t d = m.cfg.dim, ff = m.cfg.ff_inner, e = m.bf16() ? 2 : 4;
+    const size_t d = m.cfg.dim, e = m.bf16() ? 2 : 4;
+    const size_t ff = m.cfg.precision != J2K_PREC_F32 && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner) ? 0 : m.cfg.ff_inner;
     return 3 * 16 + 4 + d * 4 + d * e + 3 * d * e + d * e + ff * e;
 }
 
@@ -160,6 +195,7 @@ struct Server {
             alt0 = dev_alloc<float>(4 * want);
             altc = dev_alloc<float>(4 * want);
         }
+        int slices = 0;  // the slice table lists each request's slices in order
         for (size_t i = 0; i < active.size(); i++) {
             Request& r = *active[i];
             const size_t nb = (size_t)tab.segs[i].base, T = (size_t)r.dims.tokens();
@@ -176,6 +212,8 @@ struct Server {
                 HIPCHK(hipMemcpyAsync(altc + 4 * nb, xc + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
             }
             r.base = (int)nb;
+            r.slice0 = slices;
+            slices += r.dims.T;
         }
         std::swap(x0, alt0);
         std::swap(xc, altc);
@@ -247,7 +285,7 @@ struct Server {
         else m.get<float>(S_H, M * d);
         m.get<char>(S_QKV, M * 3 * d * e);
         m.get<char>(S_O, M * d * e);
-        m.get<char>(S_F, M * ff * e);
+        if (m.cfg.precision == J2K_PREC_F32 || !rows_fused_ok(m.cfg.dim, m.cfg.ff_inner)) m.get<char>(S_F, M * ff * e);
         m.p_tables_batch = -1;
         if (active.size() > h_resid_cap) {
             if (h_resid) HIPCHK(hipHostFree(h_resid));
@@ -298,8 +336,7 @@ struct Server {
     void finish(size_t i) {
         Request& r = *active[i];
         hipStream_t s = m.stream;
-        size_t first = 0;
-        for (size_t j = 0; j < i; j++) first += (size_t)active[j]->dims.T;
+        const size_t first = (size_t)r.slice0;  // set by repack: later rewindows do not move it
         const int n = r.dims.ny * r.dims.nx;
         std::vector<Slice> sl;
         for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, 0});
@@ -321,6 +358,43 @@ struct Server {
         r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
         r.res.violation = v;
         r.res.done_s = now_s();
+    }
+
+    // Records the window just finished (r.grid holds it) into the trajectory
+    // and sets up the next one; false when the rollout is complete.
+    bool next_window(Request& r) {
+        const int T = r.dims.T;
+        const size_t n = (size_t)r.dims.ny * r.dims.nx;
+        const int t0 = r.wi * (r.window - 1);
+        for (int c = 0; c < 3; c++)
+            for (int t = 1; t < T; t++)
+                std::memcpy(&r.traj[((size_t)c * r.ticks + t0 + t) * n], &r.grid[((size_t)c * T + t) * n],
+                            n * sizeof(float));
+        r.win_done.push_back(r.res.done_s);
+        r.win_viol.push_back(r.res.violation);
+        r.win_iters.push_back(r.iters);
+        r.converged_all = r.converged_all && r.res.converged;
+        r.wi++;
+        const int total = window_count(r.ticks, r.window);
+        if (r.wi < total) {
+            std::vector<float> anchor(3 * n);
+            for (int c = 0; c < 3; c++) std::memcpy(&anchor[c * n], &r.grid[((size_t)c * T + T - 1) * n], n * sizeof(float));
+            set_window(r, anchor.data(), window_slices(r.ticks, r.window, r.wi));
+            return true;
+        }
+        // the result of a rollout is its trajectory; counters over all windows
+        r.grid.swap(r.traj);
+        r.dims.T = r.ticks;
+        int it = 0;
+        double v = 0;
+        for (size_t w = 0; w < r.win_iters.size(); w++) {
+            it += r.win_iters[w];
+            v = std::max(v, r.win_viol[w]);
+        }
+        r.res.iters = it;
+        r.res.violation = v;
+        r.res.converged = r.converged_all ? 1 : 0;
+        return false;
     }
 
     void loop() {
@@ -353,17 +427,23 @@ struct Server {
             HIPCHK(hipStreamSynchronize(m.stream));
             std::vector<bool> out(active.size(), false);
             bool any = false;
+            bool rewindowed = false;
             for (size_t i = 0; i < active.size(); i++) {
                 Request& r = *active[i];
                 r.iters++;
                 std::memcpy(&r.resid, &h_resid[i], 4);
                 const bool stalled = r.watch.update(r.resid) && m.cfg.stall_stop;
                 if (r.resid < m.cfg.tol || r.iters >= m.cfg.max_iter || stalled) {
-                    r.res.stalled = r.resid >= m.cfg.tol && stalled ? 1 : 0;
+                    r.res.stalled = r.res.stalled || (r.resid >= m.cfg.tol && stalled) ? 1 : 0;
                     finish(i);
+                    if (r.ticks > 0 && next_window(r)) {
+                        rewindowed = true;  // stays active with its next window
+                        continue;
+                    }
                     out[i] = any = true;
                 }
             }
+            if (rewindowed) packed = false;
             if (!any) continue;
             std::lock_guard<std::mutex> lk(mu);
             std::vector<std::unique_ptr<Request>> keep;
@@ -449,6 +529,61 @@ int j2k_server_submit(void* h, const float* grid, const uint8_t* known, const fl
         if (out_id) *out_id = r->id;
         s.pending.push_back(std::move(r));
         s.cv_work.notify_one();
+    });
+}
+
+int j2k_server_submit_rollout(void* h, const float* state, const float* meta, int ny, int nx, int window, int ticks,
+                              uint64_t* out_id) {
+    return guard([&] {
+        Server& s = server_of(h);
+        if (!state || !meta) fail(J2K_ERR_INVALID, "state and meta must be non-NULL");
+        if (window < 2) fail(J2K_ERR_INVALID, "a rollout window needs at least 2 slices");
+        if (ticks < 2) fail(J2K_ERR_INVALID, "a rollout needs at least 2 ticks");
+        CellDims d{window, ny, nx};
+        check_cell_dims(s.m, d);
+        if (window_slices(ticks, window, window_count(ticks, window) - 1) < 2)
+            fail(J2K_ERR_INVALID, "internal: last rollout window shorter than 2 slices");
+        auto r = std::make_unique<Request>();
+        r->dims = d;
+        r->ticks = ticks;
+        r->window = window;
+        r->meta.assign(meta, meta + s.m.cfg.meta_dim);
+        const size_t n = (size_t)ny * nx;
+        r->traj.assign(3 * (size_t)ticks * n, 0.f);
+        for (int c = 0; c < 3; c++) std::memcpy(&r->traj[(size_t)c * ticks * n], state + c * n, n * sizeof(float));
+        set_window(*r, state, window_slices(ticks, window, 0));
+        r->bytes = (size_t)d.tokens() * bytes_per_token(s.m);
+        if (r->bytes > s.budget) fail(J2K_ERR_OOM, "a window needs more device memory than the whole budget");
+        r->res.submit_s = now_s();
+        std::lock_guard<std::mutex> lk(s.mu);
+        if (s.worker_code) fail(s.worker_code, "server stopped: " + s.worker_error);
+        r->id = r->res.id = s.next_id++;
+        if (out_id) *out_id = r->id;
+        s.pending.push_back(std::move(r));
+        s.cv_work.notify_one();
+    });
+}
+
+int j2k_server_windows(void* h, uint64_t id, int n, int* count, double* latency_s, double* violation, int* iters) {
+    return guard([&] {
+        Server& s = server_of(h);
+        std::lock_guard<std::mutex> lk(s.mu);
+        auto it = s.done.find(id);
+        if (it == s.done.end()) {
+            if (s.worker_code) fail(s.worker_code, "server stopped: " + s.worker_error);
+            if (id == 0 || id >= s.next_id) fail(J2K_ERR_INVALID, "unknown request id");
+            throw Error(J2K_PENDING, "request still running");
+        }
+        const Request& r = *it->second;
+        if (r.ticks == 0) fail(J2K_ERR_INVALID, "not a rollout request");
+        const int w = (int)r.win_done.size();
+        if (count) *count = w;
+        for (int i = 0; i < std::min(n, w); i++) {
+            const double start = i == 0 ? r.res.submit_s : r.win_done[i - 1];
+            if (latency_s) latency_s[i] = r.win_done[i] - start;
+            if (violation) violation[i] = r.win_viol[i];
+            if (iters) iters[i] = r.win_iters[i];
+        }
     });
 }
 

@@ -64,12 +64,12 @@ void tune_bucket(Model& m, int b, Events& ev) {
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
     Act* QKV = m.get<Act>(S_QKV, (size_t)M * 3 * d);
     Act* O = m.get<Act>(S_O, (size_t)M * d);
-    Act* F = m.get<Act>(S_F, (size_t)M * ff);
+    Act* F = rows_fused_ok(d, ff) ? nullptr : m.get<Act>(S_F, (size_t)M * ff);
     HIPCHK(hipMemsetAsync(X, 0, (size_t)M * d * sizeof(float), s));
     HIPCHK(hipMemsetAsync(Hh, 0, (size_t)M * d * sizeof(Act), s));
     HIPCHK(hipMemsetAsync(QKV, 0, (size_t)M * 3 * d * sizeof(Act), s));
     HIPCHK(hipMemsetAsync(O, 0, (size_t)M * d * sizeof(Act), s));
-    HIPCHK(hipMemsetAsync(F, 0, (size_t)M * ff * sizeof(Act), s));
+    if (F) HIPCHK(hipMemsetAsync(F, 0, (size_t)M * ff * sizeof(Act), s));
 
     auto gemm = [&](int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb, int bm) {

@@ -313,6 +313,62 @@ def test_server_results_same_alone_and_in_the_mix(small, precision):
         _lib.call("j2k_server_destroy", h)
 
 
+def _windows_alone(model, state, window, ticks):
+    """A chained rollout run window by window through j2k_iterate."""
+    _, ny, nx = state.shape
+    traj = np.zeros((3, ticks, ny, nx), np.float32)
+    traj[:, 0] = state
+    anchor, t0, iters, viol = state, 0, [], []
+    while t0 < ticks - 1:
+        T = min(window, ticks - t0)
+        g = np.ascontiguousarray(np.repeat(anchor[:, None], T, axis=1))
+        k = np.zeros((T, ny, nx), np.uint8)
+        k[0] = 1
+        x, it, _, v = model.iterate(g, k, META)
+        traj[:, t0 + 1:t0 + T] = x[:, 1:T]
+        anchor, t0 = x[:, T - 1], t0 + window - 1
+        iters.append(it)
+        viol.append(v)
+    return traj, iters, viol
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x2"])
+def test_server_rollout_chains_windows(small, precision):
+    """Chained rollouts (window 4, 9 ticks: windows of 4, 4 and 3 slices),
+    three at once with plain requests in the mix, each equal to its windows
+    run one by one, every window within the energy bound."""
+    model = j.Model.from_bytes(*small, precision=precision)
+    ny, nx, W, ticks = 16, 12, 4, 9
+    states = [ck.drops_grid(1, ny, nx, 2, 40 + i)[:, 0].copy() for i in range(3)]
+    alone = [_windows_alone(model, st, W, ticks) for st in states]
+    h = _server(model, 1 << 34, 16)
+    try:
+        ids = []
+        for i, st in enumerate(states):
+            rid = ctypes.c_uint64()
+            _lib.call("j2k_server_submit_rollout", h, st.ctypes.data_as(ctypes.c_void_p),
+                      META.ctypes.data_as(ctypes.c_void_p), ny, nx, W, ticks, ctypes.byref(rid))
+            ids.append(rid.value)
+            g, k = _requests()[i]
+            _submit(h, g, k)
+        _lib.call("j2k_server_drain", h)
+        for rid, (traj, iters, viol) in zip(ids, alone):
+            res, out = _result(h, rid, (3, ticks, ny, nx))
+            n = ctypes.c_int()
+            lat, vio, its = (ctypes.c_double * 8)(), (ctypes.c_double * 8)(), (ctypes.c_int * 8)()
+            _lib.call("j2k_server_windows", h, rid, 8, ctypes.byref(n), lat, vio, its)
+            d = np.abs(out - traj).max()
+            print(f"rollout {rid}: windows {n.value} iters {list(its)[:n.value]} alone {iters} max_abs {d:.2e}")
+            assert n.value == 3 and list(its)[:3] == iters
+            assert d <= 1e-6
+            assert res.iters == sum(iters) and res.violation <= 1e-6
+            assert all(vio[w] <= 1e-6 and lat[w] > 0 for w in range(3))
+        with pytest.raises(j.J2KError):
+            _lib.call("j2k_server_windows", h, ids[0] + 1, 8, None, None, None, None)  # a plain request
+    finally:
+        _lib.call("j2k_server_destroy", h)
+
+
 def test_server_budget_admits_one_at_a_time(small):
     model = j.Model.from_bytes(*small)
     reqs = _requests()[:3]

@@ -51,6 +51,10 @@ extern "C" {
 /* served form, compensated: bf16 weights; activations kept f32 and fed to the
  * bf16 MFMA as a hi + lo bf16 pair (two MFMAs per tile), f32 accumulate */
 #define J2K_PREC_BF16X2 2
+/* FP8: the four block products on E4M3 codes (weights per-tensor scaled at
+ * load; activations quantized in the GEMM with static per-tensor scales that
+ * bound them), f32 accumulate; everything else f32. */
+#define J2K_PREC_FP8 3
 
 #define J2K_FORM_PATCH2D 0 /* the reference forward (crates/engine/src/cpu.rs:249-352) */
 #define J2K_FORM_CELL_ST 1 /* per-cell space-time tokens, fixed-point iteration     */

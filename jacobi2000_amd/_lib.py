@@ -17,7 +17,7 @@ OK = 0
 ERR_NO_ADAPTER, ERR_NO_DEVICE, ERR_LIMIT, ERR_MAP, ERR_POLL = 1, 2, 3, 4, 5
 ERR_BAD_CONFIG, ERR_MISSING_TENSOR, ERR_BAD_DTYPE, ERR_SHAPE, ERR_INVALID, ERR_OOM = 6, 7, 8, 9, 10, 11
 PENDING = 12
-PREC_F32, PREC_BF16, PREC_BF16X2 = 0, 1, 2
+PREC_F32, PREC_BF16, PREC_BF16X2, PREC_FP8 = 0, 1, 2, 3
 FORM_PATCH2D, FORM_CELL_ST = 0, 1
 
 _NAMES = {1: "NoAdapter", 2: "NoDevice", 3: "Limit", 4: "Map", 5: "Poll", 6: "BadConfig", 7: "MissingTensor",

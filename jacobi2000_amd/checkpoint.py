@@ -29,18 +29,113 @@ import numpy as np
 BLOCK_LINEARS = ("q", "k", "v", "proj", "ff1", "ff2")
 
 
-def write_safetensors(tensors: dict[str, np.ndarray]) -> bytes:
+def write_safetensors(tensors: dict[str, np.ndarray], dtypes: dict[str, str] | None = None) -> bytes:
     """Byte layout of crates/engine/src/weights.rs:29-48 (8-byte LE header
-    length, JSON header, LE F32 buffer), tensors in sorted name order."""
+    length, JSON header, LE buffer), tensors in sorted name order. F32 unless
+    `dtypes` names "BF16" (the array is rounded to nearest even) or
+    "F8_E4M3" (the array holds the uint8 codes) for a tensor."""
     header, off, blobs = {}, 0, []
     for name in sorted(tensors):
-        a = np.ascontiguousarray(tensors[name], dtype="<f4")
-        header[name] = {"dtype": "F32", "shape": list(a.shape), "data_offsets": [off, off + a.nbytes]}
+        dt = (dtypes or {}).get(name, "F32")
+        if dt == "F32":
+            a = np.ascontiguousarray(tensors[name], dtype="<f4")
+        elif dt == "BF16":
+            a = f32_to_bf16_bits(np.asarray(tensors[name], dtype=np.float32)).astype("<u2")
+        elif dt == "F8_E4M3":
+            a = np.ascontiguousarray(tensors[name], dtype=np.uint8)
+        else:
+            raise ValueError(f"dtype {dt}")
+        header[name] = {"dtype": dt, "shape": list(a.shape), "data_offsets": [off, off + a.nbytes]}
         off += a.nbytes
         blobs.append(a.tobytes())
     h = json.dumps(header, separators=(",", ":")).encode()
     h += b" " * ((8 - len(h) % 8) % 8)
     return struct.pack("<Q", len(h)) + h + b"".join(blobs)
+
+
+def f32_to_bf16_bits(x: np.ndarray) -> np.ndarray:
+    """bf16 bit patterns of f32 values, rounding to nearest even."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def e4m3_decode(codes: np.ndarray) -> np.ndarray:
+    """OCP FP8 E4M3FN codes -> float32: 1 sign, 4 exponent (bias 7), 3
+    mantissa bits; exponent 0 is subnormal (m/8 * 2^-6); S.1111.111 is NaN."""
+    c = np.asarray(codes, dtype=np.uint8).astype(np.int32)
+    e, m = (c >> 3) & 15, c & 7
+    mag = np.where(e == 0, np.ldexp(m.astype(np.float64), -9), np.ldexp(1.0 + m / 8.0, e - 7))
+    mag = np.where((e == 15) & (m == 7), np.nan, mag)
+    return np.where(c >> 7, -mag, mag).astype(np.float32)
+
+
+_E4M3_POS = e4m3_decode(np.arange(0x7F, dtype=np.uint8)).astype(np.float64)  # codes 0..126, ascending
+
+
+def e4m3_encode(x: np.ndarray) -> np.ndarray:
+    """float -> OCP E4M3FN codes, round to nearest (ties to the even code),
+    saturating to +-448; NaN -> 0x7F."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.minimum(np.abs(x), 448.0)
+    lo = np.clip(np.searchsorted(_E4M3_POS, a, side="right") - 1, 0, 0x7E)
+    hi = np.minimum(lo + 1, 0x7E)
+    dlo, dhi = a - _E4M3_POS[lo], _E4M3_POS[hi] - a
+    code = np.where((dhi < dlo) | ((dhi == dlo) & (lo % 2 == 1)), hi, lo).astype(np.uint8)
+    code = np.where(np.signbit(x), code | 0x80, code).astype(np.uint8)
+    return np.where(np.isnan(x), np.uint8(0x7F), code).astype(np.uint8)
+
+
+def convert_checkpoint(st: bytes, dtype: str) -> bytes:
+    """A copy of a checkpoint with the block weight matrices (q, k, v, proj,
+    ff1, ff2) stored as BF16, or as F8_E4M3 codes with a per-tensor scale
+    `<name>.scale` (amax / 448). The extension of the reference format that
+    libj2k reads; every other tensor stays F32."""
+    if dtype not in ("BF16", "F8_E4M3"):
+        raise ValueError("dtype must be BF16 or F8_E4M3")
+    t = read_safetensors(st)
+    out, dtypes = {}, {}
+    for name, a in t.items():
+        parts = name.split(".")
+        if len(parts) == 4 and parts[0] == "blocks" and parts[2] in BLOCK_LINEARS and parts[3] == "weight":
+            if dtype == "BF16":
+                out[name], dtypes[name] = a, "BF16"
+            else:
+                # f32 arithmetic, as libj2k quantizes an F32 checkpoint at load
+                amax = np.float32(np.abs(a).max())
+                scale = amax / np.float32(448.0) if amax > 0 else np.float32(1.0)
+                out[name], dtypes[name] = e4m3_encode((a.astype(np.float32) / scale).astype(np.float32)), "F8_E4M3"
+                out[name + ".scale"] = np.array([scale], np.float32)
+        else:
+            out[name] = a
+    return write_safetensors(out, dtypes)
+
+
+def read_safetensors(data: bytes) -> dict[str, np.ndarray]:
+    """Tensors of a checkpoint as float32 (BF16 widened, F8_E4M3 decoded
+    times `<name>.scale`; the scale tensors are dropped)."""
+    n = struct.unpack("<Q", data[:8])[0]
+    header = json.loads(data[8:8 + n])
+    raw, out = {}, {}
+    for name, v in header.items():
+        if name == "__metadata__":
+            continue
+        a, b = v["data_offsets"]
+        buf = data[8 + n + a:8 + n + b]
+        if v["dtype"] == "F32":
+            raw[name] = np.frombuffer(buf, dtype="<f4").reshape(v["shape"]).copy()
+        elif v["dtype"] == "BF16":
+            bits = np.frombuffer(buf, dtype="<u2").astype(np.uint32) << 16
+            raw[name] = bits.view(np.float32).reshape(v["shape"]).copy()
+        elif v["dtype"] == "F8_E4M3":
+            raw[name] = ("f8", np.frombuffer(buf, dtype=np.uint8).reshape(v["shape"]).copy())
+        else:
+            raise ValueError(f"{name}: dtype {v['dtype']}")
+    for name, a in raw.items():
+        if isinstance(a, tuple):
+            out[name] = e4m3_decode(a[1]) * raw[name + ".scale"].reshape(-1)[0]
+        elif not (name.endswith(".scale") and isinstance(raw.get(name[:-6]), tuple)):
+            out[name] = a
+    return out
 
 
 def cell_st_shapes(dim: int, layers: int, ff: int, in_features: int = 4, meta_dim: int = 5,

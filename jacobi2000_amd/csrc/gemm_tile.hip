@@ -30,6 +30,123 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 
 constexpr int TBN = 128, TBK = 64, TLD = TBK + 8;
 
+// Epilogue of a BM x 128 tile whose accumulators acc[i][j] (row wm + i*16 +
+// 4*(lane/16) + r, column wn + j*16 + lane%16) are spread over the block's
+// four waves: v = acc * alpha + bias, then store / GELU + store / residual
+// add into X / residual add and the LayerNorm of the updated rows. alpha is
+// a[n] * colscale[n] with colscale per output column (nullptr: 1). C and H
+// are f32 (OUTF32) or bf16.
+template <int EPI, bool OUTF32, int BM, int J>
+__device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const float* __restrict__ colscale, int M,
+                                              int N, int m0, int n0, int wm, int wn, int lane,
+                                              const float* __restrict__ bias, void* __restrict__ Cv,
+                                              float* __restrict__ X, const float* __restrict__ lng,
+                                              const float* __restrict__ lnb, void* __restrict__ Hv,
+                                              float (*red)[BM]) {
+    const int r16 = lane & 15;
+    float bcol[J], acol[J];
+    bool colok[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const int n = n0 + wn + j * 16 + r16;
+        colok[j] = n < N;
+        bcol[j] = colok[j] ? bias[n] : 0.f;
+        acol[j] = colscale && colok[j] ? a * colscale[n] : a;
+    }
+    if (EPI == EPI_STORE || EPI == EPI_GELU) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+                if (m >= M) continue;
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    if (!colok[j]) continue;
+                    float v = acc[i][j][r] * acol[j] + bcol[j];
+                    if (EPI == EPI_GELU) v = gelu_fast(v);
+                    const long o = (long)m * N + n0 + wn + j * 16 + r16;
+                    if (OUTF32) static_cast<float*>(Cv)[o] = v;
+                    else static_cast<uint16_t*>(Cv)[o] = f2bf(v);
+                }
+            }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+            if (m >= M) continue;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                if (!colok[j]) continue;
+                float* p = X + (long)m * N + n0 + wn + j * 16 + r16;
+                const float v = *p + acc[i][j][r] * acol[j] + bcol[j];
+                *p = v;
+                acc[i][j][r] = v;
+            }
+        }
+    if (EPI != EPI_RESID_LN) return;
+
+    // LayerNorm of the updated rows (cpu.rs:49-65). N == 128: a row's values
+    // sit in the 16 lanes of 8 column tiles, spread over the waves sharing
+    // its rows. Each tile's 16 values are reduced by lane shuffles and the 8
+    // tile sums added in column order, so the result is the same for every
+    // block height.
+    float mean[4][4], inv[4][4];
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    const float v = acc[i][j][r];
+                    float s = pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
+                    s += __shfl_xor(s, 1, 64);
+                    s += __shfl_xor(s, 2, 64);
+                    s += __shfl_xor(s, 4, 64);
+                    s += __shfl_xor(s, 8, 64);
+                    if (r16 == 0) red[wn / 16 + j][wm + i * 16 + (lane >> 4) * 4 + r] = s;
+                }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wm + i * 16 + (lane >> 4) * 4 + r;
+                float tot = 0.f;
+#pragma unroll
+                for (int w = 0; w < TBN / 16; w++) tot += red[w][row];
+                if (pass == 0) mean[i][r] = tot / (float)N;
+                else inv[i][r] = 1.0f / sqrtf(tot / (float)N + 1e-5f);
+            }
+        __syncthreads();
+    }
+    float gc[J], bc[J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        gc[j] = lng[wn + j * 16 + r16];
+        bc[j] = lnb[wn + j * 16 + r16];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+            if (m >= M) continue;
+#pragma unroll
+            for (int j = 0; j < J; j++) {
+                const float h = (acc[i][j][r] - mean[i][r]) * inv[i][r] * gc[j] + bc[j];
+                const long o = (long)m * N + wn + j * 16 + r16;
+                if (OUTF32) static_cast<float*>(Hv)[o] = h;
+                else static_cast<uint16_t*>(Hv)[o] = f2bf(h);
+            }
+        }
+}
+
 template <int EPI, bool SPLIT, int BM>
 __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const void* __restrict__ Av,
                                                       const uint16_t* __restrict__ Bt, const float* __restrict__ bias,
@@ -137,106 +254,93 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
         }
     }
 
-    float bcol[J];
-    bool colok[J];
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        const int n = n0 + wn + j * 16 + r16;
-        colok[j] = n < N;
-        bcol[j] = colok[j] ? bias[n] : 0.f;
-    }
-    if (EPI == EPI_STORE || EPI == EPI_GELU) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-                if (m >= M) continue;
-#pragma unroll
-                for (int j = 0; j < J; j++) {
-                    if (!colok[j]) continue;
-                    float v = acc[i][j][r] + bcol[j];
-                    if (EPI == EPI_GELU) v = gelu_fast(v);
-                    const long o = (long)m * N + n0 + wn + j * 16 + r16;
-                    if (SPLIT) static_cast<float*>(Cv)[o] = v;
-                    else static_cast<uint16_t*>(Cv)[o] = f2bf(v);
-                }
-            }
-        return;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-            if (m >= M) continue;
-#pragma unroll
-            for (int j = 0; j < J; j++) {
-                if (!colok[j]) continue;
-                float* p = X + (long)m * N + n0 + wn + j * 16 + r16;
-                const float v = *p + acc[i][j][r] + bcol[j];
-                *p = v;
-                acc[i][j][r] = v;
-            }
-        }
-    if (EPI != EPI_RESID_LN) return;
+    tile_epilogue<EPI, SPLIT, BM, J>(acc, 1.0f, nullptr, M, N, m0, n0, wm, wn, lane, bias, Cv, X, lng, lnb, Hv, red);
+}
 
-    // LayerNorm of the updated rows (cpu.rs:49-65). N == 128: a row's values
-    // sit in the 16 lanes of 8 column tiles, spread over the waves sharing
-    // its rows. Each tile's 16 values are reduced by lane shuffles and the 8
-    // tile sums added in column order, so the result is the same for every
-    // block height.
-    float mean[4][4], inv[4][4];
-#pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int j = 0; j < J; j++) {
-                    const float v = acc[i][j][r];
-                    float s = pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
-                    s += __shfl_xor(s, 1, 64);
-                    s += __shfl_xor(s, 2, 64);
-                    s += __shfl_xor(s, 4, 64);
-                    s += __shfl_xor(s, 8, 64);
-                    if (r16 == 0) red[wn / 16 + j][wm + i * 16 + (lane >> 4) * 4 + r] = s;
-                }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = wm + i * 16 + (lane >> 4) * 4 + r;
-                float tot = 0.f;
-#pragma unroll
-                for (int w = 0; w < TBN / 16; w++) tot += red[w][row];
-                if (pass == 0) mean[i][r] = tot / (float)N;
-                else inv[i][r] = 1.0f / sqrtf(tot / (float)N + 1e-5f);
-            }
-        __syncthreads();
-    }
-    float gc[J], bc[J];
-#pragma unroll
-    for (int j = 0; j < J; j++) {
-        gc[j] = lng[wn + j * 16 + r16];
-        bc[j] = lnb[wn + j * 16 + r16];
-    }
+// FP8 form: A is f32 and is quantized while staged, q = clamp(a * inv_sa,
+// +-448) rounded to OCP E4M3 (v_cvt_pk_fp8_f32); Bt holds E4M3 weight codes
+// [N, K]; v_mfma_f32_16x16x32_fp8_fp8 accumulates the exact code products in
+// f32, and the epilogue rescales by sa * colscale[n] (the weights'
+// per-tensor scale, per output column because the fused qkv matrix joins
+// three tensors). Same tiling as the bf16 kernel with 8-byte fragments.
+constexpr int T8LD = TBK + 16;  // bytes per LDS row
+
+template <int EPI, int BM>
+__global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const float* __restrict__ A, float inv_sa,
+                                                    float sa, const uint8_t* __restrict__ Bt,
+                                                    const float* __restrict__ colscale, const float* __restrict__ bias,
+                                                    float* __restrict__ C, float* __restrict__ X,
+                                                    const float* __restrict__ lng, const float* __restrict__ lnb,
+                                                    float* __restrict__ H, int nt) {
+    constexpr int WMW = BM / 64, WNW = 4 / WMW;
+    constexpr int WN = TBN / WNW, J = WN / 16;
+    __shared__ __align__(16) uint8_t As[BM * T8LD];
+    __shared__ __align__(16) uint8_t Bs[TBN * T8LD];
+    __shared__ float red[TBN / 16][BM];
+
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int m0 = (t / nt) * BM, n0 = (t % nt) * TBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave / WNW) * 64, wn = (wave % WNW) * WN;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+
+    f4v acc[4][J];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-            if (m >= M) continue;
+        for (int j = 0; j < J; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+
+    constexpr int NA = 16 * BM / 256;  // float4 of A per thread per chunk
+    float4 pa[NA];
+    uint4 pb[2];
+#define J2K_F8_LOAD(kc)                                                                                        \
+    {                                                                                                          \
+        _Pragma("unroll") for (int q = 0; q < NA; q++) {                                                       \
+            const int v = tid + 256 * q, row = v >> 4, k = (kc) + (v & 15) * 4, m = m0 + row;                  \
+            pa[q] = (m < M && k < K) ? *reinterpret_cast<const float4*>(A + (long)m * K + k)                    \
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);                                        \
+        }                                                                                                      \
+        _Pragma("unroll") for (int q = 0; q < 2; q++) {                                                        \
+            const int v = tid + 256 * q, row = v >> 2, k = (kc) + (v & 3) * 16, n = n0 + row;                  \
+            pb[q] = (n < N && k < K) ? *reinterpret_cast<const uint4*>(Bt + (long)n * K + k) : make_uint4(0, 0, 0, 0); \
+        }                                                                                                      \
+    }
+    J2K_F8_LOAD(0)
+    for (int kc = 0; kc < K; kc += TBK) {
+        __syncthreads();  // the previous chunk's fragments have been read
 #pragma unroll
-            for (int j = 0; j < J; j++) {
-                const float h = (acc[i][j][r] - mean[i][r]) * inv[i][r] * gc[j] + bc[j];
-                const long o = (long)m * N + wn + j * 16 + r16;
-                if (SPLIT) static_cast<float*>(Hv)[o] = h;
-                else static_cast<uint16_t*>(Hv)[o] = f2bf(h);
-            }
+        for (int q = 0; q < NA; q++) {
+            const int v = tid + 256 * q, row = v >> 4, kv = v & 15;
+            const float e0 = fminf(fmaxf(pa[q].x * inv_sa, -448.f), 448.f);
+            const float e1 = fminf(fmaxf(pa[q].y * inv_sa, -448.f), 448.f);
+            const float e2 = fminf(fmaxf(pa[q].z * inv_sa, -448.f), 448.f);
+            const float e3 = fminf(fmaxf(pa[q].w * inv_sa, -448.f), 448.f);
+            int w = __builtin_amdgcn_cvt_pk_fp8_f32(e0, e1, 0, false);
+            w = __builtin_amdgcn_cvt_pk_fp8_f32(e2, e3, w, true);
+            *reinterpret_cast<int*>(&As[row * T8LD + kv * 4]) = w;
         }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int v = tid + 256 * q, row = v >> 2, kv = v & 3;
+            *reinterpret_cast<uint4*>(&Bs[row * T8LD + kv * 16]) = pb[q];
+        }
+        __syncthreads();
+        if (kc + TBK < K) J2K_F8_LOAD(kc + TBK)
+#pragma unroll
+        for (int ks = 0; ks < TBK; ks += 32) {
+            long a8[4], b8[J];
+#pragma unroll
+            for (int j = 0; j < J; j++) b8[j] = *reinterpret_cast<const long*>(&Bs[(wn + j * 16 + r16) * T8LD + ks + kq]);
+#pragma unroll
+            for (int i = 0; i < 4; i++) a8[i] = *reinterpret_cast<const long*>(&As[(wm + i * 16 + r16) * T8LD + ks + kq]);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a8[i], b8[j], acc[i][j], 0, 0, 0);
+        }
+    }
+#undef J2K_F8_LOAD
+    tile_epilogue<EPI, true, BM, J>(acc, sa, colscale, M, N, m0, n0, wm, wn, lane, bias, C, X, lng, lnb, H, red);
 }
 
 bool gemm_tile_ok(int N, int K, bool split) { return N % 16 == 0 && K % (split ? 4 : 8) == 0; }
@@ -271,6 +375,35 @@ static void tile(hipStream_t s, int M, int N, int K, const void* A, const uint16
     if (bm == 64) tile_bm<SPLIT, 64>(s, M, N, K, A, Bt, bias, C, X, epi, g, b, H);
     else tile_bm<SPLIT, 128>(s, M, N, K, A, Bt, bias, C, X, epi, g, b, H);
     LAUNCHCHK("gemm tile");
+}
+
+template <int BM>
+static void tile_f8_bm(hipStream_t s, int M, int N, int K, const float* A, float sa, const uint8_t* Bt,
+                       const float* cs, const float* bias, float* C, float* X, int epi, const float* g,
+                       const float* b, float* H) {
+    const int mt = cdiv(M, BM), nt = cdiv(N, TBN);
+    dim3 grid(mt * nt), block(256);
+    const float inv = 1.0f / sa;
+    switch (epi) {
+        case EPI_STORE: hipLaunchKernelGGL((k_tile_f8<EPI_STORE, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
+        case EPI_GELU: hipLaunchKernelGGL((k_tile_f8<EPI_GELU, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
+        case EPI_RESID: hipLaunchKernelGGL((k_tile_f8<EPI_RESID, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
+        default: hipLaunchKernelGGL((k_tile_f8<EPI_RESID_LN, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
+    }
+}
+
+void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, float sa, const uint8_t* Bt, const float* cs,
+                  const float* bias, float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H,
+                  int bm) {
+    if (M <= 0) return;
+    if (N % 16 || K % 16) fail(J2K_ERR_LIMIT, "gemm fp8 tile: N and K must be multiples of 16");
+    if (!(sa > 0.f)) fail(J2K_ERR_INVALID, "gemm fp8 tile: activation scale must be positive");
+    if (((uintptr_t)A & 15) || ((uintptr_t)Bt & 15)) fail(J2K_ERR_INVALID, "gemm fp8 tile: operands must be 16-byte aligned");
+    if (epi == EPI_RESID_LN && N != TBN) fail(J2K_ERR_LIMIT, "gemm fp8 tile: the LayerNorm epilogue needs N == 128");
+    if (bm <= 0) bm = gemm_tile_default_bm(M, N);
+    if (bm == 64) tile_f8_bm<64>(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
+    else tile_f8_bm<128>(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
+    LAUNCHCHK("gemm fp8 tile");
 }
 
 void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const uint16_t* Bt, const float* bias,

@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 namespace j2k {
 
@@ -186,6 +187,32 @@ const Json* Json::find(const std::string& key) const {
     return it == obj.end() ? nullptr : &it->second;
 }
 
+float e4m3_decode(uint8_t code) {
+    const int sgn = code >> 7, e = (code >> 3) & 15, mant = code & 7;
+    if (e == 15 && mant == 7) return std::numeric_limits<float>::quiet_NaN();
+    const float mag = e == 0 ? std::ldexp((float)mant, -9) : std::ldexp(1.0f + mant / 8.0f, e - 7);
+    return sgn ? -mag : mag;
+}
+
+uint8_t e4m3_encode(float v) {
+    if (std::isnan(v)) return 0x7f;
+    const uint8_t sgn = std::signbit(v) ? 0x80 : 0;
+    double a = std::fabs((double)v);
+    if (a >= 448.0) return sgn | 0x7e;  // saturate
+    // the nearest of the 127 non-negative finite codes, ties to the even code
+    int lo = 0, hi = 0x7e;
+    while (lo < hi) {  // largest code with value <= a
+        const int mid = (lo + hi + 1) / 2;
+        if ((double)e4m3_decode((uint8_t)mid) <= a) lo = mid;
+        else hi = mid - 1;
+    }
+    if (lo < 0x7e) {
+        const double below = e4m3_decode((uint8_t)lo), above = e4m3_decode((uint8_t)(lo + 1));
+        if (a - below > above - a || (a - below == above - a && (lo & 1))) lo++;
+    }
+    return sgn | (uint8_t)lo;
+}
+
 Weights read_safetensors(const uint8_t* bytes, size_t n) {
     if (!bytes || n < 8) fail(J2K_ERR_SHAPE, "safetensors: buffer shorter than its 8-byte header length");
     uint64_t hl = 0;
@@ -199,8 +226,10 @@ Weights read_safetensors(const uint8_t* bytes, size_t n) {
         if (name == "__metadata__") continue;
         const Json* dt = v.find("dtype");
         if (!dt || dt->kind != Json::Str) fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` has no dtype");
-        if (dt->str != "F32")
-            fail(J2K_ERR_BAD_DTYPE, "tensor `" + name + "`: only F32 is supported (got " + dt->str + ")");
+        const std::string& dtype = dt->str;
+        const size_t esize = dtype == "F32" ? 4 : dtype == "BF16" ? 2 : dtype == "F8_E4M3" ? 1 : 0;
+        if (!esize)
+            fail(J2K_ERR_BAD_DTYPE, "tensor `" + name + "`: dtype " + dtype + " (F32, BF16 or F8_E4M3 with a scale)");
         const Json* shp = v.find("shape");
         const Json* off = v.find("data_offsets");
         if (!shp || shp->kind != Json::Arr || !off || off->kind != Json::Arr || off->arr.size() != 2)
@@ -208,11 +237,31 @@ Weights read_safetensors(const uint8_t* bytes, size_t n) {
         Tensor t;
         for (auto& d : shp->arr) t.shape.push_back((int64_t)d.num);
         size_t a = (size_t)off->arr[0].num, b = (size_t)off->arr[1].num;
-        if (b < a || data_start + b > n || (b - a) != (size_t)t.numel() * 4)
+        if (b < a || data_start + b > n || (b - a) != (size_t)t.numel() * esize)
             fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` data range does not match its shape");
+        const uint8_t* src = bytes + data_start + a;
         t.data.resize((size_t)t.numel());
-        std::memcpy(t.data.data(), bytes + data_start + a, b - a);  // little-endian host
+        if (esize == 4) {
+            std::memcpy(t.data.data(), src, b - a);  // little-endian host
+        } else if (esize == 2) {
+            for (size_t i = 0; i < t.data.size(); i++) {
+                const uint32_t u = (uint32_t)src[2 * i] | ((uint32_t)src[2 * i + 1] << 8);
+                const uint32_t w = u << 16;
+                std::memcpy(&t.data[i], &w, 4);
+            }
+        } else {
+            t.f8.assign(src, src + t.data.size());
+        }
         out.emplace(name, std::move(t));
+    }
+    // F8 tensors take their scale from `<name>.scale`
+    for (auto& [name, t] : out) {
+        if (t.f8.empty()) continue;
+        auto it = out.find(name + ".scale");
+        if (it == out.end() || it->second.data.size() != 1 || !it->second.f8.empty())
+            fail(J2K_ERR_MISSING_TENSOR, "F8_E4M3 tensor `" + name + "` needs an F32 scale `" + name + ".scale`");
+        t.f8_scale = it->second.data[0];
+        for (size_t i = 0; i < t.f8.size(); i++) t.data[i] = e4m3_decode(t.f8[i]) * t.f8_scale;
     }
     return out;
 }

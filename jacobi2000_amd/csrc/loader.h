@@ -29,7 +29,9 @@ struct Json {
 
 struct Tensor {
     std::vector<int64_t> shape;
-    std::vector<float> data;
+    std::vector<float> data;      // f32 values (F8 tensors: decoded times their scale)
+    std::vector<uint8_t> f8;      // F8_E4M3 tensors: the codes as stored
+    float f8_scale = 0.f;         // and their dequantization scale (`<name>.scale`)
     int64_t numel() const {
         int64_t n = 1;
         for (auto s : shape) n *= s;
@@ -40,9 +42,18 @@ struct Tensor {
 using Weights = std::map<std::string, Tensor>;
 
 // crates/engine/src/weights.rs:29-48: 8-byte LE header length, JSON header
-// {name: {dtype, shape, data_offsets}}, LE F32 data. Non-F32 tensors fail
-// with J2K_ERR_BAD_DTYPE (weights.rs:39 panics).
+// {name: {dtype, shape, data_offsets}}, LE data. F32 as the reference; the
+// extension of the format adds BF16 (widened exactly) and F8_E4M3 (OCP
+// E4M3FN codes) whose f32 value is code * `<name>.scale`, an F32 tensor of
+// one element stored beside it. Other dtypes fail with J2K_ERR_BAD_DTYPE
+// (weights.rs:39 panics).
 Weights read_safetensors(const uint8_t* bytes, size_t n);
+
+// OCP FP8 E4M3FN: 1 sign, 4 exponent (bias 7), 3 mantissa bits, no
+// infinities, S.1111.111 is NaN, largest finite 448. Encoding rounds to
+// nearest even and saturates to +-448 (NaN stays NaN).
+float e4m3_decode(uint8_t code);
+uint8_t e4m3_encode(float v);
 
 // weights.rs:52-54: a named tensor or J2K_ERR_MISSING_TENSOR naming it.
 const Tensor& get(const Weights& w, const std::string& name);

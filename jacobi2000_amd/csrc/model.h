@@ -14,6 +14,12 @@ namespace j2k {
 struct LayerW {
     float *n1g, *n1b, *wqkv, *bqkv, *wp, *bp, *n2g, *n2b, *w1, *b1, *w2, *b2;
     uint16_t *wqkv_t, *wp_t, *w1_t, *w2_t;  // bf16 copies, transposed to [N, K]
+    // FP8 form: E4M3 codes [N, K], each weight tensor's scale per output
+    // column, and the static per-tensor scales of the four products' inputs
+    // (LN1 output, attention output, LN2 output, GELU output).
+    uint8_t *wqkv_8, *wp_8, *w1_8, *w2_8;
+    float *cs_qkv, *cs_p, *cs_1, *cs_2;
+    float s_h1, s_o, s_h2, s_g;
 };
 
 // Scratch slots, each grown on demand (the stream is drained before a regrow).
@@ -116,6 +122,7 @@ struct Model {
     // splits them into a bf16 pair inside the GEMM.
     bool bf16() const { return cfg.precision == J2K_PREC_BF16; }
     bool split() const { return cfg.precision == J2K_PREC_BF16X2; }
+    bool fp8() const { return cfg.precision == J2K_PREC_FP8; }
 };
 
 // Makes `m.device` current for the scope of a call.

@@ -142,6 +142,35 @@ struct Uploader {
         }
         return put(t.data.data(), t.data.size());
     }
+    // E4M3 copy of weights [K, N_i] joined along N, transposed to [N, K]:
+    // each tensor keeps the codes and scale it was stored with (F8_E4M3
+    // checkpoints) or gets scale amax / 448 and round-to-nearest-even codes.
+    // cs receives the scale of every output column.
+    uint8_t* f8_t(const std::vector<const Tensor*>& ts, int K, float** cs) {
+        int N = 0;
+        for (auto* t : ts) N += (int)t->shape[1];
+        std::vector<uint8_t> codes((size_t)K * N);
+        std::vector<float> col(N);
+        int n0 = 0;
+        for (auto* t : ts) {
+            const int Ni = (int)t->shape[1];
+            float sc = t->f8_scale;
+            if (t->f8.empty()) {
+                float amax = 0.f;
+                for (float v : t->data) amax = std::max(amax, std::fabs(v));
+                sc = amax > 0.f ? amax / 448.0f : 1.0f;
+            }
+            for (int k = 0; k < K; k++)
+                for (int n = 0; n < Ni; n++) {
+                    const size_t i = (size_t)k * Ni + n;
+                    codes[(size_t)(n0 + n) * K + k] = t->f8.empty() ? e4m3_encode(t->data[i] / sc) : t->f8[i];
+                }
+            for (int n = 0; n < Ni; n++) col[n0 + n] = sc;
+            n0 += Ni;
+        }
+        *cs = put(col.data(), col.size());
+        return put(codes.data(), codes.size());
+    }
     // [K, N] row-major f32 -> bf16 [N, K].
     uint16_t* bf16_t(const std::vector<float>& w, int K, int N) {
         std::vector<uint16_t> t((size_t)K * N);
@@ -151,11 +180,47 @@ struct Uploader {
     }
 };
 
+// Static bounds of the FP8 products' inputs (so their per-tensor scales
+// never saturate): an LN output obeys |h| <= max|gamma| sqrt(d-1) + max|beta|
+// (one element of d values of unit biased variance is at most sqrt(d-1)); a
+// product input z = h W + b obeys |z_n| <= bound(h) sum_k |W[k][n]| + |b_n|
+// (W as quantized, the weights the product uses);
+// the attention output is a convex combination of v rows; |GELU(z)| <=
+// max(bound(z), 0.17).
+float max_abs(const Tensor& t) {
+    float a = 0.f;
+    for (float v : t.data) a = std::max(a, std::fabs(v));
+    return a;
+}
+// A weight tensor as the FP8 products see it: its E4M3 codes times its
+// per-tensor scale (the tensor itself when the checkpoint stored codes).
+Tensor as_f8(const Tensor& t) {
+    if (!t.f8.empty()) return t;
+    Tensor q = t;
+    float amax = 0.f;
+    for (float v : t.data) amax = std::max(amax, std::fabs(v));
+    const float sc = amax > 0.f ? amax / 448.0f : 1.0f;
+    for (float& v : q.data) v = e4m3_decode(e4m3_encode(v / sc)) * sc;
+    return q;
+}
+float ln_bound(const Tensor& g, const Tensor& b, int d) { return max_abs(g) * std::sqrt((float)(d - 1)) + max_abs(b); }
+float lin_bound(float hb, const Tensor& w, const Tensor& b) {
+    const int K = (int)w.shape[0], N = (int)w.shape[1];
+    float worst = 0.f;
+    for (int n = 0; n < N; n++) {
+        float s = 0.f;
+        for (int k = 0; k < K; k++) s += std::fabs(w.data[(size_t)k * N + n]);
+        worst = std::max(worst, hb * s + std::fabs(b.data[n]));
+    }
+    return worst;
+}
+
 }  // namespace
 
 Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision) {
-    if (precision != J2K_PREC_F32 && precision != J2K_PREC_BF16 && precision != J2K_PREC_BF16X2)
-        fail(J2K_ERR_INVALID, "precision must be F32, BF16 or BF16X2");
+    if (precision != J2K_PREC_F32 && precision != J2K_PREC_BF16 && precision != J2K_PREC_BF16X2 &&
+        precision != J2K_PREC_FP8)
+        fail(J2K_ERR_INVALID, "precision must be F32, BF16, BF16X2 or FP8");
     if (!cfg_json) fail(J2K_ERR_INVALID, "config_json is NULL");
     Weights w = read_safetensors(st, n);
     j2k_config cfg = parse_config(cfg_json, w, precision);
@@ -219,7 +284,21 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
         L.b1 = up.tensor(w, p + "ff1.bias", {ff});
         L.w2 = up.tensor(w, p + "ff2.weight", {ff, d});
         L.b2 = up.tensor(w, p + "ff2.bias", {d});
-        if (precision != J2K_PREC_F32) {
+        if (precision == J2K_PREC_FP8) {
+            const Tensor &pw = get(w, p + "proj.weight"), &w1 = get(w, p + "ff1.weight"), &w2 = get(w, p + "ff2.weight");
+            L.wqkv_8 = up.f8_t({&q, &k, &v}, d, &L.cs_qkv);
+            L.wp_8 = up.f8_t({&pw}, d, &L.cs_p);
+            L.w1_8 = up.f8_t({&w1}, d, &L.cs_1);
+            L.w2_8 = up.f8_t({&w2}, ff, &L.cs_2);
+            const float bh1 = ln_bound(get(w, p + "norm1.gamma"), get(w, p + "norm1.beta"), d);
+            const float bh2 = ln_bound(get(w, p + "norm2.gamma"), get(w, p + "norm2.beta"), d);
+            const float bo = lin_bound(bh1, as_f8(v), get(w, p + "v.bias"));
+            const float bz = lin_bound(bh2, as_f8(w1), get(w, p + "ff1.bias"));
+            L.s_h1 = bh1 / 448.0f;
+            L.s_o = bo / 448.0f;
+            L.s_h2 = bh2 / 448.0f;
+            L.s_g = std::max(bz, 0.17f) / 448.0f;
+        } else if (precision != J2K_PREC_F32) {
             L.wqkv_t = up.bf16_t(wqkv, d, 3 * d);
             L.wp_t = up.bf16_t(get(w, p + "proj.weight").data, d, d);
             L.w1_t = up.bf16_t(get(w, p + "ff1.weight").data, d, ff);
@@ -378,6 +457,54 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
     }
 }
 
+// The trunk in the FP8 form: activations f32 in HBM, the four block
+// products on E4M3 codes (tile GEMM quantizing its A operand), attention,
+// LayerNorm and residual in f32. d = 128 takes the LayerNorms in the GEMM
+// epilogues.
+void trunk_f8(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
+    const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
+    float* Hh = m.get<float>(S_H, (size_t)M * d);
+    float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
+    float* O = m.get<float>(S_O, (size_t)M * d);
+    float* F = m.get<float>(S_F, (size_t)M * ff);
+    const double g = 2.0 * M * d, ln_b = (double)M * d * 8, at_b = 16.0 * M * d;
+    const bool fuse_ln = d == 128;
+    auto gemm = [&](int op, int N, int K, const float* A, float sa, const uint8_t* Bt, const float* cs,
+                    const float* bias, float* C, int epi, const float* lg, const float* lb) {
+        TScope t(m, s, KC_GEMM, g / d * N * K);
+        gemm_f8_tile(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, lg, lb, Hh, m.tune.choice(op, M));
+    };
+    auto norm = [&](const float* lg, const float* lb) {
+        TScope t(m, s, KC_NORM, ln_b);
+        layernorm(s, M, d, X, lg, lb, Hh, nullptr);
+    };
+    if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
+    for (int l = 0; l < nl; l++) {
+        const LayerW& L = m.L[l];
+        const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;
+        gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.s_h1, L.wqkv_8, L.cs_qkv, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        {
+            TScope t(m, s, KC_ATTN, at_b);
+            attend<float>(m, s, M, QKV, O, ap);
+        }
+        if (fuse_ln) {
+            gemm(TO_GEMM_PROJ, d, d, O, L.s_o, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
+        } else {
+            gemm(TO_GEMM_PROJ, d, d, O, L.s_o, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
+            norm(L.n2g, L.n2b);
+        }
+        gemm(TO_GEMM_FF, ff, d, Hh, L.s_h2, L.w1_8, L.cs_1, L.b1, F, EPI_GELU, nullptr, nullptr);
+        if (!next) {
+            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+        } else if (fuse_ln) {
+            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID_LN, next->n1g, next->n1b);
+        } else {
+            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            norm(next->n1g, next->n1b);
+        }
+    }
+}
+
 }  // namespace
 
 void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
@@ -421,7 +548,8 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPla
         }
         return;
     }
-    if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, ap);
+    if (m.fp8()) trunk_f8(m, s, M, X, h_ready, ap);
+    else if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, ap);
     else trunk_lowp<uint16_t>(m, s, M, X, h_ready, ap);
 }
 

@@ -87,7 +87,8 @@ void set_window(Request& r, const float* anchor, int T) {
 // vectors, its request index, and its rows of every activation buffer.
 size_t bytes_per_token(const Model& m) {
     const size_t d = m.cfg.dim, e = m.bf16() ? 2 : 4;
-    const size_t ff = m.cfg.precision != J2K_PREC_F32 && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner) ? 0 : m.cfg.ff_inner;
+    const bool rows = (m.bf16() || m.split()) && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner);
+    const size_t ff = rows ? 0 : m.cfg.ff_inner;
     return 3 * 16 + 4 + d * 4 + d * e + 3 * d * e + d * e + ff * e;
 }
 
@@ -285,7 +286,7 @@ struct Server {
         else m.get<float>(S_H, M * d);
         m.get<char>(S_QKV, M * 3 * d * e);
         m.get<char>(S_O, M * d * e);
-        if (m.cfg.precision == J2K_PREC_F32 || !rows_fused_ok(m.cfg.dim, m.cfg.ff_inner)) m.get<char>(S_F, M * ff * e);
+        if (!((m.bf16() || m.split()) && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner))) m.get<char>(S_F, M * ff * e);
         m.p_tables_batch = -1;
         if (active.size() > h_resid_cap) {
             if (h_resid) HIPCHK(hipHostFree(h_resid));

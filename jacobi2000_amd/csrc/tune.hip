@@ -213,7 +213,8 @@ void force_tuning(Model& m, const char* spec) {
 }
 
 void autotune(Model& m, long max_tokens) {
-    if (m.cfg.precision == J2K_PREC_F32) return;  // the plain form keeps its fixed shapes
+    // the plain form keeps its fixed shapes; the FP8 form its default block heights
+    if (m.cfg.precision == J2K_PREC_F32 || m.fp8()) return;
     int top = Tuning::bucket(max_tokens);
     if (top > B_MAX) top = B_MAX;
     if (top < B_MIN) top = B_MIN;

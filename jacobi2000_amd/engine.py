@@ -24,7 +24,8 @@ import numpy as np
 from . import _lib
 from ._lib import J2KError, call
 
-PRECISIONS = {"f32": _lib.PREC_F32, "fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16, "bf16x2": _lib.PREC_BF16X2}
+PRECISIONS = {"f32": _lib.PREC_F32, "fp32": _lib.PREC_F32, "bf16": _lib.PREC_BF16, "bf16x2": _lib.PREC_BF16X2,
+              "fp8": _lib.PREC_FP8}
 
 
 def _f32(a) -> np.ndarray:
@@ -42,8 +43,10 @@ def device_count() -> int:
 
 
 class Model:
-    """A checkpoint on one GPU. `precision` is "f32" (plain form) or "bf16"
-    (served form: bf16 weights/activations on MFMA, f32 accumulate)."""
+    """A checkpoint on one GPU. `precision` is "f32" (plain form), "bf16x2"
+    (served form: bf16 weights on MFMA, f32 activations as bf16 pairs), "bf16"
+    (bf16 activations too) or "fp8" (the block products on E4M3 codes with
+    per-tensor scales); every form accumulates in f32."""
 
     def __init__(self, handle: int, precision: str):
         self._h = ctypes.c_void_p(handle)

@@ -155,7 +155,7 @@ def measure_traffic(args, symbol):
         return None, "rocprofv3 not on PATH"
     child = [sys.executable, str(ROOT / "bench.py"), "--forms", args.headline, "--headline", args.headline,
              "--steps", "1", "--warmup", "1", "--no-cpu-baseline", "--no-oracle-error", "--no-traffic",
-             "--stream", "0", "--rollouts", "0", "--copy-probe", "256", "--no-plain"]
+             "--stream", "0", "--rollouts", "0", "--wide", "0", "--copy-probe", "256", "--no-plain"]
     probe_kib = 256 * 1024.0
     got = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -287,6 +287,68 @@ def run_stream(st, cj, meta, reqs, rate, served, precision="bf16x2"):
                 p50_ms=float(np.percentile(lat_ms, 50)), p99_ms=float(np.percentile(lat_ms, 99)), **extra)
 
 
+def run_wide(meta, steps, err_side=32):
+    """The wide shape (d=256, 6 layers, 8 heads x 32, MLP 1024, seed 1, 5x5x3
+    stencil) on 128x128 x 16 grids: f32, bf16x2 and FP8 throughput (one
+    request per step, fixed point to tol), and each form's error against the
+    f64 oracle on a err_side x err_side x 16 grid (the oracle at full size
+    would take minutes), same iteration count as the GPU run there."""
+    import torch
+    import jacobi2000_amd as j
+    from jacobi2000_amd import checkpoint as ck
+    from oracle import oracle as O
+
+    st, cj = ck.random_cell_st(dim=256, layers=6, heads=8, ff=1024, seed=1, radius=2, time_radius=1)
+    T_, side = 16, 128
+    dev = torch.device("cuda:0")
+    out = {"workload": "wide shape d256 L6 H8x32 ff1024 seed 1, 5x5x3 stencil (75 keys), 128x128 x 16, "
+                       "5 drops, slice 0 known, tol 1e-4, cap 32",
+           "params": 0, "forms": {}}
+    g = ck.drops_grid(T_, side, side, 5, seed=11)
+    k = ck.mask_of("forward", T_, side, side)
+    x0 = j.fill_blank(g, k)
+    ge = ck.drops_grid(T_, err_side, err_side, 2, seed=12)
+    ke = ck.mask_of("forward", T_, err_side, err_side)
+    xe = j.fill_blank(ge, ke)
+    cpu = O.FormC.load(st, cj)
+    out["params"] = int(sum(a.size for a in cpu.w.values()))
+    refs = {}
+    for prec in ("f32", "bf16x2", "fp8"):
+        model = j.Model.from_bytes(st, cj, precision=prec)
+        d_x0 = torch.from_numpy(x0).to(dev)
+        d_grid = torch.empty_like(d_x0)
+        d_k = torch.from_numpy(k).to(dev)
+        d_m = torch.from_numpy(meta).to(dev)
+        stream = torch.cuda.current_stream().cuda_stream
+
+        def one():
+            d_grid.copy_(d_x0)
+            return model.iterate_dev(d_grid.data_ptr(), d_k.data_ptr(), d_m.data_ptr(), T_, side, side, 32, 1e-4,
+                                     stream)
+
+        one()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            its.append(one()[0])
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t0) / steps
+        xg, ite, _, ve = model.iterate(xe, ke, meta, max_iter=32, tol=1e-4)
+        if ite not in refs:
+            refs[ite] = cpu.iterate(xe, ke, meta, ite, 0.0, f64=True)[0]
+        ref = refs[ite]
+        d = (xg.astype(np.float64) - ref).reshape(3, -1)
+        r = ref.reshape(3, -1)
+        out["forms"][prec] = {
+            "ms_per_request": el * 1e3, "cells_per_s": T_ * side * side / el, "iterations": its,
+            "error_grid": f"{err_side}x{err_side}x{T_}", "error_iterations": ite,
+            "rel_l2_vs_f64": float(np.linalg.norm(d) / np.linalg.norm(r)),
+            "max_abs_per_field": [float(np.abs(d[c]).max()) for c in range(3)], "energy_violation": float(ve)}
+        model.close()
+    return out
+
+
 def run_rollouts(st, cj, meta, n, served, precision="bf16x2", ny=96, nx=96, window=8, ticks=50):
     """Long rollouts: n forward requests on ny x nx grids, each chaining
     `window`-slice windows to `ticks` slices. served: all submitted at once to
@@ -362,6 +424,8 @@ def main():
     ap.add_argument("--stream", type=int, default=512,
                     help="also run the mixed request stream with this many requests (0: skip)")
     ap.add_argument("--rate", type=float, default=2000.0, help="Poisson arrival rate of the stream, requests/s")
+    ap.add_argument("--wide", type=int, default=2,
+                    help="timed requests per form of the wide shape on 128x128x16 grids (0: skip)")
     ap.add_argument("--rollouts", type=int, default=256,
                     help="also run this many chained 96x96 rollouts (8-slice windows to 50 ticks; 0: skip)")
     ap.add_argument("--headline", default="bf16x2", choices=["f32", "bf16", "bf16x2"],
@@ -461,6 +525,9 @@ def main():
                                       "{32,48,64,96}^2 x {8,16}, masks forward/rewind/reconstruction",
                           "served": s_served, "plain_fp32": s_plain,
                           "served_vs_plain": s_served["value"] / s_plain["value"]}
+    if args.wide > 0:
+        note(f"wide shape: 128x128x16, f32 / bf16x2 / fp8, {args.wide} timed requests each, errors on 32x32x16")
+        line["wide"] = run_wide(meta, args.wide)
     if args.rollouts > 0:
         note(f"rollouts: {args.rollouts} x 96x96, 8-slice windows to 50 ticks, served then plain")
         r_served = run_rollouts(st, cj, meta, args.rollouts, served=True,

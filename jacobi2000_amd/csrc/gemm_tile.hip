@@ -22,6 +22,7 @@
 // block owns whole rows), which writes the next GEMM's input H.
 #include "common.h"
 #include "kernels.h"
+#include "lnstats.h"
 
 namespace j2k {
 
@@ -42,7 +43,7 @@ __device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const f
                                               const float* __restrict__ bias, void* __restrict__ Cv,
                                               float* __restrict__ X, const float* __restrict__ lng,
                                               const float* __restrict__ lnb, void* __restrict__ Hv,
-                                              float (*red)[BM]) {
+                                              float (*red)[BM], float* mrow, float* irow) {
     const int r16 = lane & 15;
     float bcol[J], acol[J];
     bool colok[J];
@@ -89,42 +90,9 @@ __device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const f
         }
     if (EPI != EPI_RESID_LN) return;
 
-    // LayerNorm of the updated rows (cpu.rs:49-65). N == 128: a row's values
-    // sit in the 16 lanes of 8 column tiles, spread over the waves sharing
-    // its rows. Each tile's 16 values are reduced by lane shuffles and the 8
-    // tile sums added in column order, so the result is the same for every
-    // block height.
-    float mean[4][4], inv[4][4];
-#pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int j = 0; j < J; j++) {
-                    const float v = acc[i][j][r];
-                    float s = pass == 0 ? v : (v - mean[i][r]) * (v - mean[i][r]);
-                    s += __shfl_xor(s, 1, 64);
-                    s += __shfl_xor(s, 2, 64);
-                    s += __shfl_xor(s, 4, 64);
-                    s += __shfl_xor(s, 8, 64);
-                    if (r16 == 0) red[wn / 16 + j][wm + i * 16 + (lane >> 4) * 4 + r] = s;
-                }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = wm + i * 16 + (lane >> 4) * 4 + r;
-                float tot = 0.f;
-#pragma unroll
-                for (int w = 0; w < TBN / 16; w++) tot += red[w][row];
-                if (pass == 0) mean[i][r] = tot / (float)N;
-                else inv[i][r] = 1.0f / sqrtf(tot / (float)N + 1e-5f);
-            }
-        __syncthreads();
-    }
+    // LayerNorm of the updated rows (cpu.rs:49-65); N == 128, so the block
+    // owns whole rows, 8 column tiles each.
+    ln_row_stats<BM, J>(acc, wm, wn, lane, red, mrow, irow);
     float gc[J], bc[J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -135,11 +103,12 @@ __device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const f
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+            const int row = wm + i * 16 + (lane >> 4) * 4 + r, m = m0 + row;
             if (m >= M) continue;
+            const float mu = mrow[row], iv = irow[row];
 #pragma unroll
             for (int j = 0; j < J; j++) {
-                const float h = (acc[i][j][r] - mean[i][r]) * inv[i][r] * gc[j] + bc[j];
+                const float h = (acc[i][j][r] - mu) * iv * gc[j] + bc[j];
                 const long o = (long)m * N + wn + j * 16 + r16;
                 if (OUTF32) static_cast<float*>(Hv)[o] = h;
                 else static_cast<uint16_t*>(Hv)[o] = f2bf(h);
@@ -158,7 +127,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
     __shared__ __align__(16) uint16_t Ah[BM * TLD];
     __shared__ __align__(16) uint16_t Al[SPLIT ? BM * TLD : 8];
     __shared__ __align__(16) uint16_t Bs[TBN * TLD];
-    __shared__ float red[TBN / 16][BM];
+    __shared__ float red[TBN / 16][BM], mrow[BM], irow[BM];
 
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int m0 = (t / nt) * BM, n0 = (t % nt) * TBN;
@@ -254,7 +223,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
         }
     }
 
-    tile_epilogue<EPI, SPLIT, BM, J>(acc, 1.0f, nullptr, M, N, m0, n0, wm, wn, lane, bias, Cv, X, lng, lnb, Hv, red);
+    tile_epilogue<EPI, SPLIT, BM, J>(acc, 1.0f, nullptr, M, N, m0, n0, wm, wn, lane, bias, Cv, X, lng, lnb, Hv, red, mrow, irow);
 }
 
 // FP8 form: A is f32 and is quantized while staged, q = clamp(a * inv_sa,
@@ -276,7 +245,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const f
     constexpr int WN = TBN / WNW, J = WN / 16;
     __shared__ __align__(16) uint8_t As[BM * T8LD];
     __shared__ __align__(16) uint8_t Bs[TBN * T8LD];
-    __shared__ float red[TBN / 16][BM];
+    __shared__ float red[TBN / 16][BM], mrow[BM], irow[BM];
 
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int m0 = (t / nt) * BM, n0 = (t % nt) * TBN;
@@ -340,7 +309,7 @@ __global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const f
         }
     }
 #undef J2K_F8_LOAD
-    tile_epilogue<EPI, true, BM, J>(acc, sa, colscale, M, N, m0, n0, wm, wn, lane, bias, C, X, lng, lnb, H, red);
+    tile_epilogue<EPI, true, BM, J>(acc, sa, colscale, M, N, m0, n0, wm, wn, lane, bias, C, X, lng, lnb, H, red, mrow, irow);
 }
 
 bool gemm_tile_ok(int N, int K, bool split) { return N % 16 == 0 && K % (split ? 4 : 8) == 0; }

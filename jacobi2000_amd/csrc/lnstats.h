@@ -1,0 +1,64 @@
+// Row statistics of a LayerNorm over 128 columns (cpu.rs:49-65: mean, then
+// the biased variance of the centred values, 1/sqrt(var + 1e-5)) for rows
+// whose values sit in MFMA accumulators: acc[i][j][r] is row wm + i*16 +
+// 4*(lane/16) + r, column wn + j*16 + lane%16, spread over the block's waves.
+//
+// One fixed order for every kernel and block shape, so the tile GEMM
+// epilogue and the row-panel kernel give bit-identical norms: each 16-column
+// tile is summed over its 16 lanes (tile_sum), then the 8 tile sums of a row
+// are added in column order by one thread per row, which also takes the
+// division and the square root once per row.
+#pragma once
+
+#include "common.h"
+
+namespace j2k {
+
+typedef float lnf4v __attribute__((ext_vector_type(4)));
+
+// Sum over the 16 lanes of a column tile (the lanes sharing lane / 16), by
+// the butterfly xor 1, 2, 4, 8: every lane of the tile ends with the total.
+__device__ __forceinline__ float tile_sum(float x) {
+    x += __shfl_xor(x, 1, 64);
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 4, 64);
+    x += __shfl_xor(x, 8, 64);
+    return x;
+}
+
+// mean and inv (1/sqrt(var + 1e-5)) of every row into mrow / irow [BM]
+// (LDS, row = wm + i*16 + 4*(lane/16) + r); red is [8][BM] LDS. Synchronises
+// the block four times; every thread of the block must call it. The
+// statistics stay in LDS so they hold no registers across the caller.
+template <int BM, int J>
+__device__ __forceinline__ void ln_row_stats(const lnf4v (&acc)[4][J], int wm, int wn, int lane, float (*red)[BM],
+                                             float* mrow, float* irow) {
+    const int r16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = wm + i * 16 + g * 4 + r;
+                const float mu = pass == 0 ? 0.f : mrow[row];
+#pragma unroll
+                for (int j = 0; j < J; j++) {
+                    const float x = acc[i][j][r];
+                    const float s = tile_sum(pass == 0 ? x : (x - mu) * (x - mu));
+                    if (r16 == 0) red[wn / 16 + j][row] = s;
+                }
+            }
+        __syncthreads();
+        if ((int)threadIdx.x < BM) {
+            float tot = 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; t++) tot += red[t][threadIdx.x];
+            if (pass == 0) mrow[threadIdx.x] = tot / 128.0f;
+            else irow[threadIdx.x] = 1.0f / sqrtf(tot / 128.0f + 1e-5f);
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace j2k

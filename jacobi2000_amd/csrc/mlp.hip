@@ -19,10 +19,11 @@
 // (K-contiguous bf16 [N, K]) stream from L2 as 16-byte fragments, loaded a
 // whole 128 x 128 chunk ahead of their use. The ff-wide intermediate is made
 // 128 columns at a time and consumed at once as the next K chunk of ff2.
-// Row statistics of the LayerNorms: each 16-column tile's values are reduced
-// by lane shuffles and the 8 tile sums added in column order (cpu.rs:49-65).
+// Row statistics of the LayerNorms in the order every served kernel uses
+// (lnstats.h).
 #include "common.h"
 #include "kernels.h"
+#include "lnstats.h"
 
 namespace j2k {
 
@@ -110,55 +111,16 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int M, uint1
     }
 }
 
-// LayerNorm statistics of the panel held in v (row i*16 + rq + r, column
-// wave*32 + j*16 + r16): per 16-column tile by lane shuffles, then the 8
-// tile sums in column order. Synchronises the block.
-__device__ __forceinline__ void ln_stats(const f4v (&v)[4][2], float (*red)[RB], int wave, int lane,
-                                         float (&mean)[4][4], float (&inv)[4][4]) {
-    const int r16 = lane & 15, rq = (lane >> 4) * 4;
-#pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    const float x = v[i][j][r];
-                    float s = pass == 0 ? x : (x - mean[i][r]) * (x - mean[i][r]);
-                    s += __shfl_xor(s, 1, 64);
-                    s += __shfl_xor(s, 2, 64);
-                    s += __shfl_xor(s, 4, 64);
-                    s += __shfl_xor(s, 8, 64);
-                    if (r16 == 0) red[wave * 2 + j][i * 16 + rq + r] = s;
-                }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = i * 16 + rq + r;
-                float tot = 0.f;
-#pragma unroll
-                for (int t = 0; t < RD / 16; t++) tot += red[t][row];
-                if (pass == 0) mean[i][r] = tot / (float)RD;
-                else inv[i][r] = 1.0f / sqrtf(tot / (float)RD + 1e-5f);
-            }
-        __syncthreads();
-    }
-}
-
 template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT>
 __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
     __shared__ __align__(16) uint16_t Hh[RB * RLD];
     __shared__ __align__(16) uint16_t Hl[SPLIT ? RB * RLD : 8];
     __shared__ __align__(16) uint16_t Fh[RB * RLD];
     __shared__ __align__(16) uint16_t Fl[SPLIT ? RB * RLD : 8];
-    __shared__ float red[RD / 16][RB];
+    __shared__ float red[RD / 16][RB], mrow[RB], irow[RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.x * RB, wn = wave * 32;
     const int r16 = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
-    float mean[4][4], inv[4][4];
     bf8v w1[4][2], w2[4][2];
     f4v xm[4][2];  // the residual stream of the panel
 
@@ -186,7 +148,7 @@ __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b;
         }
-        ln_stats(xm, red, wave, lane, mean, inv);  // its barriers also retire every read of O
+        ln_row_stats<RB, 2>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of O
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int n = wn + j * 16 + r16;
@@ -194,8 +156,10 @@ __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-                    put_a<SPLIT>(Hh, Hl, i * 16 + rq + r, n, (xm[i][j][r] - mean[i][r]) * inv[i][r] * g + b);
+                for (int r = 0; r < 4; r++) {
+                    const int row = i * 16 + rq + r;
+                    put_a<SPLIT>(Hh, Hl, row, n, (xm[i][j][r] - mrow[row]) * irow[row] * g + b);
+                }
         }
         __syncthreads();
     }
@@ -223,7 +187,6 @@ __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
         __syncthreads();
         mma_panel<SPLIT>(acc2, Fh, Fl, w2, r16, kq);
     }
-    if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         const float b = a.b2[wn + j * 16 + r16];
@@ -240,7 +203,8 @@ __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
     if (!QKV && !LNOUT) return;
 
     // ---- the next block's LN1, then its qkv product or H itself
-    ln_stats(xm, red, wave, lane, mean, inv);  // its barriers also retire every read of H
+    ln_row_stats<RB, 2>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of H
+    if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);  // in flight while H is written
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         const int n = wn + j * 16 + r16;
@@ -249,7 +213,7 @@ __global__ __launch_bounds__(256, 2) void k_rows(int M, int ff, RowsArgs a) {
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const float h = (xm[i][j][r] - mean[i][r]) * inv[i][r] * g + b;
+                const float h = (xm[i][j][r] - mrow[i * 16 + rq + r]) * irow[i * 16 + rq + r] * g + b;
                 if (QKV) {
                     put_a<SPLIT>(Hh, Hl, i * 16 + rq + r, n, h);
                 } else {

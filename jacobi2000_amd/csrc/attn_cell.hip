@@ -10,11 +10,13 @@
 // consecutive tiles, and so the time slices whose K/V rows they share, sit
 // in one XCD's L2.
 //
-// Two staging modes, chosen by the autotuner (the arithmetic, and so the
-// result, is the same in both):
+// Three staging modes, chosen by the autotuner (the arithmetic, and so the
+// result, is the same in all):
 //   KV_JOINT  K and V of a head sit in LDS together; one phase per head.
 //   KV_SPLIT  K, then V, in one buffer of half the size: twice the blocks
 //             per CU, twice the barriers.
+//   KV_PLANE  one time plane of K or V at a time, double-buffered: a few KB
+//             of LDS, one barrier per plane.
 //
 // Four lanes share a query, each holding a quarter of the head's dimensions;
 // the dot products are reduced with two lane shuffles. The softmax is
@@ -271,12 +273,137 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
 #undef J2K_WHICH
 }
 
+// KV_PLANE: one time plane of K, then of V, at a time. Per head the phases
+// are K of slices t_lo .. t_hi (scores), then V of the same slices (weights),
+// each plane (tile + halo, one head) double-buffered in LDS with the next
+// phase's rows already in flight in registers: one barrier per phase and a
+// small LDS footprint, so more blocks share a CU. The arithmetic, and its
+// order, is that of the other modes.
+// Three waves per SIMD for the 27-key stencils; the scores of wider ones
+// need the registers of two.
+template <int HD, int RS, int RT, class T>
+__global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 32 || HD > 32 ? 2 : 3)) void k_attn_cell_plane(const T* __restrict__ qkv, T* __restrict__ out, int dim,
+                                                           int heads, const Seg* __restrict__ segs,
+                                                           const CellTile* __restrict__ tiles) {
+    constexpr int D4 = HD / 4, VEC = 16 / sizeof(T), VPR = HD / VEC;
+    constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
+    constexpr int KS = (2 * RS + 1) * (2 * RS + 1), KMAX = KS * (2 * RT + 1);
+    constexpr int NV = (PLANE * VPR + 255) / 256;  // staged vectors per thread per plane
+    extern __shared__ __align__(16) unsigned char smem[];  // two planes
+    T* const planes = reinterpret_cast<T*>(smem);
+    const CellTile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const Seg sg = segs[tl.seg];
+    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + RT, sg.T - 1), nt = t_hi - t_lo + 1;
+    const long ld = 3L * dim;
+    const int n = sg.ny * sg.nx;
+    const int tid = threadIdx.x;
+
+    // vector v of a plane: elements [e*VEC, e*VEC+VEC) of neighbourhood cell
+    // c = v / VPR; its offset in a slice (the same for every slice and head).
+    // Vectors past the plane load the tile's own slice start and are dropped.
+    int coff[NV];
+#pragma unroll
+    for (int i = 0; i < NV; i++) {
+        const int v = tid + 256 * i, c = v / VPR, cy = c / HW, cx = c - cy * HW;
+        coff[i] = v < PLANE * VPR ? wrap(tl.y0 - RS + cy, sg.ny) * sg.nx + wrap(tl.x0 - RS + cx, sg.nx) : 0;
+    }
+    u32x4 pre[NV];
+    // phase f of a head: K of slice t_lo + f for f < nt, then V of slice t_lo + f - nt
+#define J2K_PLANE_FETCH(h, f)                                                                                     \
+    {                                                                                                            \
+        const int which_ = (f) < nt ? 0 : 1, t_ = t_lo + ((f) < nt ? (f) : (f) - nt);                            \
+        const T* src_ = qkv + ((long)sg.base + (long)t_ * n) * ld + (1 + which_) * dim + (h) * HD;              \
+        _Pragma("unroll") for (int i = 0; i < NV; i++) {                                                          \
+            const int v = tid + 256 * i, e = v % VPR;                                                            \
+            pre[i] = *reinterpret_cast<const u32x4*>(src_ + (long)coff[i] * ld + e * VEC);                       \
+        }                                                                                                        \
+    }
+
+    const int q = tid >> 2, sub = tid & 3;
+    const int qy = tl.y0 + (q >> 3), qx = tl.x0 + (q & 7);
+    const bool live = qy < sg.ny && qx < sg.nx;
+    const long qrow = (long)sg.base + (long)tl.t * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
+    const float scale = 1.0f / sqrtf((float)HD);
+    const int phases = 2 * nt;
+
+    J2K_PLANE_FETCH(0, 0)
+    int b = 0;
+    for (int h = 0; h < heads; h++) {
+        float qv[D4], kv[D4], acc[D4], s[KMAX];
+        row_f32<D4>(qkv + qrow * ld + h * HD + sub * D4, qv);
+        float mx = -INFINITY, l = 0.f;
+#pragma unroll
+        for (int d = 0; d < D4; d++) acc[d] = 0.f;
+        for (int f = 0; f < phases; f++, b ^= 1) {
+            T* P = planes + (size_t)b * PLANE * HD;
+#pragma unroll
+            for (int i = 0; i < NV; i++) {
+                const int v = tid + 256 * i, c = v / VPR, e = v % VPR;
+                if (v < PLANE * VPR) *reinterpret_cast<u32x4*>(P + (size_t)c * HD + e * VEC) = pre[i];
+            }
+            __syncthreads();  // the plane is in; every thread is past the phase before the last
+            if (f + 1 < phases) {
+                J2K_PLANE_FETCH(h, f + 1)
+            } else if (h + 1 < heads) {
+                J2K_PLANE_FETCH(h + 1, 0)
+            }
+            if (f < nt) {
+                // scores of slice t_lo + f
+#pragma unroll
+                for (int ct = 0; ct < 2 * RT + 1; ct++) {
+                    if (ct != f) continue;
+#pragma unroll
+                    for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                        for (int dx = -RS; dx <= RS; dx++) {
+                            const int c = ((q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
+                            row_f32<D4>(P + (size_t)c * HD + sub * D4, kv);
+                            float dot = 0.f;
+#pragma unroll
+                            for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
+                            dot += __shfl_xor(dot, 1, 64);
+                            dot += __shfl_xor(dot, 2, 64);
+                            const float sc = dot * scale;
+                            s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
+                            mx = fmaxf(mx, sc);
+                        }
+                }
+            } else {
+                // weights of slice t_lo + f - nt
+#pragma unroll
+                for (int ct = 0; ct < 2 * RT + 1; ct++) {
+                    if (ct != f - nt) continue;
+#pragma unroll
+                    for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                        for (int dx = -RS; dx <= RS; dx++) {
+                            const int c = ((q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
+                            const float p = __expf(s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] - mx);
+                            l += p;
+                            row_f32<D4>(P + (size_t)c * HD + sub * D4, kv);
+#pragma unroll
+                            for (int d = 0; d < D4; d++) acc[d] += p * kv[d];
+                        }
+                }
+            }
+        }
+        if (live) {
+            const float inv = 1.0f / l;
+#pragma unroll
+            for (int d = 0; d < D4; d++) acc[d] *= inv;
+            row_store<D4>(out + qrow * dim + h * HD + sub * D4, acc);
+        }
+    }
+#undef J2K_PLANE_FETCH
+}
+
 namespace {
 
 template <int HD, int RS, int RT, class T>
 const void* fn_ptr(int mode) {
-    return mode == KV_JOINT ? (const void*)k_attn_cell<HD, RS, RT, T, KV_JOINT>
-                            : (const void*)k_attn_cell<HD, RS, RT, T, KV_SPLIT>;
+    return mode == KV_JOINT   ? (const void*)k_attn_cell<HD, RS, RT, T, KV_JOINT>
+           : mode == KV_SPLIT ? (const void*)k_attn_cell<HD, RS, RT, T, KV_SPLIT>
+                              : (const void*)k_attn_cell_plane<HD, RS, RT, T>;
 }
 
 // Kernel for (head size, space radius, time radius, mode); nullptr when not built.
@@ -311,7 +438,7 @@ void attn_cell_prepare() {
     for (int hd : {8, 16, 32, 64})
         for (int rs : {1, 2})
             for (int rt : {0, 1})
-                for (int mode : {KV_JOINT, KV_SPLIT}) {
+                for (int mode : {KV_JOINT, KV_SPLIT, KV_PLANE}) {
                     HIPCHK(hipFuncSetAttribute(pick<float>(hd, rs, rt, mode),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                     HIPCHK(hipFuncSetAttribute(pick<uint16_t>(hd, rs, rt, mode),
@@ -320,6 +447,7 @@ void attn_cell_prepare() {
 }
 
 size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode) {
+    if (mode == KV_PLANE) return 2ull * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;  // two planes of one head
     return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + 1) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
 }
 

@@ -107,12 +107,12 @@ int attn_tokens_per_block(int heads);
 
 // cell_st attention with K/V staged in LDS: one block per 8 x 8 tile of cells
 // of one time slice ({segment, t, y0, x0}), periodic halo `rs`, time window
-// rt. Staging modes (same results): K and V together, or K then V in half
-// the LDS.
+// rt. Staging modes (same results): K and V together, K then V in half the
+// LDS, or one time plane at a time.
 struct CellTile {
     int seg, t, y0, x0;
 };
-enum KvMode : int { KV_JOINT = 0, KV_SPLIT = 1 };
+enum KvMode : int { KV_JOINT = 0, KV_SPLIT = 1, KV_PLANE = 2 };
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles);
 size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode);
 bool attn_cell_ok(int hd, int rs, int rt, int elem, int mode);  // a tiled kernel exists and fits in LDS

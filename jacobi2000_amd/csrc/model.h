@@ -66,14 +66,15 @@ enum TuneOp : int {
     TO_GEMM_QKV = 0,   // block height of the qkv GEMM
     TO_GEMM_PROJ = 1,  // block height of the attention-output GEMM
     TO_GEMM_FF = 2,    // block height of ff1 / ff2 when the MLP is not fused
-    TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode)
+    TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode: joint, split, plane)
     TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), 1 fused
     TO_COUNT = 5
 };
 struct Tuning {
     static constexpr int NB = 31;
     signed char pick[TO_COUNT][NB];
-    float us[TO_COUNT][NB][2];  // measured µs of each candidate
+    static constexpr int NC = 3;  // candidates per op at most
+    float us[TO_COUNT][NB][NC];   // measured µs of each candidate (0: not a candidate)
     int tuned_to = -1;          // highest bucket tuned so far
     bool enabled = true;        // iterate / server tune on first use (J2K_AUTOTUNE=0 turns it off)
     Tuning();
@@ -108,6 +109,12 @@ struct Model {
     void* slot[S_COUNT] = {};
     size_t cap[S_COUNT] = {};
     int p_tables_batch = -1;  // batch size the patch2d seg/tile tables were built for
+    // Grid the single-request cell_st tables in S_SEGS / S_TILES / S_SLICES /
+    // S_CTILES were built for (T = -1: none; whoever else writes those slots
+    // resets it), with their counts.
+    struct {
+        int T = -1, ny = 0, nx = 0, M = 0, n_slices = 0, n_tiles = 0, n_ctiles = 0;
+    } cell_tables;
     int p_tiles = 0;
     Tuning tune;
 

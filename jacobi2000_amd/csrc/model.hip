@@ -574,6 +574,7 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
         HIPCHK(hipMemcpy(ds, segs.data(), segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(dt, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
         m.p_tables_batch = B;
+        m.cell_tables.T = -1;
         m.p_tiles = (int)tiles.size();
     }
     AttnPlan ap;

@@ -288,6 +288,7 @@ struct Server {
         m.get<char>(S_O, M * d * e);
         if (!((m.bf16() || m.split()) && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner))) m.get<char>(S_F, M * ff * e);
         m.p_tables_batch = -1;
+        m.cell_tables.T = -1;
         if (active.size() > h_resid_cap) {
             if (h_resid) HIPCHK(hipHostFree(h_resid));
             h_resid = nullptr;
@@ -342,6 +343,7 @@ struct Server {
         std::vector<Slice> sl;
         for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, 0});
         Slice* dsl = m.get<Slice>(S_SLICES, sl.size());
+        m.cell_tables.T = -1;  // the single-request tables shared this slot
         unsigned* vb32 = m.get<unsigned>(S_RESID, 4);
         HIPCHK(hipMemcpyAsync(dsl, sl.data(), sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
         fill_u32(s, vb32, 2, 0u);

@@ -7,7 +7,8 @@
 // split into blocks, never the arithmetic of an output element, so the
 // choice cannot change a result:
 //   qkv / proj / ff GEMMs   block height 64 or 128 rows
-//   cell attention          K and V staged together, or K then V
+//   cell attention          K and V staged together, K then V, or a time
+//                           plane at a time
 //   d = 128 token-local     proj / qkv GEMMs apart, or fused with the MLP
 #include <cmath>
 #include <cstdio>
@@ -104,8 +105,9 @@ void tune_bucket(Model& m, int b, Events& ev) {
         CellTile* dtiles = m.get<CellTile>(S_CTILES, tiles.size());
         HIPCHK(hipMemcpyAsync(dsegs, segs.data(), sizeof(Seg), hipMemcpyHostToDevice, s));
         HIPCHK(hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(CellTile), hipMemcpyHostToDevice, s));
-        m.p_tables_batch = -1;  // the patch2d tables shared S_SEGS
-        for (int mode : {KV_JOINT, KV_SPLIT}) {
+        m.p_tables_batch = -1;  // the patch2d and cell_st tables share these slots
+        m.cell_tables.T = -1;
+        for (int mode : {KV_JOINT, KV_SPLIT, KV_PLANE}) {
             if (!attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode)) continue;
             m.tune.us[TO_ATTN][b][mode] = time_us(s, ev, [&] {
                 if (split)
@@ -119,7 +121,10 @@ void tune_bucket(Model& m, int b, Events& ev) {
     }
     auto settle = [&](int op) {
         const float* u = m.tune.us[op][b];
-        if (u[0] > 0.f || u[1] > 0.f) m.tune.pick[op][b] = (u[1] > 0.f && (u[0] <= 0.f || u[1] < u[0])) ? 1 : 0;
+        int best = -1;
+        for (int c = 0; c < Tuning::NC; c++)
+            if (u[c] > 0.f && (best < 0 || u[c] < u[best])) best = c;
+        if (best >= 0) m.tune.pick[op][b] = (signed char)best;
     };
     for (int op = 0; op < TO_COUNT; op++) settle(op);
 
@@ -184,7 +189,7 @@ int Tuning::choice(int op, long M) const {
 }
 
 void force_tuning(Model& m, const char* spec) {
-    // "gemm_bm=64|128", "attn_kv_mode=0|1", "rows_fused=0|1", comma separated; every bucket
+    // "gemm_bm=64|128", "attn_kv_mode=0|1|2", "rows_fused=0|1", comma separated; every bucket
     int bm = -1, mode = -1, rows = -1;
     std::string t(spec);
     size_t pos = 0;
@@ -197,7 +202,7 @@ void force_tuning(Model& m, const char* spec) {
             const std::string k = kv.substr(0, eq);
             const int v = std::atoi(kv.c_str() + eq + 1);
             if (k == "gemm_bm" && (v == 64 || v == 128)) bm = v == 64 ? 0 : 1;
-            else if (k == "attn_kv_mode" && (v == KV_JOINT || v == KV_SPLIT)) mode = v;
+            else if (k == "attn_kv_mode" && (v == KV_JOINT || v == KV_SPLIT || v == KV_PLANE)) mode = v;
             else if (k == "rows_fused" && (v == 0 || v == 1)) rows = v;
             else fail(J2K_ERR_INVALID, "J2K_TUNE_FORCE: unknown setting '" + kv + "'");
         }
@@ -248,8 +253,12 @@ std::string tuning_json(const Model& m) {
             const int c = m.tune.pick[op][b];
             if (c < 0) continue;
             const int v = op == TO_ATTN || op == TO_ROWS ? c : GEMM_BM[c];
-            std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f]", names[op], v, names[op],
-                          m.tune.us[op][b][0], m.tune.us[op][b][1]);
+            if (op == TO_ATTN)
+                std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f]", names[op], v, names[op],
+                              m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2]);
+            else
+                std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f]", names[op], v, names[op],
+                              m.tune.us[op][b][0], m.tune.us[op][b][1]);
             out += buf;
         }
         out += "}";

@@ -104,7 +104,7 @@ int j2k_model_config(const void* model, j2k_config* out);
  * keeps the fastest per bucket of packed tokens. A choice never changes a
  * result. j2k_iterate* and j2k_server_create run it on first use unless the
  * environment sets J2K_AUTOTUNE=0; J2K_TUNE_FORCE =
- * "gemm_bm=64|128,attn_kv_mode=0|1|2,rows_fused=0|1" instead fixes the choices
+ * "gemm_bm=64|128,attn_kv_mode=0|1|2,rows_fused=0|1|2" instead fixes the choices
  * of models created after it is set. max_tokens = 0 only reports. `report`
  * (may be NULL) receives the table as JSON text, cut to report_len bytes.
  * F32 models keep their fixed shapes. Synchronises the model's stream; not

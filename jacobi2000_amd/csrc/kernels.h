@@ -57,7 +57,8 @@ struct RowsArgs {
     void* Hout = nullptr;
 };
 bool rows_fused_ok(int dim, int ff);
-void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
+// waves: 4 (32 columns of a product each) or 8 (16 each); same results.
+void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int waves = 4);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

@@ -67,7 +67,7 @@ enum TuneOp : int {
     TO_GEMM_PROJ = 1,  // block height of the attention-output GEMM
     TO_GEMM_FF = 2,    // block height of ff1 / ff2 when the MLP is not fused
     TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode: joint, split, plane)
-    TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), 1 fused
+    TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), fused on 1: 4 or 2: 8 waves
     TO_COUNT = 5
 };
 struct Tuning {

@@ -206,6 +206,131 @@ def cpu_baseline(st, cj, x0, known, meta, iters, seconds):
                       f"(restatement of engine::cpu extended to the cell_st form), {el:.1f} s"}
 
 
+def run_solver(dataset_states, seconds_cpu):
+    """The shallow-water solver on the device (crates/solvers/src/
+    shallow_water.rs), bit-identical to the reference's f32 step, in three
+    shapes, each timed with HIP events on the launch stream:
+      - dataset: dump-dataset's loop (dump_dataset.rs:168-193), B states of
+        64x64 (nu 1e-3), 80 ticks each, every frame written (resident
+        kernel, one launch); HBM carries 12 B per cell per frame plus the
+        state in and out;
+      - truth partner: one 64x64 state, 50 ticks (the rollout CLI's solver);
+      - large grid: one 4096x4096 state, 20 steps, one HBM launch per step (row march)
+        moving 24 B per cell (h, u, v in and out).
+    The CPU leg runs the oracle's restatement of ShallowWater::step on one
+    core at 64x64."""
+    import torch
+
+    from jacobi2000_amd import solvers as sw
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    # a stream of its own: torch's default stream is handle 0, which the
+    # C ABI reads as "the solver's own stream"
+    stream = torch.cuda.Stream(device=dev)
+    with torch.cuda.stream(stream):
+        return _run_solver(dataset_states, seconds_cpu, sw, O, dev, stream)
+
+
+def _run_solver(dataset_states, seconds_cpu, sw, O, dev, stream):
+    import ctypes
+
+    import torch
+
+    from jacobi2000_amd._lib import call
+
+    out = {}
+
+    def timed(fn, reps=1):
+        fn()  # warmup
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / 1e3 / reps
+
+    def state(p, batch, seed):
+        rng = np.random.default_rng(seed)
+        h = np.full((batch, p.cells()), p.depth, np.float32)
+        for i in range(batch):
+            h[i] += sw.drop_field(p, rng.uniform(8, p.nx - 8), rng.uniform(8, p.ny - 8), rng.uniform(0.02, 0.08),
+                                  rng.uniform(2.5, 6.0)).reshape(-1)
+        z = np.zeros_like(h)
+        return [torch.from_numpy(a).to(dev) for a in (h, z, z.copy())]
+
+    # dataset
+    p = sw.Params(nx=64, ny=64, nu=1e-3)
+    B, ticks, n = dataset_states, 80, 64 * 64
+    s = sw.ShallowWater(p)
+    h0, u0, v0 = state(p, B, 20)
+    h, u, v = h0.clone(), u0.clone(), v0.clone()
+    frames = ticks + 1
+    traj = torch.empty(B * 3 * frames * n, dtype=torch.float32, device=dev)
+    tk = torch.full((B,), ticks, dtype=torch.int32, device=dev)
+    off = torch.arange(B, dtype=torch.int64, device=dev) * (3 * frames * n)
+
+    def dataset():
+        h.copy_(h0), u.copy_(u0), v.copy_(v0)
+        call("j2k_sw_trajectories_dev", s._h, ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+             ctypes.c_void_p(v.data_ptr()), B, ctypes.c_void_p(tk.data_ptr()), ticks, ctypes.c_void_p(off.data_ptr()),
+             ctypes.c_void_p(traj.data_ptr()), None, None, None, ctypes.c_void_p(stream.cuda_stream))
+
+    copy_s = timed(lambda: (h.copy_(h0), u.copy_(u0), v.copy_(v0)), 3)
+    t = timed(dataset, 3) - copy_s
+    bytes_ = B * n * 4 * (3 * frames + 6)
+    out["dataset"] = {"workload": f"{B} states of 64x64, nu 1e-3, {ticks} ticks, all {frames} frames written",
+                      "kernel": "k_sw_resident<8, in place>", "ms": t * 1e3,
+                      "cell_ticks_per_s": B * n * ticks / t, "frames_per_s": B * ticks / t,
+                      "hbm_bytes": bytes_, "achieved_GBps": bytes_ / t / 1e9, "frac_hbm": bytes_ / t / PEAK["hbm"]}
+    del traj
+    # truth partner: one state, 50 ticks, one launch
+    h1, u1, v1 = state(p, 1, 1)
+    t = timed(lambda: s.steps_dev(h1.data_ptr(), u1.data_ptr(), v1.data_ptr(), 1, 50, stream.cuda_stream), 5)
+    out["truth_partner"] = {"workload": "one 64x64 state, 50 ticks in one launch (rollout's solver)",
+                            "us_per_tick": t / 50 * 1e6, "cell_ticks_per_s": n * 50 / t}
+    s.close()
+    # large grid, tiled
+    pl = sw.Params(nx=4096, ny=4096, nu=1e-3)
+    sl = sw.ShallowWater(pl)
+    hl, ul, vl = state(pl, 1, 2)
+    steps = 20
+    t = timed(lambda: sl.steps_dev(hl.data_ptr(), ul.data_ptr(), vl.data_ptr(), 1, steps, stream.cuda_stream), 2)
+    nl = pl.cells()
+    per_step = t / steps
+    # the same bytes moved by a plain device copy (3 fields in, 3 out): the
+    # practical ceiling of a step that reads and writes each cell once
+    cin = torch.empty(3 * nl, dtype=torch.float32, device=dev).fill_(1.0)
+    cout = torch.empty_like(cin)
+    tc = timed(lambda: cout.copy_(cin), 5)
+    del cin, cout
+    out["large_grid"] = {"workload": f"one 4096x4096 state, {steps} steps, one HBM launch per step (row march)",
+                         "kernel": "k_sw_band", "us_per_step": per_step * 1e6, "cell_steps_per_s": nl / per_step,
+                         "algorithmic_bytes_per_step": 24 * nl, "achieved_GBps": 24 * nl / per_step / 1e9,
+                         "frac_hbm": 24 * nl / per_step / PEAK["hbm"],
+                         "copy_same_bytes_GBps": 24 * nl / tc / 1e9, "vs_copy": tc / per_step}
+    sl.close()
+    del hl, ul, vl
+    # CPU: the oracle's step, one core, 64x64
+    if seconds_cpu > 0:
+        op = O.sw_params(nx=64, ny=64, nu=1e-3)
+        hh = np.full(n, 1.0, np.float32)
+        O.sw_add_drop(op, hh, 32.0, 32.0, 0.05, 4.0)
+        z = np.zeros(n, np.float32)
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds_cpu:
+            hh, uu, vv = O.sw_steps(op, hh, z, z, 200)
+            done += 200
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * done / el, "unit": "cell-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{done} steps of one 64x64 state, oracle restatement of "
+                                         f"ShallowWater::step, {el:.1f} s"}
+        out["dataset"]["vs_cpu_1core"] = out["dataset"]["cell_ticks_per_s"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def stream_requests(n, seed=0):
     """Mixed request stream: sizes uniform over {32,48,64,96}^2 x {8,16}
     slices, mask kinds cycling forward / rewind / reconstruction (2% of cells),
@@ -437,6 +562,8 @@ def main():
     ap.add_argument("--forms", default="f32,bf16,bf16x2",
                     help="forms to run (comma separated; f32 and the headline are always included)")
     ap.add_argument("--no-plain", action="store_true", help="skip the f32 plain form (profiling runs)")
+    ap.add_argument("--solver", type=int, default=2048, metavar="STATES",
+                    help="also time the device shallow-water solver with this many 64x64 dataset states (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure the dominant kernel's HBM bytes")
     args = ap.parse_args()
@@ -538,6 +665,9 @@ def main():
                             "unit": "grid-cells/s over the whole trajectories",
                             "served": r_served, "plain_fp32": r_plain,
                             "served_vs_plain": r_served["value"] / r_plain["value"]}
+    if args.solver > 0:
+        note(f"solver: {args.solver} dataset states, truth partner, 4096x4096 grid")
+        line["solver"] = run_solver(args.solver, 0.0 if args.no_cpu_baseline else 3.0)
     if not args.no_cpu_baseline:
         note(f"cpu baseline: about {args.cpu_seconds:g} s of the CPU oracle")
         line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)

@@ -231,6 +231,66 @@ int j2k_server_drain(void* server);
 int j2k_server_stats(void* server, uint64_t* launches, uint64_t* graph_replays, int* peak_active, size_t* peak_bytes);
 int j2k_server_destroy(void* server);
 
+/* ---- shallow-water solver (crates/solvers/src/shallow_water.rs) ----------
+ *
+ * The reference solver stepped on the device, bit for bit: the same f32
+ * operations in the same order (no contraction) as ShallowWater::step
+ * (shallow_water.rs:149-238). A state is three fields h, u, v of ny*nx
+ * floats, row-major (index y*nx + x); a batch of B states holds each field
+ * as [B][ny][nx]. */
+
+/* Params (shallow_water.rs:16-29). */
+typedef struct {
+    int nx, ny;
+    float dx, dt, g, depth, nu;
+} j2k_sw_params;
+
+/* ShallowWater::new (shallow_water.rs:136-147): a solver for one grid on
+ * `device`. nx, ny >= 1 and nx*ny < 2^31 (J2K_ERR_INVALID otherwise). */
+int j2k_sw_create(const j2k_sw_params* params, int device, void** out_solver);
+int j2k_sw_destroy(void* solver);
+/* ShallowWater::steps (shallow_water.rs:240-244) on `batch` states in host
+ * buffers, in place. */
+int j2k_sw_steps(void* solver, float* h, float* u, float* v, int batch, int steps);
+/* The same on device buffers, enqueued on `stream` (NULL: the solver's own). */
+int j2k_sw_steps_dev(void* solver, float* d_h, float* d_u, float* d_v, int batch, int steps, void* stream);
+
+/* A drop of dump-dataset's schedule (dump_dataset.rs:85-95, apply_pending):
+ * after step `tick` (1-based) of its state, h += fields[field][ny][nx], the
+ * bump State::add_drop adds (see j2k_sw_drop_field). */
+typedef struct {
+    int tick, field;
+} j2k_sw_drop;
+
+/* Solver trajectories, dump-dataset's loop (dump_dataset.rs:168-193) over
+ * `batch` independent states: state b runs min(ticks[b], max_ticks) steps;
+ * frame 0 is its input, frame t the state after step t and after the drops
+ * of tick t (applied in array order). Frames of state b go to
+ * traj + offsets[b], laid out [3][frames][ny][nx] (every h frame, then u,
+ * then v: the traj-NNNNN.f32 layout) with frames = min(ticks[b], max_ticks)+1;
+ * traj NULL records nothing. Drops of state b are drops[drop_begin[b] ..
+ * drop_begin[b+1]); drop_begin NULL means none. ticks, offsets, drop_begin,
+ * drops and fields are device arrays; h, u, v advance in place. */
+int j2k_sw_trajectories_dev(void* solver, float* d_h, float* d_u, float* d_v, int batch, const int* d_ticks,
+                            int max_ticks, const int64_t* d_offsets, float* d_traj, const int* d_drop_begin,
+                            const j2k_sw_drop* d_drops, const float* d_fields, void* stream);
+/* The same from host buffers: ticks[batch]; traj holds every state's frames
+ * back to back (state b at the sum over earlier states of 3*(ticks+1)*ny*nx);
+ * drops given as n_drops (state, tick, field) triples, applied per state in
+ * array order; fields [n_fields][ny][nx]. */
+int j2k_sw_trajectories(void* solver, float* h, float* u, float* v, int batch, const int* ticks, float* traj,
+                        int n_drops, const int* drop_state_tick_field, const float* fields, int n_fields);
+/* State::add_drop's bump (shallow_water.rs:102-111) on the host, as the
+ * reference computes it in f32: out[y*nx+x] = amp*exp(-(dx*dx+dy*dy)*inv),
+ * inv = 1/(2*sigma*sigma). Adding it to h is State::add_drop. */
+int j2k_sw_drop_field(const j2k_sw_params* params, float cx, float cy, float amp, float sigma, float* out);
+/* Kernel choice, no reference counterpart: 0 picks by grid size (a grid of
+ * up to 6,826 cells runs every step of a state inside one workgroup's
+ * LDS, a larger one runs one launch per step through HBM), 1 forces the
+ * resident kernel (J2K_ERR_LIMIT if the grid does not fit), 2 forces the
+ * HBM one. Both give identical results. */
+int j2k_sw_set_kernel(void* solver, int kernel);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,16 @@ class Result(ctypes.Structure):
                 ("start_s", ctypes.c_double), ("done_s", ctypes.c_double)]
 
 
+class SwParams(ctypes.Structure):
+    """j2k_sw_params: crates/solvers/src/shallow_water.rs:16-29."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int)] + [(f, ctypes.c_float) for f in ("dx", "dt", "g", "depth",
+                                                                                               "nu")]
+
+
+class SwDrop(ctypes.Structure):
+    _fields_ = [("tick", ctypes.c_int), ("field", ctypes.c_int)]
+
+
 P = ctypes.c_void_p
 I = ctypes.c_int
 F = ctypes.c_float
@@ -86,6 +96,14 @@ PROTOS = {
     "j2k_server_stats": [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(I),
                          ctypes.POINTER(ctypes.c_size_t)],
     "j2k_server_destroy": [P],
+    "j2k_sw_create": [ctypes.POINTER(SwParams), I, PP],
+    "j2k_sw_destroy": [P],
+    "j2k_sw_set_kernel": [P, I],
+    "j2k_sw_steps": [P, P, P, P, I, I],
+    "j2k_sw_steps_dev": [P, P, P, P, I, I, P],
+    "j2k_sw_trajectories_dev": [P, P, P, P, I, P, I, P, P, P, P, P, P],
+    "j2k_sw_trajectories": [P, P, P, P, I, P, P, I, P, P, I],
+    "j2k_sw_drop_field": [ctypes.POINTER(SwParams), F, F, F, F, P],
 }
 
 _lib = None

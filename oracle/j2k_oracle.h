@@ -101,6 +101,23 @@ float orc_gelu_f32(float x);
 /* Threads used by the parallel row loops (OpenMP); 0 = library default. */
 void orc_set_threads(int n);
 
+/* Shallow-water solver (crates/solvers/src/shallow_water.rs), sw_oracle.c. */
+typedef struct {
+    int nx, ny;
+    float dx, dt, g, depth, nu;
+} orc_sw_params;
+/* ShallowWater::steps (shallow_water.rs:149-244), in place. */
+void orc_sw_steps(const orc_sw_params* p, float* h, float* u, float* v, int steps);
+/* State::add_drop (shallow_water.rs:102-111). */
+void orc_sw_add_drop(const orc_sw_params* p, float* h, float cx, float cy, float amp, float sigma);
+/* One dump-dataset trajectory (dump_dataset.rs:168-193): traj [3][ticks+1][n]
+ * (may be NULL); drops are (tick, field) pairs applied after that tick's
+ * step in array order, fields [.][n]. */
+void orc_sw_trajectory(const orc_sw_params* p, float* h, float* u, float* v, int ticks, float* traj, int n_drops,
+                       const int* drop_tick_field, const float* fields);
+/* metrics.rs:9-23. */
+float orc_correlation(const float* a, const float* b, int n);
+
 #ifdef __cplusplus
 }
 #endif

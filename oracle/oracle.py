@@ -325,3 +325,57 @@ def energy_violation(x: np.ndarray, p: Phys, eref_t) -> float:
 
 def threads_default() -> int:
     return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+# ---- shallow-water solver (sw_oracle.c) ----------------------------------------
+
+class SwParams(ctypes.Structure):
+    """crates/solvers/src/shallow_water.rs:16-29."""
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("dx", ctypes.c_float), ("dt", ctypes.c_float),
+                ("g", ctypes.c_float), ("depth", ctypes.c_float), ("nu", ctypes.c_float)]
+
+
+def sw_params(nx=64, ny=64, dx=1.0, dt=None, g=9.81, depth=1.0, nu=0.0) -> SwParams:
+    """Params { .., ..Default::default() }: dt None takes the default's,
+    with_cfl_dt(0.4) of dx 1, g 9.81, depth 1 (shallow_water.rs:31-58),
+    evaluated in f32 as the reference does."""
+    if dt is None:
+        f = np.float32
+        c = np.sqrt(f(9.81) * f(1.0), dtype=f)
+        dt = f(0.4) * f(1.0) / (c * f(np.sqrt(2.0)))
+    return SwParams(nx, ny, dx, float(dt), g, depth, nu)
+
+
+def sw_steps(p: SwParams, h, u, v, steps: int):
+    """ShallowWater::steps on copies; returns (h, u, v)."""
+    h, u, v = (np.array(a, dtype=np.float32, order="C").copy() for a in (h, u, v))
+    lib().orc_sw_steps(ctypes.byref(p), _ptr(h), _ptr(u), _ptr(v), int(steps))
+    return h, u, v
+
+
+def sw_add_drop(p: SwParams, h: np.ndarray, cx, cy, amp, sigma) -> None:
+    """State::add_drop, in place on an f32 array."""
+    assert h.dtype == np.float32 and h.flags.c_contiguous
+    f = ctypes.c_float
+    lib().orc_sw_add_drop(ctypes.byref(p), _ptr(h), f(cx), f(cy), f(amp), f(sigma))
+
+
+def sw_trajectory(p: SwParams, h, u, v, ticks: int, drops=(), fields=None):
+    """One dump-dataset trajectory: returns (traj [3, ticks+1, ny, nx], final h, u, v).
+    drops: (tick, field) pairs into fields [k, ny, nx]."""
+    h, u, v = (np.array(a, dtype=np.float32, order="C").copy() for a in (h, u, v))
+    traj = np.empty((3, ticks + 1, p.ny, p.nx), dtype=np.float32)
+    d = np.ascontiguousarray(np.array(drops, dtype=np.int32).reshape(-1, 2))
+    fl = np.zeros((1, p.ny, p.nx), np.float32) if fields is None else np.ascontiguousarray(fields, dtype=np.float32)
+    lib().orc_sw_trajectory(ctypes.byref(p), _ptr(h), _ptr(u), _ptr(v), int(ticks), _ptr(traj), len(d), _ptr(d),
+                            _ptr(fl))
+    return traj, h, u, v
+
+
+def correlation(a, b) -> float:
+    """metrics.rs:9-23."""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=np.float32).reshape(-1)
+    f = lib().orc_correlation
+    f.restype = ctypes.c_float
+    return float(f(_ptr(a), _ptr(b), a.size))

@@ -280,11 +280,11 @@ constexpr int SW_LDS_CELLS = SW_LDS_BYTES / 24;          // two states, wider gr
 
 // One workgroup per state, every tick inside LDS; W waves share the
 // state's (band, row chunk) items.
-//  - INPLACE (one band, nx <= 64): each wave owns a chunk of rows and first
-//    copies the two rows above and below it into registers; after a
-//    barrier it overwrites its own rows in place (the march has read a row
-//    before it writes it), then a second barrier ends the step. 12 B of LDS
-//    per cell.
+//  - INPLACE (one band, nx <= 64): each wave owns a chunk of rows and
+//    overwrites them in place (the march has read a row before it writes
+//    it), except its first and last two rows, which the neighbouring chunks
+//    read as their halo: those wait in registers for a barrier, then a
+//    second barrier ends the step. 12 B of LDS per cell.
 //  - otherwise bands share halo columns, so the state ping-pongs between
 //    two LDS copies, one barrier per step. 24 B per cell.
 template <int W, bool INPLACE, bool EXTRA>
@@ -310,44 +310,62 @@ __global__ __launch_bounds__(64 * W) void k_sw_resident(SwConst k, float* gh, fl
     }
     __syncthreads();
     const int items = g.bands * g.chunks;
+    SwRow keep[4] = {};  // in place: a chunk's boundary rows, written after the barrier
+    int kept_y0 = 0, kept_y1 = 0, kept_i = 0;
+    bool kept_out = false;
     for (int t = 0; t < T; t++) {
         if (INPLACE) {
-            // items <= W: at most one chunk per wave
-            const bool busy = wave < items;
-            bool out;
-            const SwLane L = sw_lane(k, g, 0, lane, out);
-            const int y0 = wave * g.rows, y1 = min(y0 + g.rows, k.ny);
-            auto lds_row = [&](int y) -> SwRow {
-                const int i = min(max(y, 0), k.ny - 1) * k.nx + L.cx;
-                return SwRow{A[i], A[n + i], A[2 * n + i]};
-            };
-            SwRow t2{}, t1{}, b0{}, b1{};
-            if (busy) {
-                t2 = lds_row(y0 - 2);
-                t1 = lds_row(y0 - 1);
-                b0 = lds_row(y1);
-                b1 = lds_row(y1 + 1);
-            }
-            __syncthreads();  // every halo row is in registers before any row changes
-            if (busy) {
+            // items <= W: at most one chunk per wave. A chunk's first two and
+            // last two rows are the only ones another wave reads (as its
+            // halo): their new values wait in registers until every wave has
+            // finished reading, the other rows are written in place at once.
+            if (wave < items) {
+                bool out;
+                const SwLane L = sw_lane(k, g, 0, lane, out);
+                const int y0 = wave * g.rows, y1 = min(y0 + g.rows, k.ny);
                 const SwDrops dr = EXTRA ? sw_drops_of(run, b, t + 1) : SwDrops{0, 0, false};
                 auto fetch = [&](int y) -> SwRow {
-                    if (y == y0 - 2) return t2;
-                    if (y == y0 - 1) return t1;
-                    if (y == y1) return b0;
-                    if (y == y1 + 1) return b1;
-                    return lds_row(y);
+                    const int i = min(max(y, 0), k.ny - 1) * k.nx + L.cx;
+                    return SwRow{A[i], A[n + i], A[2 * n + i]};
                 };
                 auto emit = [&](int y, bool live, float h, float u, float v) {
-                    if (!live || !out) return;
-                    const int i = y * k.nx + L.gx;
-                    h = sw_drops(run, dr, t + 1, n, i, h);
-                    A[i] = h;
-                    A[n + i] = u;
-                    A[2 * n + i] = v;
-                    if (fb) sw_frame(fb, T + 1, t + 1, n, i, h, u, v);
+                    if (!live) return;
+                    const int i = y * k.nx + L.cx;
+                    if (EXTRA) h = sw_drops(run, dr, t + 1, n, i, h);
+                    if (EXTRA && fb && out) sw_frame(fb, T + 1, t + 1, n, i, h, u, v);
+                    const SwRow r{h, u, v};
+                    if (y == y0) keep[0] = r;
+                    else if (y == y0 + 1) keep[1] = r;
+                    else if (y == y1 - 2) keep[2] = r;
+                    else if (y == y1 - 1) keep[3] = r;
+                    else if (out) {
+                        A[i] = h;
+                        A[n + i] = u;
+                        A[2 * n + i] = v;
+                    }
                 };
                 sw_march<0>(k, L, y0, y1, fetch, emit);
+                kept_y0 = y0;
+                kept_y1 = y1;
+                kept_out = out;
+                kept_i = L.cx;
+            }
+            __syncthreads();  // every wave has read its halo rows
+            if (wave < items) {
+                const int y0 = kept_y0, y1 = kept_y1;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int y = j < 2 ? y0 + j : y1 - 4 + j;
+                    // rows of a chunk under four rows are kept once, under
+                    // their first slot
+                    const bool mine = y < y1 && (j < 2 || y >= y0 + 2);
+                    if (mine && kept_out) {
+                        const int i = y * k.nx + kept_i;
+                        A[i] = keep[j].h;
+                        A[n + i] = keep[j].u;
+                        A[2 * n + i] = keep[j].v;
+                    }
+                }
             }
             __syncthreads();
         } else {

@@ -231,6 +231,52 @@ int j2k_server_drain(void* server);
 int j2k_server_stats(void* server, uint64_t* launches, uint64_t* graph_replays, int* peak_active, size_t* peak_bytes);
 int j2k_server_destroy(void* server);
 
+/* ---- scores and the rollout's truth partner -------------------------------- */
+
+/* eval-metrics' scores of one trajectory at one horizon
+ * (crates/engine-cli/src/eval.rs:158-172). */
+typedef struct {
+    float corr;            /* metrics::correlation over (h, u, v), normalized fields */
+    float corr_h;          /* the same over h                                       */
+    float amplitude_ratio; /* peak |h| of the model over the solver's, decoded h    */
+    float lag;             /* wavefront lag in cells: xcorr_lag of the radial |h|   */
+                           /* profiles about the drop centre, max lag given         */
+    float bias;            /* (mean model h - mean solver h) / std solver h         */
+} j2k_eval_score;
+
+/* Scores of `batch` patch2d trajectories on the device: d_x [batch][3][ny][nx]
+ * normalized model states, d_h / d_u / d_v [batch][ny][nx] the solver's
+ * physical states (encoded with the model's normalizer, norm.rs:27-34),
+ * centers [batch][2] the drop centres (cx, cy) in cells (host), max_lag the
+ * profile lag range (8 in eval.rs:167). Sums run in f64 in a fixed order;
+ * the lag is finished on the host as metrics.rs:78-107 does. out [batch]
+ * (host). Synchronises the stream (NULL: the model's). */
+int j2k_score_dev(void* model, int batch, const float* d_x, const float* d_h, const float* d_u, const float* d_v,
+                  const float* centers, int max_lag, j2k_eval_score* out, void* stream);
+/* The same from host buffers. */
+int j2k_score(void* model, int batch, const float* x, const float* h, const float* u, const float* v,
+              const float* centers, int max_lag, j2k_eval_score* out);
+/* metrics::correlation (metrics.rs:9-23) of d_a*sa + ta against d_b*sb + tb
+ * over n values, on the current device; the f32 result goes to d_out[0]
+ * (device). decode_h is sa = std[0], ta = mean[0]; the solver's h - depth
+ * is sb = 1, tb = -depth. */
+int j2k_correlation_dev(const float* d_a, float sa, float ta, const float* d_b, float sb, float tb, int n,
+                        float* d_out, void* stream);
+/* The rollout CLI's loop (crates/engine-cli/src/main.rs:116-127) with its
+ * truth partner on the device too: the solver (a j2k_sw handle on the
+ * model's grid and device) advances the physical state h, u, v [ny][nx] one
+ * tick per step, the model steps the normalized state x [3][ny][nx] with the
+ * mass projection to the initial mass (f64 sums), and corr[step-1] gets the
+ * correlation of decode_h(x) with the solver's h - depth. Everything stays
+ * on the device until the end: x and h, u, v return their final values,
+ * corr [steps] the trace (host buffers). */
+int j2k_rollout_truth(void* model, void* solver, float* x_io, const float* meta, float* h, float* u, float* v,
+                      int steps, float* corr);
+/* The same on device buffers (corr: device, [steps]); enqueued on `stream`
+ * (NULL: the model's). */
+int j2k_rollout_truth_dev(void* model, void* solver, float* d_x, const float* d_meta, float* d_h, float* d_u,
+                          float* d_v, int steps, float* d_corr, void* stream);
+
 /* ---- shallow-water solver (crates/solvers/src/shallow_water.rs) ----------
  *
  * The reference solver stepped on the device, bit for bit: the same f32

@@ -55,6 +55,11 @@ class SwParams(ctypes.Structure):
                                                                                                "nu")]
 
 
+class Score(ctypes.Structure):
+    """j2k_eval_score: eval-metrics' scores (crates/engine-cli/src/eval.rs:158-172)."""
+    _fields_ = [(f, ctypes.c_float) for f in ("corr", "corr_h", "amplitude_ratio", "lag", "bias")]
+
+
 class SwDrop(ctypes.Structure):
     _fields_ = [("tick", ctypes.c_int), ("field", ctypes.c_int)]
 
@@ -96,6 +101,11 @@ PROTOS = {
     "j2k_server_stats": [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(I),
                          ctypes.POINTER(ctypes.c_size_t)],
     "j2k_server_destroy": [P],
+    "j2k_score_dev": [P, I, P, P, P, P, P, I, P, P],
+    "j2k_score": [P, I, P, P, P, P, P, I, P],
+    "j2k_correlation_dev": [P, F, F, P, F, F, I, P, P],
+    "j2k_rollout_truth": [P, P, P, P, P, P, P, I, P],
+    "j2k_rollout_truth_dev": [P, P, P, P, P, P, P, I, P, P],
     "j2k_sw_create": [ctypes.POINTER(SwParams), I, PP],
     "j2k_sw_destroy": [P],
     "j2k_sw_set_kernel": [P, I],

@@ -8,6 +8,7 @@
 #include "abi.h"
 #include "cell.h"
 #include "model.h"
+#include "score.h"
 
 namespace j2k {
 
@@ -59,6 +60,37 @@ void rollout_dev(Model& m, hipStream_t s, float* x, const float* meta, int steps
         project_p(s, 1, n, nxt, mass0, pm, e + t - 1, pe, e + t, c.g, c.norm_mean, c.norm_std, c.norm_depth,
                   c.energy_r_max);
         std::swap(cur, nxt);
+    }
+    if (cur != x) HIPCHK(hipMemcpyAsync(x, cur, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
+}
+
+// main.rs:116-127 with the solver's frames: the truth partner runs every
+// tick in one launch, then per step the model step, the mass projection and
+// the correlation of decode_h(x) against frame t's h - depth.
+void rollout_truth_dev(Model& m, hipStream_t s, void* solver, float* x, const float* meta, float* h, float* u,
+                       float* v, int steps, float* corr) {
+    const j2k_config& c = m.cfg;
+    int snx = 0, sny = 0, sdev = 0;
+    float sdepth = 0.0f;
+    sw_grid_of(solver, &snx, &sny, &sdepth, &sdev);
+    if (snx != c.nx || sny != c.ny)
+        fail(J2K_ERR_SHAPE, "rollout truth: the solver's grid is " + std::to_string(snx) + "x" + std::to_string(sny) +
+                                ", the model's " + std::to_string(c.nx) + "x" + std::to_string(c.ny));
+    if (sdev != m.device) fail(J2K_ERR_INVALID, "rollout truth: the solver and the model are on different devices");
+    const int n = c.ny * c.nx;
+    const float* frames = sw_frames_on(solver, s, h, u, v, steps);
+    float* xb = m.get<float>(S_WIN_B, 3 * (size_t)n);
+    double* mass0 = m.get<double>(S_SUMS, 2);
+    grid_sums(s, 1, n, x, mass0, mass0 + 1, c.g, c.norm_mean, c.norm_std, c.norm_depth);
+    float* cur = x;
+    float* nxt = xb;
+    for (int t = 1; t <= steps; t++) {
+        forward_p(m, s, 1, cur, meta, nxt, true);
+        project_p(s, 1, n, nxt, mass0, 1, nullptr, 0, nullptr, c.g, c.norm_mean, c.norm_std, c.norm_depth,
+                  c.energy_r_max);
+        std::swap(cur, nxt);
+        // metrics.rs:111-113 decode_h against main.rs:123 h - depth
+        corr_affine(s, cur, c.norm_std[0], c.norm_mean[0], frames + (size_t)t * n, 1.0f, -sdepth, n, corr + t - 1);
     }
     if (cur != x) HIPCHK(hipMemcpyAsync(x, cur, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
@@ -486,6 +518,105 @@ int j2k_cell_forward_dev(void* h, const float* d_grid, const uint8_t* d_known, c
         float* x4 = m.get<float>(S_X0, 4 * (size_t)run.pack.M);
         cell_pack(s, run.pack.M, d_grid, d_known, x4);
         cell_model(m, s, run.pack, x4, d_y);
+    });
+}
+
+int j2k_score_dev(void* h, int batch, const float* d_x, const float* d_h, const float* d_u, const float* d_v,
+                  const float* centers, int max_lag, j2k_eval_score* out, void* st) {
+    return guard([&] {
+        Model& m = model_of(h);
+        need_form(m, J2K_FORM_PATCH2D, "j2k_score_dev");
+        if (batch < 0) fail(J2K_ERR_INVALID, "batch must be >= 0");
+        if (max_lag < 0) fail(J2K_ERR_INVALID, "max_lag must be >= 0");
+        if (batch == 0) return;
+        need(d_x, "x");
+        need(d_h, "h");
+        need(d_u, "u");
+        need(d_v, "v");
+        need(centers, "centers");
+        need(out, "out");
+        DeviceGuard dg(m.device);
+        score_batch(m, stream_of(m, st), batch, d_x, d_h, d_u, d_v, centers, max_lag, out);
+    });
+}
+
+int j2k_score(void* h, int batch, const float* x, const float* hh, const float* u, const float* v,
+              const float* centers, int max_lag, j2k_eval_score* out) {
+    return guard([&] {
+        Model& m = model_of(h);
+        need_form(m, J2K_FORM_PATCH2D, "j2k_score");
+        if (batch < 0) fail(J2K_ERR_INVALID, "batch must be >= 0");
+        if (max_lag < 0) fail(J2K_ERR_INVALID, "max_lag must be >= 0");
+        if (batch == 0) return;
+        need(x, "x");
+        need(hh, "h");
+        need(u, "u");
+        need(v, "v");
+        need(centers, "centers");
+        need(out, "out");
+        DeviceGuard dg(m.device);
+        hipStream_t s = m.stream;
+        const size_t n = (size_t)m.cfg.nx * m.cfg.ny, cells = (size_t)batch * n;
+        float* dx = m.get<float>(S_X0, 3 * cells);
+        float* ds = m.get<float>(S_XA, 3 * cells);
+        h2d(dx, x, 3 * cells * 4, s);
+        h2d(ds, hh, cells * 4, s);
+        h2d(ds + cells, u, cells * 4, s);
+        h2d(ds + 2 * cells, v, cells * 4, s);
+        score_batch(m, s, batch, dx, ds, ds + cells, ds + 2 * cells, centers, max_lag, out);
+    });
+}
+
+int j2k_rollout_truth_dev(void* h, void* solver, float* d_x, const float* d_meta, float* d_h, float* d_u, float* d_v,
+                          int steps, float* d_corr, void* st) {
+    return guard([&] {
+        Model& m = model_of(h);
+        need_form(m, J2K_FORM_PATCH2D, "j2k_rollout_truth_dev");
+        need(solver, "solver");
+        need(d_x, "x");
+        need(d_meta, "meta");
+        need(d_h, "h");
+        need(d_u, "u");
+        need(d_v, "v");
+        if (steps < 0) fail(J2K_ERR_INVALID, "steps must be >= 0");
+        if (steps > 0) need(d_corr, "corr");
+        DeviceGuard dg(m.device);
+        rollout_truth_dev(m, stream_of(m, st), solver, d_x, d_meta, d_h, d_u, d_v, steps, d_corr);
+    });
+}
+
+int j2k_rollout_truth(void* h, void* solver, float* x_io, const float* meta, float* hh, float* u, float* v, int steps,
+                      float* corr) {
+    return guard([&] {
+        Model& m = model_of(h);
+        need_form(m, J2K_FORM_PATCH2D, "j2k_rollout_truth");
+        need(solver, "solver");
+        need(x_io, "x");
+        need(meta, "meta");
+        need(hh, "h");
+        need(u, "u");
+        need(v, "v");
+        if (steps < 0) fail(J2K_ERR_INVALID, "steps must be >= 0");
+        if (steps > 0) need(corr, "corr");
+        DeviceGuard dg(m.device);
+        const j2k_config& c = m.cfg;
+        const size_t n = (size_t)c.nx * c.ny;
+        hipStream_t s = m.stream;
+        float* dx = m.get<float>(S_GRID, 3 * n);
+        float* dm = m.get<float>(S_META, c.meta_dim);
+        float* ds = m.get<float>(S_XA, 3 * n + (size_t)steps + 1);
+        float* dc = ds + 3 * n;
+        h2d(dx, x_io, 3 * n * 4, s);
+        h2d(dm, meta, (size_t)c.meta_dim * 4, s);
+        h2d(ds, hh, n * 4, s);
+        h2d(ds + n, u, n * 4, s);
+        h2d(ds + 2 * n, v, n * 4, s);
+        rollout_truth_dev(m, s, solver, dx, dm, ds, ds + n, ds + 2 * n, steps, dc);
+        if (steps > 0) d2h(corr, dc, (size_t)steps * 4, s);
+        d2h(x_io, dx, 3 * n * 4, s);
+        d2h(hh, ds, n * 4, s);
+        d2h(u, ds + n, n * 4, s);
+        d2h(v, ds + 2 * n, n * 4, s);
     });
 }
 

@@ -24,6 +24,7 @@
 
 #include "abi.h"
 #include "common.h"
+#include "score.h"
 
 #pragma clang fp contract(off)
 
@@ -521,6 +522,10 @@ struct Solver {
     size_t pp_floats = 0;
     float* io = nullptr;
     size_t io_floats = 0;
+    // frames of the model loops' truth partner, and its one frame offset (0)
+    float* fr = nullptr;
+    size_t fr_floats = 0;
+    int64_t* off0 = nullptr;
 
     ~Solver() {
         int prev = -1;
@@ -529,6 +534,8 @@ struct Solver {
         if (stream) (void)hipStreamSynchronize(stream);
         if (pp) (void)hipFree(pp);
         if (io) (void)hipFree(io);
+        if (fr) (void)hipFree(fr);
+        if (off0) (void)hipFree(off0);
         if (stream) (void)hipStreamDestroy(stream);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
@@ -687,6 +694,31 @@ void check_batch(int batch, int steps) {
 }
 
 }  // namespace
+
+float* sw_frames_on(void* solver, hipStream_t st, float* h, float* u, float* v, int steps) {
+    Solver& s = solver_of(solver);
+    Dev dg(s.device);
+    if (!s.off0) {
+        HIPCHK(hipMalloc(&s.off0, sizeof(int64_t)));
+        const int64_t zero = 0;
+        HIPCHK(hipMemcpy(s.off0, &zero, sizeof(zero), hipMemcpyHostToDevice));
+    }
+    float* fr = grow(s.fr, s.fr_floats, 3 * (size_t)(steps + 1) * s.cells());
+    SwRun r = plain_run(1, steps);
+    r.offsets = s.off0;
+    r.traj = fr;
+    run(s, st, h, u, v, r);
+    return fr;
+}
+
+void sw_grid_of(const void* solver, int* nx, int* ny, float* depth, int* device) {
+    const Solver& s = solver_of(const_cast<void*>(solver));
+    *nx = s.p.nx;
+    *ny = s.p.ny;
+    *depth = s.p.depth;
+    *device = s.device;
+}
+
 }  // namespace j2k
 
 using namespace j2k;

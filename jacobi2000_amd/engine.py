@@ -132,6 +132,36 @@ class Model:
              ctypes.byref(v))
         return s, e, v.value
 
+    def rollout_truth(self, solver, state, h, u, v, meta, steps: int):
+        """The rollout CLI's loop (engine-cli main.rs:116-127) with its solver
+        on the device: the solver (solvers.ShallowWater on this grid) steps
+        the physical (h, u, v), the model steps the normalized state with the
+        mass projection, and each step's correlation of decode_h(state) with
+        h - depth is recorded. Returns (state, h, u, v, corr [steps])."""
+        x = _f32(state).reshape(-1).copy()
+        hh, uu, vv = (_f32(a).reshape(-1).copy() for a in (h, u, v))
+        m = _f32(meta).reshape(-1)
+        self._check_p(x, m)
+        corr = np.zeros(max(steps, 0), dtype=np.float32)
+        call("j2k_rollout_truth", self._h, solver._h, _ptr(x), _ptr(m), _ptr(hh), _ptr(uu), _ptr(vv), steps,
+             _ptr(corr))
+        return x, hh, uu, vv, corr
+
+    def score(self, x, h, u, v, centers, max_lag: int = 8) -> list[dict]:
+        """eval-metrics' scores (engine-cli eval.rs:158-172) of a batch of
+        model states x [B, 3, ny, nx] (normalized) against solver states
+        h, u, v [B, ny, nx] (physical), drop centres [B, 2], on the device."""
+        xx = _f32(x).reshape(-1)
+        hh, uu, vv = (_f32(a).reshape(-1) for a in (h, u, v))
+        n = self.cfg.nx * self.cfg.ny
+        B = hh.size // n
+        c = _f32(centers).reshape(-1)
+        if xx.size != 3 * B * n or uu.size != B * n or vv.size != B * n or c.size != 2 * B:
+            raise J2KError(_lib.ERR_SHAPE, "score: x must be [B, 3, ny, nx], h/u/v [B, ny, nx], centers [B, 2]")
+        out = (_lib.Score * B)()
+        call("j2k_score", self._h, B, _ptr(xx), _ptr(hh), _ptr(uu), _ptr(vv), _ptr(c), max_lag, out)
+        return [{f: getattr(o, f) for f, _ in _lib.Score._fields_} for o in out]
+
     def iterate_window(self, window, meta, iters: int, project_mass: bool = False,
                        project_energy: bool = False) -> np.ndarray:
         """Jacobi-over-time window [T, channels, ny, nx] (slice 0 = initial state)."""

@@ -5,8 +5,12 @@
 //
 // Steps the shallow-water solver beside the model from the same initial
 // condition and prints the model's correlation against it every step
-// (main.rs:99-142). The model runs on the GPU; the CPU engine is the
-// reference's own, so `--backend cpu` is refused with exit code 2.
+// (main.rs:99-142). The whole loop runs on the GPU (j2k_rollout_truth):
+// the solver, the model step with its mass projection and the correlation,
+// with one copy in and one out; the lines print when it returns. The CPU
+// engine is the reference's own, so `--backend cpu` is refused with exit
+// code 2.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -122,24 +126,17 @@ int main(int argc, char** argv) {
         const engine::Normalizer norm = model.norm();
         std::vector<float> x;
         norm.encode(truth.h, truth.u, truth.v, x);
-        const float mass0 = (float)engine::project::mass(model, x, n);
         solvers::ShallowWater solver(p);
-        std::vector<float> model_h(n), solver_h(n);
+        std::vector<float> corr((size_t)std::max(a.steps, 0));
         const auto t0 = std::chrono::steady_clock::now();
-        for (int step = 1; step <= a.steps; step++) {
-            solver.step(truth);
-            std::vector<float> next = model.step(x, meta.data());
-            engine::project::project_mass(model, next, mass0, n);
-            x.swap(next);
-            for (int i = 0; i < n; i++) {
-                solver_h[i] = truth.h[i] - p.depth;
-                model_h[i] = x[i] * norm.std[0] + norm.mean[0];  // metrics.rs decode_h
-            }
-            std::printf("step %4d corr=%.5f\n", step, solvers::correlation(model_h, solver_h));
-        }
+        engine::check(j2k_rollout_truth(model.handle(), solver.handle(), x.data(), meta.data(), truth.h.data(),
+                                        truth.u.data(), truth.v.data(), a.steps, corr.data()));
         const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int step = 1; step <= a.steps; step++) std::printf("step %4d corr=%.5f\n", step, corr[step - 1]);
         std::printf("%d steps in %.1f ms (%.2f ms/step, hip)\n", a.steps, ms, ms / (a.steps > 0 ? a.steps : 1));
         if (!a.out.empty()) {
+            std::vector<float> model_h(n);
+            for (int i = 0; i < n; i++) model_h[i] = x[i] * norm.std[0] + norm.mean[0];  // metrics.rs decode_h
             write_npy(a.out, model_h, p.ny, p.nx);
             std::printf("wrote %s\n", a.out.c_str());
         }

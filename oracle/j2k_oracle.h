@@ -118,6 +118,19 @@ void orc_sw_trajectory(const orc_sw_params* p, float* h, float* u, float* v, int
 /* metrics.rs:9-23. */
 float orc_correlation(const float* a, const float* b, int n);
 
+/* The other field scores (crates/engine/src/metrics.rs), metrics_oracle.c. */
+float orc_peak(const float* f, int n);
+float orc_mean(const float* f, int n);
+float orc_std_dev(const float* f, int n);
+float orc_bias(const float* model, const float* solver, int n);
+/* Radial profile into out (at most cap bins); returns the bin count or -1. */
+int orc_radial_profile(const float* f, int nx, int ny, float cx, float cy, double* out, int cap);
+float orc_xcorr_lag(const double* model, const double* solver, int n, int max_lag);
+/* norm.rs:27-34 encode and metrics.rs:109-113 decode_h. */
+void orc_encode(const float* h, const float* u, const float* v, int n, const float* mean, const float* stdv,
+                float depth, float* out);
+void orc_decode_h(const float* x, int n, const float* mean, const float* stdv, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -379,3 +379,85 @@ def correlation(a, b) -> float:
     f = lib().orc_correlation
     f.restype = ctypes.c_float
     return float(f(_ptr(a), _ptr(b), a.size))
+
+
+# ---- field scores (metrics_oracle.c) -------------------------------------------
+
+def _f32v(a):
+    return np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+
+
+def _fn(name, restype=ctypes.c_float):
+    f = getattr(lib(), name)
+    f.restype = restype
+    return f
+
+
+def peak(f) -> float:
+    f = _f32v(f)
+    return float(_fn("orc_peak")(_ptr(f), f.size))
+
+
+def mean(f) -> float:
+    f = _f32v(f)
+    return float(_fn("orc_mean")(_ptr(f), f.size))
+
+
+def std_dev(f) -> float:
+    f = _f32v(f)
+    return float(_fn("orc_std_dev")(_ptr(f), f.size))
+
+
+def bias(model, solver) -> float:
+    a, b = _f32v(model), _f32v(solver)
+    return float(_fn("orc_bias")(_ptr(a), _ptr(b), a.size))
+
+
+def radial_profile(f, nx: int, ny: int, cx: float, cy: float) -> np.ndarray:
+    f = _f32v(f)
+    out = np.zeros(nx + ny + 2, dtype=np.float64)
+    fn = _fn("orc_radial_profile", ctypes.c_int)
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_void_p,
+                   ctypes.c_int]
+    k = fn(_ptr(f), nx, ny, cx, cy, _ptr(out), out.size)
+    assert k > 0
+    return out[:k].copy()
+
+
+def xcorr_lag(model, solver, max_lag: int) -> float:
+    a = np.ascontiguousarray(model, dtype=np.float64)
+    b = np.ascontiguousarray(solver, dtype=np.float64)
+    assert a.size == b.size
+    return float(_fn("orc_xcorr_lag")(_ptr(a), _ptr(b), a.size, int(max_lag)))
+
+
+def encode(h, u, v, mean3, std3, depth: float) -> np.ndarray:
+    h, u, v = _f32v(h), _f32v(u), _f32v(v)
+    m, s = _f32v(mean3), _f32v(std3)
+    out = np.empty(3 * h.size, dtype=np.float32)
+    fn = _fn("orc_encode", None)
+    fn.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float,
+                                           ctypes.c_void_p]
+    fn(_ptr(h), _ptr(u), _ptr(v), h.size, _ptr(m), _ptr(s), depth, _ptr(out))
+    return out
+
+
+def decode_h(x, n: int, mean3, std3) -> np.ndarray:
+    x = _f32v(x)
+    m, s = _f32v(mean3), _f32v(std3)
+    out = np.empty(n, dtype=np.float32)
+    fn = _fn("orc_decode_h", None)
+    fn(_ptr(x), n, _ptr(m), _ptr(s), _ptr(out))
+    return out
+
+
+def eval_scores(x, h, u, v, nx, ny, mean3, std3, depth, cx, cy, max_lag=8) -> dict:
+    """eval.rs:158-172 for one trajectory at one horizon."""
+    n = nx * ny
+    x = _f32v(x)
+    target = encode(h, u, v, mean3, std3, depth)
+    ph, th = decode_h(x, n, mean3, std3), decode_h(target, n, mean3, std3)
+    mp, tp = radial_profile(ph, nx, ny, cx, cy), radial_profile(th, nx, ny, cx, cy)
+    return {"corr": correlation(x, target), "corr_h": correlation(x[:n], target[:n]),
+            "amplitude_ratio": float(np.float32(peak(ph)) / np.float32(peak(th))),
+            "lag": xcorr_lag(mp, tp, max_lag), "bias": bias(ph, th)}

@@ -35,3 +35,29 @@ def test_rollout_runs_and_writes_npy(fixture_dir, tmp_path):
     assert len(lines) == 5 and all("corr=" in l for l in lines)
     h = np.load(out)
     assert h.shape == (16, 16) and np.isfinite(h).all()
+
+
+@pytest.mark.gpu
+def test_rollout_prints_the_device_loops_trace(fixture_dir):
+    """The printed correlations are j2k_rollout_truth's for the `drop` case
+    (main.rs:63-78: centre drop, amplitude 0.05, sigma 4)."""
+    import jacobi2000_amd as j
+    from jacobi2000_amd import checkpoint as ck
+    from jacobi2000_amd import solvers as sw
+
+    r = run("--model", str(fixture_dir), "--steps", "4", "--case", "drop")
+    assert r.returncode == 0, r.stderr
+    printed = [float(l.split("corr=")[1]) for l in r.stdout.splitlines() if l.startswith("step")]
+    m = j.Model.load(fixture_dir)
+    c = m.cfg
+    p = sw.Params(nx=c.nx, ny=c.ny, dx=c.dx, dt=c.dt, g=c.g, depth=c.depth, nu=c.nu)
+    st = sw.State.flat(p)
+    st.add_drop(p, c.nx / 2.0, c.ny / 2.0, 0.05, 4.0)
+    n = c.nx * c.ny
+    x0 = np.concatenate([(st.h - np.float32(c.norm_depth) - np.float32(c.norm_mean[0])) / np.float32(c.norm_std[0]),
+                         (st.u - np.float32(c.norm_mean[1])) / np.float32(c.norm_std[1]),
+                         (st.v - np.float32(c.norm_mean[2])) / np.float32(c.norm_std[2])]).astype(np.float32)
+    _, _, _, _, corr = m.rollout_truth(sw.ShallowWater(p), x0, st.h, st.u, st.v,
+                                       ck.meta_of(c.dx, c.dt, c.g, c.depth, c.nu), 4)
+    assert len(printed) == 4 and x0.size == 3 * n
+    assert all(abs(a - float(b)) <= 6e-6 for a, b in zip(printed, corr)), (printed, corr)

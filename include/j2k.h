@@ -76,6 +76,10 @@ typedef struct {
     int max_iter;
     float tol;
     int stall_stop; /* cell_st: retire an iteration stuck at its noise floor (default 1) */
+    /* The excess-energy band training exported (training/jacobi2000_pt/export.py:
+     * 34-35,67-68): upper and lower ratio, NaN when config.json has none (or null).
+     * The reference engine reads neither; see j2k_project_excess_energy. */
+    float excess_r, excess_r_min;
 } j2k_config;
 
 /* Thread-local message of the last failure on this thread ("" if none). */
@@ -276,6 +280,26 @@ int j2k_rollout_truth(void* model, void* solver, float* x_io, const float* meta,
  * (NULL: the model's). */
 int j2k_rollout_truth_dev(void* model, void* solver, float* d_x, const float* d_meta, float* d_h, float* d_u,
                           float* d_v, int steps, float* d_corr, void* stream);
+
+/* ---- excess-energy band (training/jacobi2000_pt/project.py:66-140) -------
+ *
+ * The band projection of training, on the device, for patch2d grids with the
+ * model's normalizer and g: x [batch][3][ny][nx] normalized. Per-cell values
+ * are f32 as the reference's tensor ops compute them, the sums f64 (the
+ * reference sums in f32). dt_dx = dt/dx adds the forward-backward solver's
+ * cross term; 0 gives the plain excess energy. */
+
+/* project.py:98-108 excess_energy of each grid into out [batch] (host). */
+int j2k_excess_energy(void* model, int batch, const float* x, float dt_dx, double* out);
+int j2k_excess_energy_dev(void* model, int batch, const float* d_x, float dt_dx, double* d_out, void* stream);
+/* project.py:111-140 project_excess_energy in place: grids whose excess lies
+ * outside [prev*r_min, prev*r_max] get (eta, u, v) scaled by one s (mass
+ * exact), the others are untouched; r_min = 0 is the upper-only form. prev
+ * [batch] is each grid's excess before the step. */
+int j2k_project_excess_energy(void* model, int batch, float* x, const double* prev, float dt_dx, float r_min,
+                              float r_max);
+int j2k_project_excess_energy_dev(void* model, int batch, float* d_x, const double* d_prev, float dt_dx, float r_min,
+                                  float r_max, void* stream);
 
 /* ---- shallow-water solver (crates/solvers/src/shallow_water.rs) ----------
  *

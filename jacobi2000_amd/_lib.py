@@ -40,7 +40,8 @@ class Config(ctypes.Structure):
                                             "time_radius")]
                 + [("norm_mean", ctypes.c_float * 3), ("norm_std", ctypes.c_float * 3), ("norm_depth", ctypes.c_float)]
                 + [(f, ctypes.c_float) for f in ("dx", "dt", "g", "depth", "nu", "energy_r_max")]
-                + [("max_iter", ctypes.c_int), ("tol", ctypes.c_float), ("stall_stop", ctypes.c_int)])
+                + [("max_iter", ctypes.c_int), ("tol", ctypes.c_float), ("stall_stop", ctypes.c_int)]
+                + [("excess_r", ctypes.c_float), ("excess_r_min", ctypes.c_float)])
 
 
 class Result(ctypes.Structure):
@@ -106,6 +107,10 @@ PROTOS = {
     "j2k_correlation_dev": [P, F, F, P, F, F, I, P, P],
     "j2k_rollout_truth": [P, P, P, P, P, P, P, I, P],
     "j2k_rollout_truth_dev": [P, P, P, P, P, P, P, I, P, P],
+    "j2k_excess_energy": [P, I, P, F, P],
+    "j2k_excess_energy_dev": [P, I, P, F, P, P],
+    "j2k_project_excess_energy": [P, I, P, P, F, F, F],
+    "j2k_project_excess_energy_dev": [P, I, P, P, F, F, F, P],
     "j2k_sw_create": [ctypes.POINTER(SwParams), I, PP],
     "j2k_sw_destroy": [P],
     "j2k_sw_set_kernel": [P, I],

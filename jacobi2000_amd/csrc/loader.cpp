@@ -309,6 +309,13 @@ j2k_config parse_config(const std::string& json_text, const Weights& w, int prec
     c.nu = (float)num(j, "nu");
     const Json* rm = j.find("energy_r_max");
     c.energy_r_max = rm ? (float)rm->num : 1.0f;  // config.rs:33-35 default
+    // export.py:67-68; the reference engine ignores both
+    auto opt = [&](const char* key) {
+        const Json* e = j.find(key);
+        return e && e->kind == Json::Num ? (float)e->num : std::numeric_limits<float>::quiet_NaN();
+    };
+    c.excess_r = opt("excess_r");
+    c.excess_r_min = opt("excess_r_min");
     if (c.channels != 3) fail(J2K_ERR_BAD_CONFIG, "config.json: channels must be 3 (h, u, v)");
     if (c.dim == 0 || c.heads == 0 || c.dim % c.heads != 0)
         fail(J2K_ERR_BAD_CONFIG, "config.json: dim must be a positive multiple of heads");

@@ -126,6 +126,12 @@ float orc_bias(const float* model, const float* solver, int n);
 /* Radial profile into out (at most cap bins); returns the bin count or -1. */
 int orc_radial_profile(const float* f, int nx, int ny, float cx, float cy, double* out, int cap);
 float orc_xcorr_lag(const double* model, const double* solver, int n, int max_lag);
+/* Excess-energy band (training/jacobi2000_pt/project.py:66-140), f64,
+ * excess_oracle.c. x is one normalized grid [3][ny][nx]. */
+void orc_excess_terms(const double* x, int nx, int ny, const orc_phys* p, double dt_dx, double* a, double* q);
+double orc_excess_energy(const double* x, int nx, int ny, const orc_phys* p, double dt_dx);
+double orc_project_excess_energy(double* x, int nx, int ny, const orc_phys* p, double dt_dx, double prev,
+                                 double r_min, double r_max);
 /* norm.rs:27-34 encode and metrics.rs:109-113 decode_h. */
 void orc_encode(const float* h, const float* u, const float* v, int n, const float* mean, const float* stdv,
                 float depth, float* out);

@@ -461,3 +461,36 @@ def eval_scores(x, h, u, v, nx, ny, mean3, std3, depth, cx, cy, max_lag=8) -> di
     return {"corr": correlation(x, target), "corr_h": correlation(x[:n], target[:n]),
             "amplitude_ratio": float(np.float32(peak(ph)) / np.float32(peak(th))),
             "lag": xcorr_lag(mp, tp, max_lag), "bias": bias(ph, th)}
+
+
+# ---- excess-energy band (excess_oracle.c) --------------------------------------
+
+def _f64v(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+def excess_terms(x, nx: int, ny: int, p: Phys, dt_dx: float):
+    """project.py:87-101 for one grid x [3, ny, nx] (f64): (a, q)."""
+    xx = _f64v(x)
+    a, q = ctypes.c_double(), ctypes.c_double()
+    fn = lib().orc_excess_terms
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p,
+                   ctypes.c_void_p]
+    fn(_ptr(xx), nx, ny, ctypes.byref(p), dt_dx, ctypes.byref(a), ctypes.byref(q))
+    return a.value, q.value
+
+
+def excess_energy(x, nx: int, ny: int, p: Phys, dt_dx: float) -> float:
+    a, q = excess_terms(x, nx, ny, p, dt_dx)
+    return a + q
+
+
+def project_excess_energy(x, nx: int, ny: int, p: Phys, dt_dx: float, prev: float, r_min: float, r_max: float):
+    """project.py:104-140 on a copy (f64); returns (x', s)."""
+    xx = _f64v(x).copy()
+    fn = lib().orc_project_excess_energy
+    fn.restype = ctypes.c_double
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                   ctypes.c_double, ctypes.c_double]
+    s = fn(_ptr(xx), nx, ny, ctypes.byref(p), dt_dx, prev, r_min, r_max)
+    return xx, s

@@ -331,6 +331,96 @@ def _run_solver(dataset_states, seconds_cpu, sw, O, dev, stream):
     return out
 
 
+def run_scores(batch, grids, seconds_cpu):
+    """eval-metrics' scores (metrics.rs via eval.rs:158-172) of `batch`
+    64x64 trajectories in one j2k_score_dev call, and the excess-energy band
+    (training project.py:66-140) over `grids` 64x64 grids in one
+    j2k_project_excess_energy_dev call, with HIP events on a stream of their
+    own; the CPU leg times the oracle's restatements on one core."""
+    import ctypes
+
+    import torch
+
+    from jacobi2000_amd import checkpoint as ck
+    from jacobi2000_amd._lib import Score, call
+    import jacobi2000_amd as j
+    from oracle import oracle as O
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    st, cj = ck.random_patch2d(seed=0)
+    cfg = json.loads(cj)
+    cfg["norm"] = {"mean": [0.0008, 0.0004, 0.0005], "std": [0.0063, 0.0138, 0.0135], "depth": 1.0}
+    m = j.Model.from_bytes(st, json.dumps(cfg))
+    n = 64 * 64
+    rng = np.random.default_rng(0)
+    out = {}
+    with torch.cuda.stream(stream):
+        x = torch.from_numpy(rng.standard_normal((batch, 3, n)).astype(np.float32)).to(dev)
+        h = torch.from_numpy((1.0 + 0.01 * rng.standard_normal((batch, n))).astype(np.float32)).to(dev)
+        u = torch.from_numpy((0.01 * rng.standard_normal((batch, n))).astype(np.float32)).to(dev)
+        v = torch.from_numpy((0.01 * rng.standard_normal((batch, n))).astype(np.float32)).to(dev)
+        centers = np.full((batch, 2), 32.0, np.float32)
+        res = (Score * batch)()
+        P = ctypes.c_void_p
+
+        def score():
+            call("j2k_score_dev", m.handle, batch, P(x.data_ptr()), P(h.data_ptr()), P(u.data_ptr()),
+                 P(v.data_ptr()), centers.ctypes.data_as(P), 8, res, P(stream.cuda_stream))
+
+        score()
+        t0 = time.perf_counter()
+        reps = 20
+        for _ in range(reps):
+            score()  # each call synchronises (the lag is finished on the host)
+        t = (time.perf_counter() - t0) / reps
+        out["scores"] = {"workload": f"{batch} trajectories of 64x64, one call (f64 sums on the device, lag on the "
+                                     "host)", "ms_per_call": t * 1e3, "trajectories_per_s": batch / t}
+        xg = torch.from_numpy(rng.standard_normal((grids, 3, n)).astype(np.float32)).to(dev)
+        e = torch.zeros(grids, dtype=torch.float64, device=dev)
+        call("j2k_excess_energy_dev", m.handle, grids, P(xg.data_ptr()), 0.0903, P(e.data_ptr()), P(stream.cuda_stream))
+        prev = e * 0.5
+        work = xg.clone()
+
+        def project():
+            work.copy_(xg)
+            call("j2k_project_excess_energy_dev", m.handle, grids, P(work.data_ptr()), P(prev.data_ptr()), 0.0903,
+                 0.9999, 1.0004, P(stream.cuda_stream))
+
+        project()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(10):
+            work.copy_(xg)
+        b.record(stream)
+        torch.cuda.synchronize()
+        tcopy = a.elapsed_time(b) / 10 / 1e3
+        a.record(stream)
+        for _ in range(10):
+            project()
+        b.record(stream)
+        torch.cuda.synchronize()
+        tp = a.elapsed_time(b) / 10 / 1e3 - tcopy
+        byt = grids * n * 12 * 2  # every grid is outside its band: read twice-ish, written once
+        out["excess_band"] = {"workload": f"{grids} grids of 64x64 projected (all outside the band), one call",
+                              "kernel": "k_excess<true>", "us_per_call": tp * 1e6, "grids_per_s": grids / tp,
+                              "bytes_per_call": byt, "achieved_GBps": byt / tp / 1e9}
+    if seconds_cpu > 0:
+        mean, std, depth = cfg["norm"]["mean"], cfg["norm"]["std"], cfg["norm"]["depth"]
+        xc, hc, uc, vc = (t_.cpu().numpy() for t_ in (x, h, u, v))
+        done, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < seconds_cpu:
+            b_ = done % batch
+            O.eval_scores(xc[b_], hc[b_], uc[b_], vc[b_], 64, 64, mean, std, depth, 32.0, 32.0, 8)
+            done += 1
+        el = time.perf_counter() - t0
+        out["scores"]["cpu_baseline"] = {"value": done / el, "unit": "trajectories/s", "cores": 1, "kind": "port",
+                                         "sample": f"{done} trajectories, oracle restatement of metrics.rs, "
+                                                   f"{el:.1f} s"}
+    return out
+
+
 def stream_requests(n, seed=0):
     """Mixed request stream: sizes uniform over {32,48,64,96}^2 x {8,16}
     slices, mask kinds cycling forward / rewind / reconstruction (2% of cells),
@@ -564,6 +654,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the f32 plain form (profiling runs)")
     ap.add_argument("--solver", type=int, default=2048, metavar="STATES",
                     help="also time the device shallow-water solver with this many 64x64 dataset states (0: skip)")
+    ap.add_argument("--scores", type=int, default=64, metavar="TRAJ",
+                    help="also time the device scores on this many 64x64 trajectories and the excess band on 16x "
+                         "as many grids (0: skip)")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure the dominant kernel's HBM bytes")
     args = ap.parse_args()
@@ -668,6 +761,9 @@ def main():
     if args.solver > 0:
         note(f"solver: {args.solver} dataset states, truth partner, 4096x4096 grid")
         line["solver"] = run_solver(args.solver, 0.0 if args.no_cpu_baseline else 3.0)
+    if args.scores > 0:
+        note(f"scores: {args.scores} trajectories, excess band on {16 * args.scores} grids")
+        line.update(run_scores(args.scores, 16 * args.scores, 0.0 if args.no_cpu_baseline else 2.0))
     if not args.no_cpu_baseline:
         note(f"cpu baseline: about {args.cpu_seconds:g} s of the CPU oracle")
         line["cpu_baseline"] = cpu_baseline(st, cj, x0, known, meta, max(plain["iterations"][0], 1), args.cpu_seconds)

@@ -194,18 +194,22 @@ __device__ __forceinline__ void sw_mrow(const SwConst& k, const SwLane& L, bool 
 template <int D, class Fetch, class Emit>
 __device__ __forceinline__ void sw_march(const SwConst& k, const SwLane& L, int y0, int y1, Fetch fetch, Emit emit) {
     const int last = k.ny - 1;
+    // the last row this chunk reads: rows past it only feed rows past y1,
+    // which are dropped, so their fetches repeat it (a cache hit, no new
+    // HBM line)
+    const int lim = min(y1 + 1, last);
     constexpr int NB = D + 3;
     SwRow ring[D > 0 ? NB : 1];
     if constexpr (D > 0) {
         // the rows ahead first, in row order, so they are the oldest loads
 #pragma unroll
         for (int j = 3; j < NB; j++) {
-            ring[j] = fetch(min(y0 + j - 1, last));
+            ring[j] = fetch(min(y0 + j - 1, lim));
             __builtin_amdgcn_sched_barrier(0);
         }
     }
     // h_new of row y0-1 (needs rows y0-2 .. y0) and the flux south of y0
-    const SwRow r0 = fetch(y0), r1 = fetch(min(y0 + 1, last));
+    const SwRow r0 = fetch(y0), r1 = fetch(min(y0 + 1, lim));
     SwRow rsouth{0.0f, 0.0f, 0.0f};
     float hm = 0.0f, fys = 0.0f;
     if (y0 > 0) {
@@ -235,7 +239,7 @@ __device__ __forceinline__ void sw_march(const SwConst& k, const SwLane& L, int 
     if constexpr (D == 0) {
         SwRow rm = rsouth, rc = r0, rp = r1;
         for (int y = y0; y < y1; y++) {
-            const SwRow rq = fetch(min(y + 2, last));
+            const SwRow rq = fetch(min(y + 2, lim));
             body(y, rm, rc, rp, rq);
             rm = rc;
             rc = rp;
@@ -250,7 +254,7 @@ __device__ __forceinline__ void sw_march(const SwConst& k, const SwLane& L, int 
             for (int j = 0; j < NB; j++) {
                 const int y = yg + j;
                 body(y, ring[j], ring[(j + 1) % NB], ring[(j + 2) % NB], ring[(j + 3) % NB]);
-                ring[j] = fetch(min(y + NB - 1, last));
+                ring[j] = fetch(min(y + NB - 1, lim));
             }
         }
     }

@@ -222,11 +222,101 @@ __global__ __launch_bounds__(256) void k_cell_embed_ln(int M, int dim, const flo
     }
 }
 
+// Sum over the L lanes of one token's group (L a power of two up to 32).
+template <int L>
+__device__ __forceinline__ float group_sum(float x) {
+#pragma unroll
+    for (int o = 1; o < L; o <<= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// k_cell_embed_ln for dim = D (128 or 256): L = D/8 lanes per token, each
+// with 8 contiguous columns (vector loads and stores), 64/L tokens per wave.
+// Same per-element arithmetic as the generic kernel; the row sums group the
+// lanes' partial sums.
+template <int D>
+__global__ __launch_bounds__(256) void k_cell_embed_ln_v(int M, const float4* __restrict__ x4,
+                                                         const float* __restrict__ we, const float* __restrict__ be,
+                                                         const float* __restrict__ mvec, const int* __restrict__ tok_req,
+                                                         const float* __restrict__ g, const float* __restrict__ b,
+                                                         float* __restrict__ X, float* __restrict__ H,
+                                                         uint16_t* __restrict__ Hb) {
+    constexpr int L = D / 8, TPW = 64 / L;
+    const int lane = threadIdx.x & 63, j = lane % L;
+    const int m = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW + lane / L;
+    if (m >= M) return;  // a whole group leaves together
+    const float4 f = x4[m];
+    const float* mv = mvec + (tok_req ? (long)tok_req[m] * D : 0);
+    const int d0 = j * 8;
+    float a[8];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const f4v bv = *reinterpret_cast<const f4v*>(be + d0 + 4 * h);
+        const f4v w0 = *reinterpret_cast<const f4v*>(we + d0 + 4 * h);
+        const f4v w1 = *reinterpret_cast<const f4v*>(we + D + d0 + 4 * h);
+        const f4v w2 = *reinterpret_cast<const f4v*>(we + 2 * D + d0 + 4 * h);
+        const f4v w3 = *reinterpret_cast<const f4v*>(we + 3 * D + d0 + 4 * h);
+        const f4v mw = *reinterpret_cast<const f4v*>(mv + d0 + 4 * h);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            float v = bv[k];
+            if (f.x != 0.f) v += f.x * w0[k];
+            if (f.y != 0.f) v += f.y * w1[k];
+            if (f.z != 0.f) v += f.z * w2[k];
+            if (f.w != 0.f) v += f.w * w3[k];
+            v += mw[k];
+            a[4 * h + k] = v;
+        }
+    }
+    float* xr = X + (long)m * D + d0;
+    *reinterpret_cast<f4v*>(xr) = f4v{a[0], a[1], a[2], a[3]};
+    *reinterpret_cast<f4v*>(xr + 4) = f4v{a[4], a[5], a[6], a[7]};
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += a[k];
+    const float mean = group_sum<L>(s) / (float)D;
+    float q = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float c = a[k] - mean;
+        q += c * c;
+    }
+    const float inv = 1.0f / sqrtf(group_sum<L>(q) / (float)D + 1e-5f);
+    float o[8];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const f4v gv = *reinterpret_cast<const f4v*>(g + d0 + 4 * h);
+        const f4v bb = *reinterpret_cast<const f4v*>(b + d0 + 4 * h);
+#pragma unroll
+        for (int k = 0; k < 4; k++) o[4 * h + k] = (a[4 * h + k] - mean) * inv * gv[k] + bb[k];
+    }
+    if (H) {
+        float* hr = H + (long)m * D + d0;
+        *reinterpret_cast<f4v*>(hr) = f4v{o[0], o[1], o[2], o[3]};
+        *reinterpret_cast<f4v*>(hr + 4) = f4v{o[4], o[5], o[6], o[7]};
+    }
+    if (Hb) {
+        typedef uint32_t u4v __attribute__((ext_vector_type(4)));
+        *reinterpret_cast<u4v*>(Hb + (long)m * D + d0) =
+            u4v{f2bf2(o[0], o[1]), f2bf2(o[2], o[3]), f2bf2(o[4], o[5]), f2bf2(o[6], o[7])};
+    }
+}
+
 void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* we, const float* be, const float* mvec,
                    const int* tok_req, const float* g, const float* b, float* X, float* H, uint16_t* Hb) {
     if (dim > 1024) fail(J2K_ERR_LIMIT, "cell_embed_ln: dim > 1024 is not supported");
-    hipLaunchKernelGGL(k_cell_embed_ln, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, (const float4*)x4, we, be, mvec,
-                       tok_req, g, b, X, H, Hb);
+    if (dim == 128) {
+        hipLaunchKernelGGL(k_cell_embed_ln_v<128>, dim3(cdiv(M, 16)), dim3(256), 0, s, M, (const float4*)x4, we, be,
+                           mvec, tok_req, g, b, X, H, Hb);
+    } else if (dim == 256) {
+        hipLaunchKernelGGL(k_cell_embed_ln_v<256>, dim3(cdiv(M, 8)), dim3(256), 0, s, M, (const float4*)x4, we, be,
+                           mvec, tok_req, g, b, X, H, Hb);
+    } else {
+        hipLaunchKernelGGL(k_cell_embed_ln, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, (const float4*)x4, we, be, mvec,
+                           tok_req, g, b, X, H, Hb);
+    }
     LAUNCHCHK("cell_embed_ln");
 }
 
@@ -285,11 +375,71 @@ __global__ __launch_bounds__(256) void k_cell_readout(int M, int dim, const floa
     }
 }
 
+// k_cell_readout for dim = D (128 or 256), L = D/8 lanes per token as in
+// k_cell_embed_ln_v.
+template <int D>
+__global__ __launch_bounds__(256) void k_cell_readout_v(int M, const float* __restrict__ X,
+                                                        const float* __restrict__ g, const float* __restrict__ b,
+                                                        const float* __restrict__ wr, const float* __restrict__ br,
+                                                        const float4* __restrict__ x0, float4* __restrict__ xn,
+                                                        float* __restrict__ y) {
+    constexpr int L = D / 8, TPW = 64 / L;
+    const int lane = threadIdx.x & 63, j = lane % L;
+    const int m = (blockIdx.x * 4 + (threadIdx.x >> 6)) * TPW + lane / L;
+    if (m >= M) return;
+    const int d0 = j * 8;
+    const float* xr = X + (long)m * D + d0;
+    const f4v lo = *reinterpret_cast<const f4v*>(xr), hi = *reinterpret_cast<const f4v*>(xr + 4);
+    const float v[8] = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += v[k];
+    const float mean = group_sum<L>(s) / (float)D;
+    float q = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const float c = v[k] - mean;
+        q += c * c;
+    }
+    const float inv = 1.0f / sqrtf(group_sum<L>(q) / (float)D + 1e-5f);
+    float y0 = 0.f, y1 = 0.f, y2 = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int d = d0 + k;
+        const float o = (v[k] - mean) * inv * g[d] + b[d];
+        y0 += o * wr[d * 3 + 0];
+        y1 += o * wr[d * 3 + 1];
+        y2 += o * wr[d * 3 + 2];
+    }
+    y0 = group_sum<L>(y0) + br[0];
+    y1 = group_sum<L>(y1) + br[1];
+    y2 = group_sum<L>(y2) + br[2];
+    if (j == 0) {
+        if (y) {
+            y[(long)m * 3] = y0;
+            y[(long)m * 3 + 1] = y1;
+            y[(long)m * 3 + 2] = y2;
+        }
+        if (xn) {
+            const float4 a = x0[m];
+            xn[m] = a.w != 0.f ? a : make_float4(a.x + y0, a.y + y1, a.z + y2, 0.f);
+        }
+    }
+}
+
 void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g, const float* b, const float* wr,
                   const float* br, const float* x0, float* xn, float* y) {
     if (dim > 1024) fail(J2K_ERR_LIMIT, "cell_readout: dim > 1024 is not supported");
-    hipLaunchKernelGGL(k_cell_readout, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, X, g, b, wr, br, (const float4*)x0,
-                       (float4*)xn, y);
+    if (dim == 128) {
+        hipLaunchKernelGGL(k_cell_readout_v<128>, dim3(cdiv(M, 16)), dim3(256), 0, s, M, X, g, b, wr, br,
+                           (const float4*)x0, (float4*)xn, y);
+    } else if (dim == 256) {
+        hipLaunchKernelGGL(k_cell_readout_v<256>, dim3(cdiv(M, 8)), dim3(256), 0, s, M, X, g, b, wr, br,
+                           (const float4*)x0, (float4*)xn, y);
+    } else {
+        hipLaunchKernelGGL(k_cell_readout, dim3(cdiv(M, 4)), dim3(256), 0, s, M, dim, X, g, b, wr, br,
+                           (const float4*)x0, (float4*)xn, y);
+    }
     LAUNCHCHK("cell_readout");
 }
 

@@ -12,10 +12,13 @@
 namespace j2k {
 
 constexpr int PT = 256;
+// Cell kernels work one block per slice (a few per request): a whole CU's
+// worth of threads each keeps their dependent passes short.
+constexpr int CPT = 1024;
 
 // Block-wide f64 sum of up to 5 values; every thread gets the totals.
-template <int NV>
-__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[PT / 64]) {
+template <int NV, int T = PT>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[T / 64]) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int j = 0; j < NV; j++) {
@@ -27,7 +30,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*sh)[PT / 64]
     for (int j = 0; j < NV; j++) {
         double t = 0;
 #pragma unroll
-        for (int w = 0; w < PT / 64; w++) t += sh[j][w];
+        for (int w = 0; w < T / 64; w++) t += sh[j][w];
         v[j] = t;
     }
     __syncthreads();
@@ -145,19 +148,19 @@ __device__ __forceinline__ void phys_huv(const float4& c, const Phys& ph, double
     v = (double)c.z * ph.stdv[2] + ph.mean[2];
 }
 
-__global__ __launch_bounds__(PT) void k_cell_targets(const Slice* __restrict__ sl, const float4* __restrict__ x0,
+__global__ __launch_bounds__(CPT) void k_cell_targets(const Slice* __restrict__ sl, const float4* __restrict__ x0,
                                                      double* __restrict__ mass_t, double* __restrict__ eref_t, Phys ph) {
-    __shared__ double sh[2][PT / 64];
+    __shared__ double sh[2][CPT / 64];
     const Slice S = sl[blockIdx.x];
     double v[2] = {0, 0};
-    for (int i = threadIdx.x; i < S.n; i += PT) {
+    for (int i = threadIdx.x; i < S.n; i += CPT) {
         const float4 c = x0[S.start + i];
         double h, u, w;
         phys_huv(c, ph, h, u, w);
         v[0] += (double)c.x;
         v[1] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
     }
-    block_sum<2>(v, sh);
+    block_sum<2, CPT>(v, sh);
     if (threadIdx.x == 0) {
         mass_t[blockIdx.x] = v[0] / (double)S.n;
         eref_t[blockIdx.x] = v[1];
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(PT) void k_cell_targets(const Slice* __restrict__ s
 void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
                   float g, const float* mean, const float* stdv, float depth) {
     if (n_slices <= 0) return;
-    hipLaunchKernelGGL(k_cell_targets, dim3(n_slices), dim3(PT), 0, s, sl, (const float4*)x0, mass_t, eref_t,
+    hipLaunchKernelGGL(k_cell_targets, dim3(n_slices), dim3(CPT), 0, s, sl, (const float4*)x0, mass_t, eref_t,
                        phys_of(g, mean, stdv, depth, 1.0f));
     LAUNCHCHK("cell_targets");
 }
@@ -176,32 +179,32 @@ void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0,
 // down to r_max*Eref*(1-1e-7); if potential alone is over it, zero blank
 // (u, v) and shrink blank h anomalies about their mean (mass unchanged).
 // Then the slice's max |xn - xc| over blank cells.
-__global__ __launch_bounds__(PT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
+__global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
                                                      const float4* __restrict__ xc, const double* __restrict__ mass_t,
                                                      const double* __restrict__ eref_t, unsigned* __restrict__ resid,
                                                      Phys ph) {
-    __shared__ double sh[5][PT / 64];
-    __shared__ float shm[PT / 64];
+    __shared__ double sh[5][CPT / 64];
+    __shared__ float shm[CPT / 64];
     const Slice S = sl[blockIdx.x];
     float4* x = xn + S.start;
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
     const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
     {
         double v[2] = {0, 0};
-        for (int i = threadIdx.x; i < S.n; i += PT) {
+        for (int i = threadIdx.x; i < S.n; i += CPT) {
             const float4 c = x[i];
             v[0] += (double)c.x;
             v[1] += c.w != 0.f ? 0.0 : 1.0;
         }
-        block_sum<2>(v, sh);
+        block_sum<2, CPT>(v, sh);
         const double nb = v[1];
         if (nb > 0) {
             const double delta = (mass_t[blockIdx.x] * (double)S.n - v[0]) / nb;
-            for (int i = threadIdx.x; i < S.n; i += PT)
+            for (int i = threadIdx.x; i < S.n; i += CPT)
                 if (x[i].w == 0.f) x[i].x = (float)((double)x[i].x + delta);
             __syncthreads();
             double e[5] = {0, 0, 0, 0, 0};  // pk, kk, pb, kb, sum of blank h
-            for (int i = threadIdx.x; i < S.n; i += PT) {
+            for (int i = threadIdx.x; i < S.n; i += CPT) {
                 const float4 c = x[i];
                 double h, u, w;
                 phys_huv(c, ph, h, u, w);
@@ -215,14 +218,14 @@ __global__ __launch_bounds__(PT) void k_cell_project(const Slice* __restrict__ s
                     e[4] += h;
                 }
             }
-            block_sum<5>(e, sh);
+            block_sum<5, CPT>(e, sh);
             const double bound = (double)ph.r_max * eref_t[blockIdx.x];
             if (e[0] + e[1] + e[2] + e[3] > bound) {
                 const double target = bound * (1.0 - 1e-7);
                 const double budget = target - e[0] - e[1] - e[2];
                 double su = 0.0;
                 if (budget >= 0) su = fmin(sqrt(budget / fmax(e[3], 1e-12)), 1.0);
-                for (int i = threadIdx.x; i < S.n; i += PT) {
+                for (int i = threadIdx.x; i < S.n; i += CPT) {
                     float4 c = x[i];
                     if (c.w != 0.f) continue;
                     const double u = (double)c.y * s1 + m1, w = (double)c.z * s2 + m2;
@@ -233,17 +236,17 @@ __global__ __launch_bounds__(PT) void k_cell_project(const Slice* __restrict__ s
                 if (budget < 0) {
                     const double hbar = e[4] / nb;
                     double a[1] = {0};
-                    for (int i = threadIdx.x; i < S.n; i += PT) {
+                    for (int i = threadIdx.x; i < S.n; i += CPT) {
                         const float4 c = x[i];
                         if (c.w != 0.f) continue;
                         const double d = (double)c.x * s0 + m0 - hbar;
                         a[0] += d * d;
                     }
-                    block_sum<1>(a, sh);
+                    block_sum<1, CPT>(a, sh);
                     const double avail = target - e[0] - e[1] - 0.5 * g * nb * hbar * hbar;
                     double lam = 0.0;
                     if (avail > 0 && a[0] > 0) lam = fmin(sqrt(avail / (0.5 * g * a[0])), 1.0);
-                    for (int i = threadIdx.x; i < S.n; i += PT) {
+                    for (int i = threadIdx.x; i < S.n; i += CPT) {
                         float4 c = x[i];
                         if (c.w != 0.f) continue;
                         const double h = (double)c.x * s0 + m0;
@@ -258,7 +261,7 @@ __global__ __launch_bounds__(PT) void k_cell_project(const Slice* __restrict__ s
     // Residual over blank cells (all three channels), max is order-free.
     float r = 0.f;
     const float4* p = xc + S.start;
-    for (int i = threadIdx.x; i < S.n; i += PT) {
+    for (int i = threadIdx.x; i < S.n; i += CPT) {
         const float4 a = x[i], b = p[i];
         if (a.w != 0.f) continue;
         r = fmaxf(r, fmaxf(fabsf(a.x - b.x), fmaxf(fabsf(a.y - b.y), fabsf(a.z - b.z))));
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(PT) void k_cell_project(const Slice* __restrict__ s
     __syncthreads();
     if (threadIdx.x == 0) {
         float t = 0.f;
-        for (int w = 0; w < PT / 64; w++) t = fmaxf(t, shm[w]);
+        for (int w = 0; w < CPT / 64; w++) t = fmaxf(t, shm[w]);
         atomicMax(resid + S.req, __float_as_uint(t));
     }
 }
@@ -277,23 +280,23 @@ void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
                   float depth, float r_max) {
     if (n_slices <= 0) return;
-    hipLaunchKernelGGL(k_cell_project, dim3(n_slices), dim3(PT), 0, s, sl, (float4*)xn, (const float4*)xc, mass_t,
+    hipLaunchKernelGGL(k_cell_project, dim3(n_slices), dim3(CPT), 0, s, sl, (float4*)xn, (const float4*)xc, mass_t,
                        eref_t, resid_bits, phys_of(g, mean, stdv, depth, r_max));
     LAUNCHCHK("cell_project");
 }
 
-__global__ __launch_bounds__(PT) void k_cell_violation(const Slice* __restrict__ sl, const float4* __restrict__ x4,
+__global__ __launch_bounds__(CPT) void k_cell_violation(const Slice* __restrict__ sl, const float4* __restrict__ x4,
                                                        const double* __restrict__ eref_t,
                                                        unsigned long long* __restrict__ viol, Phys ph) {
-    __shared__ double sh[1][PT / 64];
+    __shared__ double sh[1][CPT / 64];
     const Slice S = sl[blockIdx.x];
     double v[1] = {0};
-    for (int i = threadIdx.x; i < S.n; i += PT) {
+    for (int i = threadIdx.x; i < S.n; i += CPT) {
         double h, u, w;
         phys_huv(x4[S.start + i], ph, h, u, w);
         v[0] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
     }
-    block_sum<1>(v, sh);
+    block_sum<1, CPT>(v, sh);
     if (threadIdx.x == 0) {
         const double e = eref_t[blockIdx.x];
         const double r = fmax((v[0] - (double)ph.r_max * e) / e, 0.0);
@@ -305,7 +308,7 @@ void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x
                     unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,
                     float r_max) {
     if (n_slices <= 0) return;
-    hipLaunchKernelGGL(k_cell_violation, dim3(n_slices), dim3(PT), 0, s, sl, (const float4*)x4, eref_t, viol_bits,
+    hipLaunchKernelGGL(k_cell_violation, dim3(n_slices), dim3(CPT), 0, s, sl, (const float4*)x4, eref_t, viol_bits,
                        phys_of(g, mean, stdv, depth, r_max));
     LAUNCHCHK("cell_violation");
 }

@@ -172,8 +172,49 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     int it = 0;
     float r = 0.f;
     StallWatch watch;
+    // Replay the iteration from a graph once this grid and these buffers
+    // have run eagerly (not while per-kernel timing records events).
+    auto& cg = m.cell_graph;
+    const float* xa0 = xa;
+    if (cg.T != T || cg.ny != ny || cg.nx != nx || cg.x0 != x0 || cg.xa != xa || cg.xb != xb ||
+        cg.epoch != m.graph_epoch) {
+        m.drop_cell_graphs();
+        cg.T = T, cg.ny = ny, cg.nx = nx, cg.x0 = x0, cg.xa = xa, cg.xb = xb, cg.epoch = m.graph_epoch;
+    }
+    if (cg.calls < 2) cg.calls++;
+    const bool replay = !m.timer.on && !m.graphs_off && cg.calls >= 2;
+    auto iteration = [&](float* xc, float* xn) {
+        const int parity = xc == xa0 ? 0 : 1;
+        if (replay && !cg.g[parity]) {
+            hipGraph_t g = nullptr;
+            const unsigned before = m.graph_epoch;
+            bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
+            if (ok) {
+                m.timer.capturing = true;
+                try {
+                    cell_iteration(m, s, p, x0, xc, xn);
+                } catch (const Error&) {
+                    ok = false;
+                }
+                m.timer.capturing = false;
+                const hipError_t e = hipStreamEndCapture(s, &g);
+                ok = ok && e == hipSuccess && g &&
+                     hipGraphInstantiate(&cg.g[parity], g, nullptr, nullptr, 0) == hipSuccess;
+                if (g) (void)hipGraphDestroy(g);
+            }
+            if (!ok || m.graph_epoch != before) {  // a capture must not reallocate
+                m.drop_cell_graphs();
+                m.graphs_off = !ok;
+                (void)hipGetLastError();
+                cell_iteration(m, s, p, x0, xc, xn);
+                return;
+            }
+        }
+        if (replay && cg.g[parity]) HIPCHK(hipGraphLaunch(cg.g[parity], s));
+        else cell_iteration(m, s, p, x0, xc, xn);
+    };
     while (it < max_iter) {
-        cell_iteration(m, s, p, x0, xa, xb);
+        iteration(xa, xb);
         unsigned bits = 0;
         d2h(&bits, p.resid, sizeof(unsigned), s);
         std::memcpy(&r, &bits, 4);
@@ -181,6 +222,12 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
         it++;
         if (r < tol) break;
         if (watch.update(r) && m.cfg.stall_stop) break;
+    }
+    // growth during an eager call is adopted before any capture; after
+    // one, it retires the graphs (they point into the old slots)
+    if (cg.epoch != m.graph_epoch) {
+        if (cg.g[0] || cg.g[1]) m.drop_cell_graphs();
+        else cg.epoch = m.graph_epoch;
     }
     unsigned long long* vb = reinterpret_cast<unsigned long long*>(p.resid + 2);
     fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);

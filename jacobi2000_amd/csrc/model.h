@@ -117,6 +117,21 @@ struct Model {
     } cell_tables;
     int p_tiles = 0;
     Tuning tune;
+    // Replayed single-request cell_st iterations (j2k_iterate*): one graph
+    // per ping-pong parity, for the grid and buffers it was captured with.
+    // graph_epoch moves whenever a scratch slot is reallocated or the tuning
+    // changes, which retires them; the first call with a key runs eagerly so
+    // that every slot has its size before a capture.
+    struct {
+        int T = -1, ny = 0, nx = 0;
+        const void *x0 = nullptr, *xa = nullptr, *xb = nullptr;
+        unsigned epoch = 0;
+        int calls = 0;
+        hipGraphExec_t g[2] = {nullptr, nullptr};
+    } cell_graph;
+    unsigned graph_epoch = 0;
+    bool graphs_off = false;  // J2K_GRAPHS=0 at creation, or a capture failed: run eagerly
+    void drop_cell_graphs();
 
     ~Model();
     void* scratch(int s, size_t bytes);

@@ -17,11 +17,22 @@ DeviceGuard::~DeviceGuard() {
     if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
 }
 
+void Model::drop_cell_graphs() {
+    for (hipGraphExec_t& g : cell_graph.g)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+    cell_graph.T = -1;
+    cell_graph.calls = 0;
+}
+
 Model::~Model() {
     int prev = -1;
     (void)hipGetDevice(&prev);
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_cell_graphs();
     for (void* p : owned) (void)hipFree(p);
     for (int i = 0; i < S_COUNT; i++)
         if (slot[i]) (void)hipFree(slot[i]);
@@ -32,6 +43,7 @@ Model::~Model() {
 void* Model::scratch(int s, size_t bytes) {
     if (bytes == 0) bytes = 16;
     if (cap[s] >= bytes) return slot[s];
+    graph_epoch++;  // captured graphs may point into the old slot
     if (slot[s]) {
         HIPCHK(hipStreamSynchronize(stream));
         HIPCHK(hipFree(slot[s]));
@@ -233,6 +245,7 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
     m->cfg = cfg;
     if (const char* e = std::getenv("J2K_AUTOTUNE")) m->tune.enabled = std::strcmp(e, "0") != 0;
     if (const char* e = std::getenv("J2K_TUNE_FORCE")) force_tuning(*m, e);
+    if (const char* e = std::getenv("J2K_GRAPHS")) m->graphs_off = std::strcmp(e, "0") == 0;
     m->device = device;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
     if (cfg.form == J2K_FORM_CELL_ST) attn_cell_prepare();

@@ -137,7 +137,8 @@ struct Server {
     unsigned* h_resid = nullptr;  // pinned
     size_t h_resid_cap = 0;
 
-    Server(Model& model, size_t b, int ma) : m(model), budget(b), max_active(ma) {}
+    Server(Model& model, size_t b, int ma)
+        : m(model), budget(b), max_active(ma), graphs_enabled(!model.graphs_off) {}
 
     ~Server() {
         {

@@ -217,6 +217,7 @@ void force_tuning(Model& m, const char* spec) {
     }
     m.tune.tuned_to = B_MAX;
     m.tune.enabled = false;
+    m.graph_epoch++;
 }
 
 void autotune(Model& m, long max_tokens) {
@@ -238,6 +239,7 @@ void autotune(Model& m, long max_tokens) {
     HIPCHK(hipStreamSynchronize(m.stream));
     m.timer.on = timing;
     m.tune.tuned_to = top;
+    m.graph_epoch++;  // captured iterations used the old choices
 }
 
 std::string tuning_json(const Model& m) {

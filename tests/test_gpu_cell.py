@@ -247,6 +247,53 @@ def test_autotune_table_and_results(model20_cell):
     assert np.array_equal(xa, xb)
 
 
+@pytest.mark.parametrize("precision", ["f32", "bf16x2"])
+def test_iteration_graph_replays_match_eager_launches(model20_cell, precision):
+    """From the second call with a grid shape on, j2k_iterate* replays each
+    iteration from a HIP graph. Against a model that never captures
+    (J2K_GRAPHS=0), every call must agree bit for bit (grid, iterations,
+    residual, violation) across: repeats of one shape (capture, then
+    replays), a larger shape (scratch grows, graphs retire), the first shape
+    again, eager forwards on the same scratch between replays, a call with
+    per-kernel timing on, and device calls on a stream of their own."""
+    st, cj = model20_cell
+    eager = _model_env(st, cj, precision, J2K_GRAPHS="0")
+    graphs = _model_env(st, cj, precision)
+    dev = torch.device("cuda:0")
+    side = torch.cuda.Stream()
+    reqs = []
+    for i, (T, ny, nx, kind) in enumerate([(4, 16, 16, "forward"), (4, 16, 16, "rewind"), (4, 16, 16, "forward"),
+                                          (8, 32, 32, "forward"), (8, 32, 32, "reconstruction"),
+                                          (8, 32, 32, "forward"), (4, 16, 16, "rewind"), (4, 16, 16, "forward"),
+                                          (4, 16, 16, "reconstruction")]):
+        g = ck.drops_grid(T, ny, nx, 2, 40 + i)
+        k = ck.mask_of(kind, T, ny, nx, seed=i)
+        reqs.append((O.fill_blank(g, k), k))
+    for i, (x0, k) in enumerate(reqs):
+        _, T, ny, nx = x0.shape
+        want = eager.iterate(x0, k, META, max_iter=6, tol=0.0)
+        if i == 5:
+            _lib.call("j2k_model_timing", graphs.handle, 1, None)
+        if i in (2, 7):  # device form on a stream of its own
+            g = torch.from_numpy(np.ascontiguousarray(x0)).to(dev)
+            kd = torch.from_numpy(np.ascontiguousarray(k, dtype=np.uint8)).to(dev)
+            md = torch.from_numpy(META).to(dev)
+            torch.cuda.synchronize()
+            it, r, v = graphs.iterate_dev(g.data_ptr(), kd.data_ptr(), md.data_ptr(), T, ny, nx, 6, 0.0,
+                                          stream=side.cuda_stream)
+            side.synchronize()
+            got = (g.cpu().numpy(), it, r, v)
+        else:
+            got = graphs.iterate(x0, k, META, max_iter=6, tol=0.0)
+        if i == 5:
+            _lib.call("j2k_model_timing", graphs.handle, 0, None)
+        assert got[1:] == want[1:], i
+        assert np.array_equal(got[0].view(np.uint32), want[0].view(np.uint32)), i
+        if i in (1, 4, 6):  # eager work on the same scratch between replays
+            y = gpu_forward(graphs, x0, k)
+            assert np.array_equal(y.view(np.uint32), gpu_forward(eager, x0, k).view(np.uint32)), i
+
+
 @pytest.mark.parametrize("shape", ["model20", "wide"])
 def test_fp8_form_error_vs_f64_oracle(model20_cell, shape):
     """The FP8 form (E4M3 block products, per-tensor scales, f32 accumulate)

@@ -8,19 +8,27 @@
 //   LNOUT  Hout = LN1'(X)                            (when the qkv GEMM runs apart)
 //
 // With PROJ and QKV a layer is two launches: this kernel and the attention.
-// The same arithmetic, in the same order, as the tile GEMM epilogues and the
-// MLP alone, so every combination gives bit-identical results and the
-// autotuner may pick any of them.
+// The same arithmetic, in the same order, as the tile GEMM epilogues, so
+// every combination gives bit-identical results and the autotuner may pick
+// any of them.
 //
 // A block owns 64 tokens; its four waves split every 128-column product (32
 // columns each, 2 x 4 tiles of v_mfma_f32_16x16x32_bf16, f32 accumulate).
-// A operands (O, H, the GELU intermediate) sit in LDS as bf16 (or, SPLIT, as
-// a bf16 hi + lo pair of an f32 value, two MFMAs per tile). Weights
-// (K-contiguous bf16 [N, K]) stream from L2 as 16-byte fragments, loaded a
-// whole 128 x 128 chunk ahead of their use. The ff-wide intermediate is made
-// 128 columns at a time and consumed at once as the next K chunk of ff2.
-// Row statistics of the LayerNorms in the order every served kernel uses
-// (lnstats.h).
+// The products run transposed: the MFMA's first operand is a 16 x 32
+// fragment of W (K-contiguous bf16 [N, K], so output columns are its rows),
+// the second a 32 x 16 fragment of the panel (tokens) from LDS, as bf16 or,
+// SPLIT, as a bf16 hi + lo pair of an f32 value (two MFMAs per tile). Each
+// output element is the same dot product over K as in the untransposed
+// tile GEMM. A lane's accumulator then holds four consecutive columns of
+// one token: residual, biases and results move as 16-byte vectors (raw
+// buffer operations bounded by the panel's rows, so a ragged last panel
+// needs no branches), and the next product's operand is written as 8-byte
+// LDS stores. Weights stream from L2 as 16-byte fragments, loaded a whole
+// 128 x 128 chunk ahead of their use. The ff-wide intermediate is made 128
+// columns at a time and consumed at once as the next K chunk of ff2. The
+// residual after the projection goes to X and is read back after the MLP,
+// so it holds no registers while the MLP runs. Row statistics of the
+// LayerNorms in the order every served kernel uses (lnstats.h).
 #include "common.h"
 #include "kernels.h"
 #include "lnstats.h"
@@ -31,10 +39,29 @@ namespace {
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+typedef unsigned u2v __attribute__((ext_vector_type(2)));
 
 constexpr int RB = 64, RD = 128, RLD = RD + 8;
 
 __device__ __forceinline__ bf8v frag(const uint16_t* p) { return __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(p)); }
+
+// A raw buffer over `bytes` from p: loads past the end read 0, stores past it
+// are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rows_rsrc(const void* p, int bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+__device__ __forceinline__ f4v bload4(__amdgpu_buffer_rsrc_t r, int byte_off) {
+    return __builtin_bit_cast(f4v, __builtin_amdgcn_raw_buffer_load_b128(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void bstore4(__amdgpu_buffer_rsrc_t r, int byte_off, f4v v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4v, v), r, byte_off, 0, 0);
+}
+// Four bf16 (round to nearest even) of v, first element at the lower address.
+__device__ __forceinline__ u2v bf4(f4v v) { return u2v{f2bf2(v[0], v[1]), f2bf2(v[2], v[3])}; }
+__device__ __forceinline__ void bstore4(__amdgpu_buffer_rsrc_t r, int byte_off, u2v v) {
+    __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
+}
 
 // Fragments of W rows n0 + wn + j*16 + r16 (j < J), K columns k0 .. k0+128.
 template <int J>
@@ -46,27 +73,29 @@ __device__ __forceinline__ void load_w(bf8v (&w)[4][J], const uint16_t* Wt, long
         for (int j = 0; j < J; j++) w[k4][j] = frag(Wt + (long)(n0 + wn + j * 16 + r16) * ldk + k0 + k4 * 32 + kq);
 }
 
-// acc += A (64 x 128 in LDS, hi and lo) @ the loaded W chunk.
+// acc += (A (64 x 128 in LDS, hi and lo) @ the loaded W chunk), transposed:
+// acc[i][j][r] is token i*16 + r16, column wn + j*16 + 4*(lane/16) + r.
 template <bool SPLIT, int J>
 __device__ __forceinline__ void mma_panel(f4v (&acc)[4][J], const uint16_t* Ah, const uint16_t* Al,
                                           const bf8v (&w)[4][J], int r16, int kq) {
 #pragma unroll
     for (int k4 = 0; k4 < 4; k4++) {
-        bf8v a[4];
+        bf8v a[4], l[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) a[i] = frag(&Ah[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+        for (int i = 0; i < 4; i++) {
+            a[i] = frag(&Ah[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+            if (SPLIT) l[i] = frag(&Al[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+        }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w[k4][j], acc[i][j], 0, 0, 0);
+            for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], a[i], acc[i][j], 0, 0, 0);
         if (SPLIT) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) a[i] = frag(&Al[(i * 16 + r16) * RLD + k4 * 32 + kq]);
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < J; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], w[k4][j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], l[i], acc[i][j], 0, 0, 0);
         }
     }
 }
@@ -79,36 +108,37 @@ __device__ __forceinline__ void zero(f4v (&acc)[4][J]) {
         for (int j = 0; j < J; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 }
 
-// v as an A operand at (row, col): bf16, and with SPLIT the bf16 remainder.
+// Four consecutive columns of one row as A operands: bf16, and with SPLIT
+// the bf16 remainder.
 template <bool SPLIT>
-__device__ __forceinline__ void put_a(uint16_t* Ah, uint16_t* Al, int row, int col, float v) {
-    const uint16_t h = f2bf(v);
-    Ah[row * RLD + col] = h;
-    if (SPLIT) Al[row * RLD + col] = f2bf(v - bf2f(h));
+__device__ __forceinline__ void put_a4(uint16_t* Ah, uint16_t* Al, int row, int col, f4v v) {
+    const u2v h = bf4(v);
+    *reinterpret_cast<u2v*>(&Ah[row * RLD + col]) = h;
+    if (SPLIT) {
+        const f4v r = {v[0] - __uint_as_float(h[0] << 16), v[1] - __uint_as_float(h[0] & 0xffff0000u),
+                       v[2] - __uint_as_float(h[1] << 16), v[3] - __uint_as_float(h[1] & 0xffff0000u)};
+        *reinterpret_cast<u2v*>(&Al[row * RLD + col]) = bf4(r);
+    }
 }
 
-// 64 rows x 128 of global A (bf16, or f32 split into hi + lo) into LDS.
+// The panel's rows x 128 of global A (bf16, or f32 split into hi + lo) into
+// LDS; rows past the panel's end read as 0.
 template <bool SPLIT, int NT>
-__device__ __forceinline__ void stage_rows(const void* src, int m0, int M, uint16_t* Ah, uint16_t* Al, int tid) {
+__device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, uint16_t* Ah, uint16_t* Al, int tid) {
     if (!SPLIT) {
-        const uint16_t* A = static_cast<const uint16_t*>(src);
+        const auto r = rows_rsrc(static_cast<const uint16_t*>(src) + (long)m0 * RD, rows * RD * 2);
 #pragma unroll
         for (int q = 0; q < 1024 / NT; q++) {
-            const int v = tid + NT * q, row = v >> 4, kv = v & 15, m = m0 + row;
-            const uint4 val = m < M ? *reinterpret_cast<const uint4*>(A + (long)m * RD + kv * 8) : make_uint4(0, 0, 0, 0);
-            *reinterpret_cast<uint4*>(&Ah[row * RLD + kv * 8]) = val;
+            const int v = tid + NT * q, row = v >> 4, kv = v & 15;
+            const u4v val = __builtin_amdgcn_raw_buffer_load_b128(r, (row * RD + kv * 8) * 2, 0, 0);
+            *reinterpret_cast<u4v*>(&Ah[row * RLD + kv * 8]) = val;
         }
     } else {
-        const float* A = static_cast<const float*>(src);
+        const auto r = rows_rsrc(static_cast<const float*>(src) + (long)m0 * RD, rows * RD * 4);
 #pragma unroll
         for (int q = 0; q < 2048 / NT; q++) {
-            const int v = tid + NT * q, row = v >> 5, kv = v & 31, m = m0 + row;
-            const float4 f = m < M ? *reinterpret_cast<const float4*>(A + (long)m * RD + kv * 4) : make_float4(0.f, 0.f, 0.f, 0.f);
-            const uint32_t h01 = f2bf2(f.x, f.y), h23 = f2bf2(f.z, f.w);
-            const uint32_t l01 = f2bf2(f.x - __uint_as_float(h01 << 16), f.y - __uint_as_float(h01 & 0xffff0000u));
-            const uint32_t l23 = f2bf2(f.z - __uint_as_float(h23 << 16), f.w - __uint_as_float(h23 & 0xffff0000u));
-            *reinterpret_cast<uint2*>(&Ah[row * RLD + kv * 4]) = make_uint2(h01, h23);
-            *reinterpret_cast<uint2*>(&Al[row * RLD + kv * 4]) = make_uint2(l01, l23);
+            const int v = tid + NT * q, row = v >> 5, kv = v & 31;
+            put_a4<true>(Ah, Al, row, kv * 4, bload4(r, (row * RD + kv * 4) * 4));
         }
     }
 }
@@ -123,49 +153,53 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
     __shared__ __align__(16) uint16_t Hl[SPLIT ? RB * RLD : 8];
     __shared__ __align__(16) uint16_t Fh[RB * RLD];
     __shared__ __align__(16) uint16_t Fl[SPLIT ? RB * RLD : 8];
-    __shared__ float red[RD / 16][RB], mrow[RB], irow[RB];
+    __shared__ __align__(16) float red[RD / 16][RB][4];
+    __shared__ float mrow[RB], irow[RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * RB, wn = wave * 16 * J;
-    const int r16 = lane & 15, kq = (lane >> 4) * 8, rq = (lane >> 4) * 4;
+    const int m0 = blockIdx.x * RB, wn = wave * 16 * J, rows = min(RB, M - m0);
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
+    // byte offset of the lane's four columns of tile (i, j) in a [64][ld] f32 panel
+    auto at = [&](int i, int j, int ld) { return ((i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
     bf8v w1[4][J], w2[4][J];
-    f4v xm[4][J];  // the residual stream of the panel
 
-    // ---- the residual stream, and H, the MLP's input
-    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, M, Hh, Hl, tid);
+    // ---- H, the MLP's input: LN2 of the projected residual, or Hin
+    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hh, Hl, tid);
     if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, r16, kq);
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int m = m0 + i * 16 + rq + r;
-#pragma unroll
-            for (int j = 0; j < J; j++) xm[i][j][r] = m < M ? a.X[(long)m * RD + wn + j * 16 + r16] : 0.f;
-        }
     __syncthreads();
     if (PROJ) {
+        f4v xm[4][J];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < J; j++) xm[i][j] = bload4(rX, at(i, j, RD));
         f4v acc[4][J];
         zero(acc);
         mma_panel<SPLIT, J>(acc, Hh, Hl, w1, r16, kq);
 #pragma unroll
         for (int j = 0; j < J; j++) {
-            const float b = a.bp[wn + j * 16 + r16];
+            const f4v b = *reinterpret_cast<const f4v*>(a.bp + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 4; i++) {
 #pragma unroll
-                for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b;
+                for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b[r];
+                bstore4(rX, at(i, j, RD), xm[i][j]);  // read back after the MLP
+            }
         }
-        ln_row_stats<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of O
+        ln_row_stats_t<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of O
 #pragma unroll
         for (int j = 0; j < J; j++) {
-            const int n = wn + j * 16 + r16;
-            const float g = a.n2g[n], b = a.n2b[n];
+            const f4v g = *reinterpret_cast<const f4v*>(a.n2g + wn + j * 16 + c4);
+            const f4v b = *reinterpret_cast<const f4v*>(a.n2b + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 4; i++) {
+                const int row = i * 16 + r16;
+                const float mu = mrow[row], iv = irow[row];
+                f4v h;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = i * 16 + rq + r;
-                    put_a<SPLIT>(Hh, Hl, row, n, (xm[i][j][r] - mrow[row]) * irow[row] * g + b);
-                }
+                for (int r = 0; r < 4; r++) h[r] = (xm[i][j][r] - mu) * iv * g[r] + b[r];
+                put_a4<SPLIT>(Hh, Hl, row, wn + j * 16 + c4, h);
+            }
         }
         __syncthreads();
     }
@@ -179,59 +213,62 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
         f4v acc1[4][J];
         zero(acc1);
         mma_panel<SPLIT, J>(acc1, Hh, Hl, w1, r16, kq);
-        float bb[J];
+        f4v bb[J];
 #pragma unroll
-        for (int j = 0; j < J; j++) bb[j] = a.b1[c0 + wn + j * 16 + r16];
+        for (int j = 0; j < J; j++) bb[j] = *reinterpret_cast<const f4v*>(a.b1 + c0 + wn + j * 16 + c4);
         if (c0 + RD < ff) load_w(w1, a.W1, RD, c0 + RD, 0, wn, r16, kq);  // in flight during GELU and ff2
         __syncthreads();  // every wave has finished reading the previous intermediate chunk
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < J; j++)
+            for (int j = 0; j < J; j++) {
+                f4v v;
 #pragma unroll
-                for (int r = 0; r < 4; r++) put_a<SPLIT>(Fh, Fl, i * 16 + rq + r, wn + j * 16 + r16, gelu_fast(acc1[i][j][r] + bb[j]));
+                for (int r = 0; r < 4; r++) v[r] = gelu_fast(acc1[i][j][r] + bb[j][r]);
+                put_a4<SPLIT>(Fh, Fl, i * 16 + r16, wn + j * 16 + c4, v);
+            }
         __syncthreads();
         mma_panel<SPLIT, J>(acc2, Fh, Fl, w2, r16, kq);
     }
+    f4v xm[4][J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
-        const float b = a.b2[wn + j * 16 + r16];
+        const f4v b = *reinterpret_cast<const f4v*>(a.b2 + wn + j * 16 + c4);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++) {
+            xm[i][j] = bload4(rX, at(i, j, RD));
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int m = m0 + i * 16 + rq + r;
-                const float v = xm[i][j][r] + acc2[i][j][r] + b;
-                xm[i][j][r] = v;
-                if (m < M) a.X[(long)m * RD + wn + j * 16 + r16] = v;
-            }
+            for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc2[i][j][r] + b[r];
+            bstore4(rX, at(i, j, RD), xm[i][j]);
+        }
     }
     if (!QKV && !LNOUT) return;
 
     // ---- the next block's LN1, then its qkv product or H itself
-    ln_row_stats<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of H
+    ln_row_stats_t<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of H
     if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);  // in flight while H is written
+    const auto rH = rows_rsrc(LNOUT ? static_cast<const char*>(a.Hout) + (long)m0 * RD * (SPLIT ? 4 : 2) : nullptr,
+                              LNOUT ? rows * RD * (SPLIT ? 4 : 2) : 0);
 #pragma unroll
     for (int j = 0; j < J; j++) {
-        const int n = wn + j * 16 + r16;
-        const float g = a.n1g[n], b = a.n1b[n];
+        const f4v g = *reinterpret_cast<const f4v*>(a.n1g + wn + j * 16 + c4);
+        const f4v b = *reinterpret_cast<const f4v*>(a.n1b + wn + j * 16 + c4);
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++) {
+            const int row = i * 16 + r16;
+            const float mu = mrow[row], iv = irow[row];
+            f4v h;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const float h = (xm[i][j][r] - mrow[i * 16 + rq + r]) * irow[i * 16 + rq + r] * g + b;
-                if (QKV) {
-                    put_a<SPLIT>(Hh, Hl, i * 16 + rq + r, n, h);
-                } else {
-                    const int m = m0 + i * 16 + rq + r;
-                    if (m >= M) continue;
-                    if (SPLIT) static_cast<float*>(a.Hout)[(long)m * RD + n] = h;
-                    else static_cast<uint16_t*>(a.Hout)[(long)m * RD + n] = f2bf(h);
-                }
-            }
+            for (int r = 0; r < 4; r++) h[r] = (xm[i][j][r] - mu) * iv * g[r] + b[r];
+            if (QKV) put_a4<SPLIT>(Hh, Hl, row, wn + j * 16 + c4, h);
+            else if (SPLIT) bstore4(rH, at(i, j, RD), h);
+            else bstore4(rH, at(i, j, RD) / 2, bf4(h));
+        }
     }
     if (!QKV) return;
     __syncthreads();
+    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * RD * (SPLIT ? 4 : 2),
+                              rows * 3 * RD * (SPLIT ? 4 : 2));
 #pragma unroll 1
     for (int c = 0; c < 3; c++) {
         f4v acc[4][J];
@@ -240,18 +277,16 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
         if (c < 2) load_w(w1, a.Wqkv, RD, (c + 1) * RD, 0, wn, r16, kq);
 #pragma unroll
         for (int j = 0; j < J; j++) {
-            const int n = c * RD + wn + j * 16 + r16;
-            const float b = a.bqkv[n];
+            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * RD + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < 4; i++) {
+                f4v v;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int m = m0 + i * 16 + rq + r;
-                    if (m >= M) continue;
-                    const float v = acc[i][j][r] + b;
-                    if (SPLIT) static_cast<float*>(a.QKV)[(long)m * 3 * RD + n] = v;
-                    else static_cast<uint16_t*>(a.QKV)[(long)m * 3 * RD + n] = f2bf(v);
-                }
+                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
+                const int o = at(i, j, 3 * RD) + c * RD * 4;
+                if (SPLIT) bstore4(rQ, o, v);
+                else bstore4(rQ, o / 2, bf4(v));
+            }
         }
     }
 }

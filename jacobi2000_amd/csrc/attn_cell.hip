@@ -19,7 +19,7 @@
 //             of LDS, one barrier per plane.
 //
 // Four lanes share a query, each holding a quarter of the head's dimensions;
-// the dot products are reduced with two lane shuffles. The softmax is
+// the dot products are reduced with two DPP quad permutations. The softmax is
 // two-pass over the scores kept in registers (max, then exp / sum / weighted
 // V in one sweep), f32 throughout; keys are visited in the order time
 // offset, y offset, x offset (the order of the generic kernel and of the
@@ -228,8 +228,8 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
                     float dot = 0.f;
 #pragma unroll
                     for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
-                    dot += __shfl_xor(dot, 1, 64);
-                    dot += __shfl_xor(dot, 2, 64);
+                    dot += quad_xor1(dot);
+                    dot += quad_xor2(dot);
                     const float sc = dot * scale;
                     s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
                     mx = fmaxf(mx, sc);
@@ -361,8 +361,8 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
                             float dot = 0.f;
 #pragma unroll
                             for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
-                            dot += __shfl_xor(dot, 1, 64);
-                            dot += __shfl_xor(dot, 2, 64);
+                            dot += quad_xor1(dot);
+                            dot += quad_xor2(dot);
                             const float sc = dot * scale;
                             s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] = sc;
                             mx = fmaxf(mx, sc);

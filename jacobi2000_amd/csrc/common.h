@@ -23,6 +23,23 @@ inline void hip_check(hipError_t e, const char* what, int code = J2K_ERR_POLL) {
 
 constexpr int WAVE = 64;
 
+// x of lane ^ 1 and of lane ^ 2 (DPP quad permutations: a VALU move, where
+// __shfl_xor goes through the LDS unit's permute).
+__device__ __forceinline__ float quad_xor1(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0xB1, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float quad_xor2(float x) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, true));
+}
+
+// x of lane ^ M within each 32 lanes, M = 4, 8 or 16 (ds_swizzle in its
+// bit-mask mode: the LDS unit's crossbar without an address or a bank access).
+template <int M>
+__device__ __forceinline__ float swz_xor(float x) {
+    static_assert(M == 4 || M == 8 || M == 16, "swizzle xor within 32 lanes");
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, x), 0x1F | (M << 10)));
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -47,18 +64,20 @@ __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(
 // GELU of the served forms, whose activations are rounded to bf16 (pairs)
 // right after: erfc(z) = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2),
 // t = 1 / (1 + p z), z = |x| / sqrt(2) (Abramowitz & Stegun 7.1.26, Hastings'
-// approximation, |error of erf| <= 1.5e-7), then 1 + erf(x/sqrt2) = 2 - erfc
-// for x >= 0 and erfc for x < 0 (no cancellation on the negative side). About
-// a third of the instructions of the erff form.
+// approximation, |error of erf| <= 1.5e-7). With h = |x| erfc(z) / 2,
+// GELU(x) = max(x, 0) - h on both sides (x - h for x >= 0, and for x < 0
+// x erfc(z) / 2 = -h, with no cancellation). The 1/2 and 1/sqrt2 ride in the
+// constants, exp(-z^2) = 2^(x^2 (-log2(e) / 2)): twelve VALU operations and
+// two transcendentals.
 __device__ __forceinline__ float gelu_fast(float x) {
-    const float z = fabsf(x) * 0.70710678118654752f;
-    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
-    float p = fmaf(1.061405429f, t, -1.453152027f);
-    p = fmaf(p, t, 1.421413741f);
-    p = fmaf(p, t, -0.284496736f);
-    p = fmaf(p, t, 0.254829592f);
-    const float q = p * t * __builtin_amdgcn_exp2f(-(z * z) * 1.4426950408889634f);
-    return 0.5f * x * (x >= 0.f ? 2.0f - q : q);
+    const float a = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f * 0.70710678118654752f, a, 1.0f));
+    float p = fmaf(0.5f * 1.061405429f, t, 0.5f * -1.453152027f);
+    p = fmaf(p, t, 0.5f * 1.421413741f);
+    p = fmaf(p, t, 0.5f * -0.284496736f);
+    p = fmaf(p, t, 0.5f * 0.254829592f);
+    const float h = p * t * __builtin_amdgcn_exp2f(x * x * -0.72134752044448170f);
+    return fmaf(-a, h, fmaxf(x, 0.0f));
 }
 
 // f32 -> bf16 rounding to nearest even, on gfx950's conversion instruction

@@ -57,8 +57,11 @@ struct RowsArgs {
     void* Hout = nullptr;
 };
 bool rows_fused_ok(int dim, int ff);
-// waves: 4 (32 columns of a product each) or 8 (16 each); same results.
-void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int waves = 4);
+// Launch shapes of the row-panel kernel (same results in all): four waves of
+// 32 product columns each with 128-column MLP chunks; eight waves of 16; four
+// waves with 64-column MLP chunks (three blocks per CU).
+enum { ROWS_4W = 1, ROWS_8W = 2, ROWS_4W_F64 = 3 };
+void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int shape = ROWS_4W);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

@@ -23,10 +23,10 @@ typedef float lnf4v __attribute__((ext_vector_type(4)));
 // the butterfly xor 1, 2, 4, 8: every lane of the tile ends with the total.
 __device__ __forceinline__ float tile_sum(float x) {
 #pragma clang fp contract(off)
-    x += __shfl_xor(x, 1, 64);
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 4, 64);
-    x += __shfl_xor(x, 8, 64);
+    x += quad_xor1(x);
+    x += quad_xor2(x);
+    x += swz_xor<4>(x);
+    x += swz_xor<8>(x);
     return x;
 }
 

@@ -225,8 +225,12 @@ __global__ __launch_bounds__(256) void k_cell_embed_ln(int M, int dim, const flo
 // Sum over the L lanes of one token's group (L a power of two up to 32).
 template <int L>
 __device__ __forceinline__ float group_sum(float x) {
-#pragma unroll
-    for (int o = 1; o < L; o <<= 1) x += __shfl_xor(x, o, 64);
+    static_assert(L >= 4 && L <= 32 && (L & (L - 1)) == 0, "group of 4 to 32 lanes");
+    x += quad_xor1(x);
+    x += quad_xor2(x);
+    if (L > 4) x += swz_xor<4>(x);
+    if (L > 8) x += swz_xor<8>(x);
+    if (L > 16) x += swz_xor<16>(x);
     return x;
 }
 

@@ -63,28 +63,29 @@ __device__ __forceinline__ void bstore4(__amdgpu_buffer_rsrc_t r, int byte_off, 
     __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
 }
 
-// Fragments of W rows n0 + wn + j*16 + r16 (j < J), K columns k0 .. k0+128.
-template <int J>
-__device__ __forceinline__ void load_w(bf8v (&w)[4][J], const uint16_t* Wt, long ldk, int n0, int k0, int wn, int r16,
+// Fragments of W rows n0 + wn + j*16 + r16 (j < J), K columns k0 .. k0+32*K4.
+template <int J, int K4 = 4>
+__device__ __forceinline__ void load_w(bf8v (&w)[K4][J], const uint16_t* Wt, long ldk, int n0, int k0, int wn, int r16,
                                        int kq) {
 #pragma unroll
-    for (int k4 = 0; k4 < 4; k4++)
+    for (int k4 = 0; k4 < K4; k4++)
 #pragma unroll
         for (int j = 0; j < J; j++) w[k4][j] = frag(Wt + (long)(n0 + wn + j * 16 + r16) * ldk + k0 + k4 * 32 + kq);
 }
 
-// acc += (A (64 x 128 in LDS, hi and lo) @ the loaded W chunk), transposed:
-// acc[i][j][r] is token i*16 + r16, column wn + j*16 + 4*(lane/16) + r.
-template <bool SPLIT, int J>
+// acc += (A (64 x 32*K4 in LDS with row stride LD, hi and lo) @ the loaded W
+// chunk), transposed: acc[i][j][r] is token i*16 + r16, column
+// wn + j*16 + 4*(lane/16) + r.
+template <bool SPLIT, int J, int K4 = 4, int LD = RLD>
 __device__ __forceinline__ void mma_panel(f4v (&acc)[4][J], const uint16_t* Ah, const uint16_t* Al,
-                                          const bf8v (&w)[4][J], int r16, int kq) {
+                                          const bf8v (&w)[K4][J], int r16, int kq) {
 #pragma unroll
-    for (int k4 = 0; k4 < 4; k4++) {
+    for (int k4 = 0; k4 < K4; k4++) {
         bf8v a[4], l[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            a[i] = frag(&Ah[(i * 16 + r16) * RLD + k4 * 32 + kq]);
-            if (SPLIT) l[i] = frag(&Al[(i * 16 + r16) * RLD + k4 * 32 + kq]);
+            a[i] = frag(&Ah[(i * 16 + r16) * LD + k4 * 32 + kq]);
+            if (SPLIT) l[i] = frag(&Al[(i * 16 + r16) * LD + k4 * 32 + kq]);
         }
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -110,14 +111,14 @@ __device__ __forceinline__ void zero(f4v (&acc)[4][J]) {
 
 // Four consecutive columns of one row as A operands: bf16, and with SPLIT
 // the bf16 remainder.
-template <bool SPLIT>
+template <bool SPLIT, int LD = RLD>
 __device__ __forceinline__ void put_a4(uint16_t* Ah, uint16_t* Al, int row, int col, f4v v) {
     const u2v h = bf4(v);
-    *reinterpret_cast<u2v*>(&Ah[row * RLD + col]) = h;
+    *reinterpret_cast<u2v*>(&Ah[row * LD + col]) = h;
     if (SPLIT) {
         const f4v r = {v[0] - __uint_as_float(h[0] << 16), v[1] - __uint_as_float(h[0] & 0xffff0000u),
                        v[2] - __uint_as_float(h[1] << 16), v[3] - __uint_as_float(h[1] & 0xffff0000u)};
-        *reinterpret_cast<u2v*>(&Al[row * RLD + col]) = bf4(r);
+        *reinterpret_cast<u2v*>(&Al[row * LD + col]) = bf4(r);
     }
 }
 
@@ -144,24 +145,30 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
 }
 
 // WAVES = 4: each wave owns 32 of a product's 128 columns (2 tiles); 8:
-// 16 columns (1 tile), half the registers, twice the waves per block. The
-// arithmetic of an output element is the same.
-template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, RowsArgs a) {
-    constexpr int J = 8 / WAVES, NT = 64 * WAVES;
-    __shared__ __align__(16) uint16_t Hh[RB * RLD];
-    __shared__ __align__(16) uint16_t Hl[SPLIT ? RB * RLD : 8];
-    __shared__ __align__(16) uint16_t Fh[RB * RLD];
-    __shared__ __align__(16) uint16_t Fl[SPLIT ? RB * RLD : 8];
-    __shared__ __align__(16) float red[RD / 16][RB][4];
+// 16 columns (1 tile), half the registers, twice the waves per block. FC:
+// intermediate columns per MLP chunk; at 64 the block's LDS (52.5 KB in the
+// split form) lets three blocks share a CU. The arithmetic of an output
+// element is the same in every shape. The LayerNorm row sums borrow LDS that
+// is idle at the time: F's before the MLP, H's after it.
+template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WAVES, int FC>
+__global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 2) void k_rows(int M, int ff,
+                                                                                                 RowsArgs a) {
+    constexpr int J = 8 / WAVES, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
+    constexpr int FLD = FC + 8, JF = FC / (16 * WAVES), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
+    static_assert(JF >= 1 && FC % 32 == 0, "each wave owns at least one 16-column tile of an MLP chunk");
+    static_assert(RB * FLD * P * 2 >= 8 * RB * 16, "the LayerNorm row sums fit in F");
+    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
+    __shared__ __align__(16) uint16_t Fs[P * RB * FLD];
     __shared__ float mrow[RB], irow[RB];
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
+    uint16_t *const Fh = Fs, *const Fl = Fs + (SPLIT ? RB * FLD : 0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * RB, wn = wave * 16 * J, rows = min(RB, M - m0);
+    const int m0 = blockIdx.x * RB, wn = wave * 16 * J, wf = wave * (FC / WAVES), rows = min(RB, M - m0);
     const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
     const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
     // byte offset of the lane's four columns of tile (i, j) in a [64][ld] f32 panel
     auto at = [&](int i, int j, int ld) { return ((i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
-    bf8v w1[4][J], w2[4][J];
+    bf8v w1[4][J];
 
     // ---- H, the MLP's input: LN2 of the projected residual, or Hin
     stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hh, Hl, tid);
@@ -186,7 +193,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
                 bstore4(rX, at(i, j, RD), xm[i][j]);  // read back after the MLP
             }
         }
-        ln_row_stats_t<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of O
+        ln_row_stats_t<RB, J>(xm, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Fs), mrow, irow);  // its barriers also retire every read of O
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const f4v g = *reinterpret_cast<const f4v*>(a.n2g + wn + j * 16 + c4);
@@ -204,31 +211,32 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
         __syncthreads();
     }
 
-    // ---- MLP: 128 intermediate columns at a time
+    // ---- MLP: FC intermediate columns at a time
     f4v acc2[4][J];
     zero(acc2);
-    load_w(w1, a.W1, RD, 0, 0, wn, r16, kq);
-    for (int c0 = 0; c0 < ff; c0 += RD) {
-        load_w(w2, a.W2, ff, 0, c0, wn, r16, kq);  // in flight during ff1
-        f4v acc1[4][J];
+    bf8v v1[4][JF], w2[K4F][J];
+    load_w<JF>(v1, a.W1, RD, 0, 0, wf, r16, kq);
+    for (int c0 = 0; c0 < ff; c0 += FC) {
+        load_w<J, K4F>(w2, a.W2, ff, 0, c0, wn, r16, kq);  // in flight during ff1
+        f4v acc1[4][JF];
         zero(acc1);
-        mma_panel<SPLIT, J>(acc1, Hh, Hl, w1, r16, kq);
-        f4v bb[J];
+        mma_panel<SPLIT, JF>(acc1, Hh, Hl, v1, r16, kq);
+        f4v bb[JF];
 #pragma unroll
-        for (int j = 0; j < J; j++) bb[j] = *reinterpret_cast<const f4v*>(a.b1 + c0 + wn + j * 16 + c4);
-        if (c0 + RD < ff) load_w(w1, a.W1, RD, c0 + RD, 0, wn, r16, kq);  // in flight during GELU and ff2
+        for (int j = 0; j < JF; j++) bb[j] = *reinterpret_cast<const f4v*>(a.b1 + c0 + wf + j * 16 + c4);
+        if (c0 + FC < ff) load_w<JF>(v1, a.W1, RD, c0 + FC, 0, wf, r16, kq);  // in flight during GELU and ff2
         __syncthreads();  // every wave has finished reading the previous intermediate chunk
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
-            for (int j = 0; j < J; j++) {
+            for (int j = 0; j < JF; j++) {
                 f4v v;
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[r] = gelu_fast(acc1[i][j][r] + bb[j][r]);
-                put_a4<SPLIT>(Fh, Fl, i * 16 + r16, wn + j * 16 + c4, v);
+                put_a4<SPLIT, FLD>(Fh, Fl, i * 16 + r16, wf + j * 16 + c4, v);
             }
         __syncthreads();
-        mma_panel<SPLIT, J>(acc2, Fh, Fl, w2, r16, kq);
+        mma_panel<SPLIT, J, K4F, FLD>(acc2, Fh, Fl, w2, r16, kq);
     }
     f4v xm[4][J];
 #pragma unroll
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
     if (!QKV && !LNOUT) return;
 
     // ---- the next block's LN1, then its qkv product or H itself
-    ln_row_stats_t<RB, J>(xm, 0, wn, lane, red, mrow, irow);  // its barriers also retire every read of H
+    ln_row_stats_t<RB, J>(xm, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Hs), mrow, irow);  // every ff1 read of H is done
     if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);  // in flight while H is written
     const auto rH = rows_rsrc(LNOUT ? static_cast<const char*>(a.Hout) + (long)m0 * RD * (SPLIT ? 4 : 2) : nullptr,
                               LNOUT ? rows * RD * (SPLIT ? 4 : 2) : 0);
@@ -271,6 +279,9 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
                               rows * 3 * RD * (SPLIT ? 4 : 2));
 #pragma unroll 1
     for (int c = 0; c < 3; c++) {
+        // Under the smaller register budgets, H's fragments are read again
+        // for each chunk rather than held across all three (which spills).
+        if (WAVES == 8 || FC == 64) asm volatile("" ::: "memory");
         f4v acc[4][J];
         zero(acc);
         mma_panel<SPLIT, J>(acc, Hh, Hl, w1, r16, kq);
@@ -295,19 +306,21 @@ __global__ __launch_bounds__(64 * WAVES, WAVES / 2) void k_rows(int M, int ff, R
 
 bool rows_fused_ok(int dim, int ff) { return dim == RD && ff % RD == 0; }
 
-void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int waves) {
+void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int shape) {
     if (M <= 0) return;
     if (!rows_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "fused row kernel needs dim == 128 and ff a multiple of 128");
     const bool proj = a.Wp != nullptr, qkv = a.Wqkv != nullptr, lnout = !qkv && a.n1g != nullptr;
     if (proj ? !a.O : !a.Hin) fail(J2K_ERR_INVALID, "fused row kernel: missing input");
-    if (waves != 4 && waves != 8) fail(J2K_ERR_INVALID, "fused row kernel: 4 or 8 waves");
+    if (shape < ROWS_4W || shape > ROWS_4W_F64) fail(J2K_ERR_INVALID, "fused row kernel: unknown launch shape");
+    const int waves = shape == ROWS_8W ? 8 : 4;
     dim3 grid(cdiv(M, RB)), block(64 * waves);
-#define J2K_ROWS(S, P, Q, L)                                                                   \
-    if (split == S && proj == P && qkv == Q && lnout == L) {                                   \
-        if (waves == 4) hipLaunchKernelGGL((k_rows<S, P, Q, L, 4>), grid, block, 0, s, M, ff, a); \
-        else hipLaunchKernelGGL((k_rows<S, P, Q, L, 8>), grid, block, 0, s, M, ff, a);           \
-        LAUNCHCHK("rows_fused");                                                               \
-        return;                                                                                \
+#define J2K_ROWS(S, P, Q, L)                                                                                       \
+    if (split == S && proj == P && qkv == Q && lnout == L) {                                                       \
+        if (shape == ROWS_4W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 4, 128>), grid, block, 0, s, M, ff, a);      \
+        else if (shape == ROWS_8W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 8, 128>), grid, block, 0, s, M, ff, a); \
+        else hipLaunchKernelGGL((k_rows<S, P, Q, L, 4, 64>), grid, block, 0, s, M, ff, a);                        \
+        LAUNCHCHK("rows_fused");                                                                                   \
+        return;                                                                                                    \
     }
     J2K_ROWS(true, true, true, false)
     J2K_ROWS(true, true, false, false)

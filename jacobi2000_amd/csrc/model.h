@@ -73,7 +73,7 @@ enum TuneOp : int {
 struct Tuning {
     static constexpr int NB = 31;
     signed char pick[TO_COUNT][NB];
-    static constexpr int NC = 3;  // candidates per op at most
+    static constexpr int NC = 4;  // candidates per op at most
     float us[TO_COUNT][NB][NC];   // measured µs of each candidate (0: not a candidate)
     int tuned_to = -1;          // highest bucket tuned so far
     bool enabled = true;        // iterate / server tune on first use (J2K_AUTOTUNE=0 turns it off)

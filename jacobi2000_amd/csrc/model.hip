@@ -393,8 +393,8 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
     Act* F = rows ? nullptr : m.get<Act>(S_F, (size_t)M * ff);  // the row kernel keeps it on chip
     const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const int rows_pick = rows ? m.tune.choice(TO_ROWS, M) : 0;
-    const bool fused = rows_pick == 1 || rows_pick == 2;
-    const int rows_waves = rows_pick == 2 ? 8 : 4;
+    const bool fused = rows_pick >= ROWS_4W;
+    const int rows_shape = fused ? rows_pick : ROWS_4W;
     auto gemm = [&](int op, int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb) {
         TScope t(m, s, KC_GEMM, g / d * N * K);
@@ -451,7 +451,7 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
                 if (next) bytes += (double)M * d * sizeof(Act);
             }
             TScope t(m, s, a.QKV ? KC_ROWS : KC_ROWS_TAIL, bytes);
-            rows_fused(s, M, d, ff, split, a, rows_waves);
+            rows_fused(s, M, d, ff, split, a, rows_shape);
             continue;
         }
         if (d == 128) {

@@ -103,13 +103,13 @@ int j2k_model_config(const void* model, j2k_config* out);
  * shape per kernel): times the candidate launch shapes of the BF16 / BF16X2
  * trunk (GEMM block height, attention K/V staging: joint, split or per time
  * plane, the d = 128 token-local work apart or fused into one launch of
- * four or eight waves, or of four waves with 64-column MLP chunks) on
- * synthetic packs of 2^b tokens for
+ * four waves (in a row or 2 x 2, or with 64-column MLP chunks) or of eight
+ * waves) on synthetic packs of 2^b tokens for
  * every b from 12 up to that of max_tokens (at most 2^20) not yet tuned, and
  * keeps the fastest per bucket of packed tokens. A choice never changes a
  * result. j2k_iterate* and j2k_server_create run it on first use unless the
  * environment sets J2K_AUTOTUNE=0; J2K_TUNE_FORCE =
- * "gemm_bm=64|128,attn_kv_mode=0|1|2,rows_fused=0|1|2|3" instead fixes the choices
+ * "gemm_bm=64|128,attn_kv_mode=0|1|2,rows_fused=0..4" instead fixes the choices
  * of models created after it is set. max_tokens = 0 only reports. `report`
  * (may be NULL) receives the table as JSON text, cut to report_len bytes.
  * F32 models keep their fixed shapes. Synchronises the model's stream; not

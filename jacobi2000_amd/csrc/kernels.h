@@ -42,6 +42,7 @@ void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, float sa, 
 // X += ff2(GELU(ff1(H))); then, if Wqkv, QKV = LN1'(X) @ Wqkv + bqkv, else
 // if n1g, Hout = LN1'(X). Activations (O, Hin, Hout, QKV) are bf16, or f32
 // with split; weights are the bf16 [N, K] copies.
+// Weights Wp, W1, W2, Wqkv: bf16 in MFMA fragment order (LayerW::*_f).
 struct RowsArgs {
     const void* O = nullptr;
     const uint16_t* Wp = nullptr;
@@ -59,8 +60,9 @@ struct RowsArgs {
 bool rows_fused_ok(int dim, int ff);
 // Launch shapes of the row-panel kernel (same results in all): four waves of
 // 32 product columns each with 128-column MLP chunks; eight waves of 16; four
-// waves with 64-column MLP chunks (three blocks per CU).
-enum { ROWS_4W = 1, ROWS_8W = 2, ROWS_4W_F64 = 3 };
+// waves with 64-column MLP chunks (three blocks per CU); four waves as 2 x 2
+// over 32 tokens and 64 columns each.
+enum { ROWS_4W = 1, ROWS_8W = 2, ROWS_4W_F64 = 3, ROWS_2X2 = 4 };
 void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int shape = ROWS_4W);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------

@@ -66,21 +66,21 @@ __device__ __forceinline__ void ln_row_stats(const lnf4v (&acc)[4][J], int wm, i
     }
 }
 
-// The same statistics for rows held transposed: acc[i][j][r] is row
+// The same statistics for rows held transposed: acc[i][j][r] (i < MI) is row
 // wm + i*16 + lane%16, column wn + j*16 + 4*(lane/16) + r. A lane adds its
 // four columns as the butterfly's first two levels do, (c0 + c1) + (c2 + c3);
 // the four lanes of a row and tile leave their sums in red[tile][row][0..3]
 // and the row's thread finishes the tree, (s0 + s1) + (s2 + s3), before the
 // column-order sum over tiles. red is [8][BM][4] LDS (16-byte aligned).
-template <int BM, int J>
-__device__ __forceinline__ void ln_row_stats_t(const lnf4v (&acc)[4][J], int wm, int wn, int lane,
+template <int BM, int MI, int J>
+__device__ __forceinline__ void ln_row_stats_t(const lnf4v (&acc)[MI][J], int wm, int wn, int lane,
                                                float (*red)[BM][4], float* mrow, float* irow) {
 #pragma clang fp contract(off)
     const int r16 = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < MI; i++) {
             const int row = wm + i * 16 + r16;
             const float mu = pass == 0 ? 0.f : mrow[row];
 #pragma unroll

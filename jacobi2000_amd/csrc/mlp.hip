@@ -63,37 +63,39 @@ __device__ __forceinline__ void bstore4(__amdgpu_buffer_rsrc_t r, int byte_off, 
     __builtin_amdgcn_raw_buffer_store_b64(v, r, byte_off, 0, 0);
 }
 
-// Fragments of W rows n0 + wn + j*16 + r16 (j < J), K columns k0 .. k0+32*K4.
+// Fragments of W^T rows n0 + wn + j*16 .. +16 (j < J), K columns k0 .. k0+32*K4,
+// from W in fragment order (16-row, 32-column blocks of 64 lanes x 8 values,
+// K-blocks fastest; ldk = K): every load is 1 KB in a row.
 template <int J, int K4 = 4>
-__device__ __forceinline__ void load_w(bf8v (&w)[K4][J], const uint16_t* Wt, long ldk, int n0, int k0, int wn, int r16,
-                                       int kq) {
+__device__ __forceinline__ void load_w(bf8v (&w)[K4][J], const uint16_t* Wf, int ldk, int n0, int k0, int wn, int lane) {
+    const uint16_t* p = Wf + (((long)(n0 + wn) / 16 * (ldk / 32) + k0 / 32) * 64 + lane) * 8;
 #pragma unroll
     for (int k4 = 0; k4 < K4; k4++)
 #pragma unroll
-        for (int j = 0; j < J; j++) w[k4][j] = frag(Wt + (long)(n0 + wn + j * 16 + r16) * ldk + k0 + k4 * 32 + kq);
+        for (int j = 0; j < J; j++) w[k4][j] = frag(p + ((long)j * (ldk / 32) + k4) * 512);
 }
 
-// acc += (A (64 x 32*K4 in LDS with row stride LD, hi and lo) @ the loaded W
-// chunk), transposed: acc[i][j][r] is token i*16 + r16, column
+// acc += (A (16*MI x 32*K4 in LDS with row stride LD, hi and lo) @ the loaded
+// W chunk), transposed: acc[i][j][r] is A's row i*16 + r16, column
 // wn + j*16 + 4*(lane/16) + r.
-template <bool SPLIT, int J, int K4 = 4, int LD = RLD>
-__device__ __forceinline__ void mma_panel(f4v (&acc)[4][J], const uint16_t* Ah, const uint16_t* Al,
+template <bool SPLIT, int MI, int J, int K4 = 4, int LD = RLD>
+__device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah, const uint16_t* Al,
                                           const bf8v (&w)[K4][J], int r16, int kq) {
 #pragma unroll
     for (int k4 = 0; k4 < K4; k4++) {
-        bf8v a[4], l[4];
+        bf8v a[MI], l[MI];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < MI; i++) {
             a[i] = frag(&Ah[(i * 16 + r16) * LD + k4 * 32 + kq]);
             if (SPLIT) l[i] = frag(&Al[(i * 16 + r16) * LD + k4 * 32 + kq]);
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < MI; i++)
 #pragma unroll
             for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], a[i], acc[i][j], 0, 0, 0);
         if (SPLIT) {
 #pragma unroll
-            for (int i = 0; i < 4; i++)
+            for (int i = 0; i < MI; i++)
 #pragma unroll
                 for (int j = 0; j < J; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], l[i], acc[i][j], 0, 0, 0);
@@ -101,10 +103,10 @@ __device__ __forceinline__ void mma_panel(f4v (&acc)[4][J], const uint16_t* Ah, 
     }
 }
 
-template <int J>
-__device__ __forceinline__ void zero(f4v (&acc)[4][J]) {
+template <int MI, int J>
+__device__ __forceinline__ void zero(f4v (&acc)[MI][J]) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < MI; i++)
 #pragma unroll
         for (int j = 0; j < J; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 }
@@ -144,90 +146,97 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
     }
 }
 
-// WAVES = 4: each wave owns 32 of a product's 128 columns (2 tiles); 8:
-// 16 columns (1 tile), half the registers, twice the waves per block. FC:
-// intermediate columns per MLP chunk; at 64 the block's LDS (52.5 KB in the
-// split form) lets three blocks share a CU. The arithmetic of an output
-// element is the same in every shape. The LayerNorm row sums borrow LDS that
-// is idle at the time: F's before the MLP, H's after it.
-template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WAVES, int FC>
-__global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 2) void k_rows(int M, int ff,
-                                                                                                 RowsArgs a) {
-    constexpr int J = 8 / WAVES, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
-    constexpr int FLD = FC + 8, JF = FC / (16 * WAVES), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
+// The block's waves form a WM x WN grid over (tokens, product columns): a
+// wave owns 64/WM tokens and 128/WN columns of every 128-column product.
+// WM = 1, WN = 4 (32 columns a wave) reads the whole 64-token panel from LDS
+// for every product; WM = WN = 2 halves those reads per MFMA and loads twice
+// the weight fragments; WN = 8 (16 columns) holds half the registers, with
+// twice the waves per block. FC: intermediate columns per MLP chunk; at 64
+// the block's LDS (52.5 KB in the split form) lets three blocks share a CU.
+// The arithmetic of an output element is the same in every shape. The
+// LayerNorm row sums borrow LDS that is idle at the time: F's before the
+// MLP, H's after it.
+template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WM, int WN, int FC>
+__global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM * WN / 2) void k_rows(int M, int ff,
+                                                                                                     RowsArgs a) {
+    constexpr int WAVES = WM * WN, MI = 4 / WM, J = 8 / WN, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
+    constexpr int FLD = FC + 8, JF = FC / (16 * WN), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
     static_assert(JF >= 1 && FC % 32 == 0, "each wave owns at least one 16-column tile of an MLP chunk");
     static_assert(RB * FLD * P * 2 >= 8 * RB * 16, "the LayerNorm row sums fit in F");
     __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
     __shared__ __align__(16) uint16_t Fs[P * RB * FLD];
     __shared__ float mrow[RB], irow[RB];
-    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
-    uint16_t *const Fh = Fs, *const Fl = Fs + (SPLIT ? RB * FLD : 0);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int m0 = blockIdx.x * RB, wn = wave * 16 * J, wf = wave * (FC / WAVES), rows = min(RB, M - m0);
+    const int wm = (wave / WN) * 16 * MI, wnx = wave % WN;  // the wave's first token, its column slot
+    const int m0 = blockIdx.x * RB, wn = wnx * 16 * J, wf = wnx * (FC / WN), rows = min(RB, M - m0);
     const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    // the wave's tokens of H and F (split: hi, lo)
+    uint16_t *const Hh = Hs + wm * RLD, *const Hl = Hs + (SPLIT ? RB * RLD : 0) + wm * RLD;
+    uint16_t *const Fh = Fs + wm * FLD, *const Fl = Fs + (SPLIT ? RB * FLD : 0) + wm * FLD;
     const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
     // byte offset of the lane's four columns of tile (i, j) in a [64][ld] f32 panel
-    auto at = [&](int i, int j, int ld) { return ((i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
+    auto at = [&](int i, int j, int ld) { return ((wm + i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
     bf8v w1[4][J];
 
     // ---- H, the MLP's input: LN2 of the projected residual, or Hin
-    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hh, Hl, tid);
-    if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, r16, kq);
+    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hs, Hs + (SPLIT ? RB * RLD : 0), tid);
+    if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, lane);
     __syncthreads();
     if (PROJ) {
-        f4v xm[4][J];
+        f4v xm[MI][J];
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < MI; i++)
 #pragma unroll
             for (int j = 0; j < J; j++) xm[i][j] = bload4(rX, at(i, j, RD));
-        f4v acc[4][J];
+        f4v acc[MI][J];
         zero(acc);
-        mma_panel<SPLIT, J>(acc, Hh, Hl, w1, r16, kq);
+        mma_panel<SPLIT, MI, J>(acc, Hh, Hl, w1, r16, kq);
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const f4v b = *reinterpret_cast<const f4v*>(a.bp + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < MI; i++) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b[r];
                 bstore4(rX, at(i, j, RD), xm[i][j]);  // read back after the MLP
             }
         }
-        ln_row_stats_t<RB, J>(xm, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Fs), mrow, irow);  // its barriers also retire every read of O
+        // its barriers also retire every read of O
+        ln_row_stats_t<RB, MI, J>(xm, wm, wn, lane, reinterpret_cast<float(*)[RB][4]>(Fs), mrow, irow);
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const f4v g = *reinterpret_cast<const f4v*>(a.n2g + wn + j * 16 + c4);
             const f4v b = *reinterpret_cast<const f4v*>(a.n2b + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int row = i * 16 + r16;
+            for (int i = 0; i < MI; i++) {
+                const int row = wm + i * 16 + r16;
                 const float mu = mrow[row], iv = irow[row];
                 f4v h;
 #pragma unroll
                 for (int r = 0; r < 4; r++) h[r] = (xm[i][j][r] - mu) * iv * g[r] + b[r];
-                put_a4<SPLIT>(Hh, Hl, row, wn + j * 16 + c4, h);
+                put_a4<SPLIT>(Hh, Hl, i * 16 + r16, wn + j * 16 + c4, h);
             }
         }
         __syncthreads();
     }
 
     // ---- MLP: FC intermediate columns at a time
-    f4v acc2[4][J];
+    f4v acc2[MI][J];
     zero(acc2);
     bf8v v1[4][JF], w2[K4F][J];
-    load_w<JF>(v1, a.W1, RD, 0, 0, wf, r16, kq);
+    load_w<JF>(v1, a.W1, RD, 0, 0, wf, lane);
     for (int c0 = 0; c0 < ff; c0 += FC) {
-        load_w<J, K4F>(w2, a.W2, ff, 0, c0, wn, r16, kq);  // in flight during ff1
-        f4v acc1[4][JF];
+        load_w<J, K4F>(w2, a.W2, ff, 0, c0, wn, lane);  // in flight during ff1
+        f4v acc1[MI][JF];
         zero(acc1);
-        mma_panel<SPLIT, JF>(acc1, Hh, Hl, v1, r16, kq);
+        mma_panel<SPLIT, MI, JF>(acc1, Hh, Hl, v1, r16, kq);
         f4v bb[JF];
 #pragma unroll
         for (int j = 0; j < JF; j++) bb[j] = *reinterpret_cast<const f4v*>(a.b1 + c0 + wf + j * 16 + c4);
-        if (c0 + FC < ff) load_w<JF>(v1, a.W1, RD, c0 + FC, 0, wf, r16, kq);  // in flight during GELU and ff2
+        if (c0 + FC < ff) load_w<JF>(v1, a.W1, RD, c0 + FC, 0, wf, lane);  // in flight during GELU and ff2
         __syncthreads();  // every wave has finished reading the previous intermediate chunk
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < MI; i++)
 #pragma unroll
             for (int j = 0; j < JF; j++) {
                 f4v v;
@@ -236,14 +245,14 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 
                 put_a4<SPLIT, FLD>(Fh, Fl, i * 16 + r16, wf + j * 16 + c4, v);
             }
         __syncthreads();
-        mma_panel<SPLIT, J, K4F, FLD>(acc2, Fh, Fl, w2, r16, kq);
+        mma_panel<SPLIT, MI, J, K4F, FLD>(acc2, Fh, Fl, w2, r16, kq);
     }
-    f4v xm[4][J];
+    f4v xm[MI][J];
 #pragma unroll
     for (int j = 0; j < J; j++) {
         const f4v b = *reinterpret_cast<const f4v*>(a.b2 + wn + j * 16 + c4);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
+        for (int i = 0; i < MI; i++) {
             xm[i][j] = bload4(rX, at(i, j, RD));
 #pragma unroll
             for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc2[i][j][r] + b[r];
@@ -253,8 +262,9 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 
     if (!QKV && !LNOUT) return;
 
     // ---- the next block's LN1, then its qkv product or H itself
-    ln_row_stats_t<RB, J>(xm, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Hs), mrow, irow);  // every ff1 read of H is done
-    if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, r16, kq);  // in flight while H is written
+    // (every ff1 read of H is done)
+    ln_row_stats_t<RB, MI, J>(xm, wm, wn, lane, reinterpret_cast<float(*)[RB][4]>(Hs), mrow, irow);
+    if (QKV) load_w(w1, a.Wqkv, RD, 0, 0, wn, lane);  // in flight while H is written
     const auto rH = rows_rsrc(LNOUT ? static_cast<const char*>(a.Hout) + (long)m0 * RD * (SPLIT ? 4 : 2) : nullptr,
                               LNOUT ? rows * RD * (SPLIT ? 4 : 2) : 0);
 #pragma unroll
@@ -262,13 +272,13 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 
         const f4v g = *reinterpret_cast<const f4v*>(a.n1g + wn + j * 16 + c4);
         const f4v b = *reinterpret_cast<const f4v*>(a.n1b + wn + j * 16 + c4);
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int row = i * 16 + r16;
+        for (int i = 0; i < MI; i++) {
+            const int row = wm + i * 16 + r16;
             const float mu = mrow[row], iv = irow[row];
             f4v h;
 #pragma unroll
             for (int r = 0; r < 4; r++) h[r] = (xm[i][j][r] - mu) * iv * g[r] + b[r];
-            if (QKV) put_a4<SPLIT>(Hh, Hl, row, wn + j * 16 + c4, h);
+            if (QKV) put_a4<SPLIT>(Hh, Hl, i * 16 + r16, wn + j * 16 + c4, h);
             else if (SPLIT) bstore4(rH, at(i, j, RD), h);
             else bstore4(rH, at(i, j, RD) / 2, bf4(h));
         }
@@ -281,16 +291,16 @@ __global__ __launch_bounds__(64 * WAVES, (WAVES == 4 && FC == 64) ? 3 : WAVES / 
     for (int c = 0; c < 3; c++) {
         // Under the smaller register budgets, H's fragments are read again
         // for each chunk rather than held across all three (which spills).
-        if (WAVES == 8 || FC == 64) asm volatile("" ::: "memory");
-        f4v acc[4][J];
+        if (WAVES == 8 || FC == 64 || WM == 2) asm volatile("" ::: "memory");
+        f4v acc[MI][J];
         zero(acc);
-        mma_panel<SPLIT, J>(acc, Hh, Hl, w1, r16, kq);
-        if (c < 2) load_w(w1, a.Wqkv, RD, (c + 1) * RD, 0, wn, r16, kq);
+        mma_panel<SPLIT, MI, J>(acc, Hh, Hl, w1, r16, kq);
+        if (c < 2) load_w(w1, a.Wqkv, RD, (c + 1) * RD, 0, wn, lane);
 #pragma unroll
         for (int j = 0; j < J; j++) {
             const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * RD + wn + j * 16 + c4);
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+            for (int i = 0; i < MI; i++) {
                 f4v v;
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
@@ -311,16 +321,17 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
     if (!rows_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "fused row kernel needs dim == 128 and ff a multiple of 128");
     const bool proj = a.Wp != nullptr, qkv = a.Wqkv != nullptr, lnout = !qkv && a.n1g != nullptr;
     if (proj ? !a.O : !a.Hin) fail(J2K_ERR_INVALID, "fused row kernel: missing input");
-    if (shape < ROWS_4W || shape > ROWS_4W_F64) fail(J2K_ERR_INVALID, "fused row kernel: unknown launch shape");
+    if (shape < ROWS_4W || shape > ROWS_2X2) fail(J2K_ERR_INVALID, "fused row kernel: unknown launch shape");
     const int waves = shape == ROWS_8W ? 8 : 4;
     dim3 grid(cdiv(M, RB)), block(64 * waves);
-#define J2K_ROWS(S, P, Q, L)                                                                                       \
-    if (split == S && proj == P && qkv == Q && lnout == L) {                                                       \
-        if (shape == ROWS_4W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 4, 128>), grid, block, 0, s, M, ff, a);      \
-        else if (shape == ROWS_8W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 8, 128>), grid, block, 0, s, M, ff, a); \
-        else hipLaunchKernelGGL((k_rows<S, P, Q, L, 4, 64>), grid, block, 0, s, M, ff, a);                        \
-        LAUNCHCHK("rows_fused");                                                                                   \
-        return;                                                                                                    \
+#define J2K_ROWS(S, P, Q, L)                                                                                          \
+    if (split == S && proj == P && qkv == Q && lnout == L) {                                                          \
+        if (shape == ROWS_4W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 1, 4, 128>), grid, block, 0, s, M, ff, a);      \
+        else if (shape == ROWS_8W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 1, 8, 128>), grid, block, 0, s, M, ff, a); \
+        else if (shape == ROWS_4W_F64) hipLaunchKernelGGL((k_rows<S, P, Q, L, 1, 4, 64>), grid, block, 0, s, M, ff, a); \
+        else hipLaunchKernelGGL((k_rows<S, P, Q, L, 2, 2, 128>), grid, block, 0, s, M, ff, a);                        \
+        LAUNCHCHK("rows_fused");                                                                                      \
+        return;                                                                                                       \
     }
     J2K_ROWS(true, true, true, false)
     J2K_ROWS(true, true, false, false)

@@ -14,6 +14,8 @@ namespace j2k {
 struct LayerW {
     float *n1g, *n1b, *wqkv, *bqkv, *wp, *bp, *n2g, *n2b, *w1, *b1, *w2, *b2;
     uint16_t *wqkv_t, *wp_t, *w1_t, *w2_t;  // bf16 copies, transposed to [N, K]
+    // the same in MFMA fragment order (the row-panel kernel's operands; d = 128)
+    uint16_t *wqkv_f = nullptr, *wp_f = nullptr, *w1_f = nullptr, *w2_f = nullptr;
     // FP8 form: E4M3 codes [N, K], each weight tensor's scale per output
     // column, and the static per-tensor scales of the four products' inputs
     // (LN1 output, attention output, LN2 output, GELU output).
@@ -73,7 +75,7 @@ enum TuneOp : int {
 struct Tuning {
     static constexpr int NB = 31;
     signed char pick[TO_COUNT][NB];
-    static constexpr int NC = 4;  // candidates per op at most
+    static constexpr int NC = 5;  // candidates per op at most
     float us[TO_COUNT][NB][NC];   // measured µs of each candidate (0: not a candidate)
     int tuned_to = -1;          // highest bucket tuned so far
     bool enabled = true;        // iterate / server tune on first use (J2K_AUTOTUNE=0 turns it off)

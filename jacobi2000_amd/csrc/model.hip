@@ -190,6 +190,23 @@ struct Uploader {
             for (int n = 0; n < N; n++) t[(size_t)n * K + k] = host_bf16(w[(size_t)k * N + n]);
         return put(t.data(), t.size());
     }
+    // W [K, N] as bf16 in MFMA fragment order: block (nt, ks) of 16 rows n and
+    // 32 columns k of W^T is 64 lanes x 8 values, lane l holding
+    // W^T[16 nt + l%16][32 ks + 8 (l/16) .. +8]; blocks run ks-fastest. One
+    // wave's fragment load is then 1 KB in a row.
+    uint16_t* bf16_frag(const std::vector<float>& w, int K, int N) {
+        if (N % 16 || K % 32) fail(J2K_ERR_LIMIT, "fragment order needs N % 16 == 0 and K % 32 == 0");
+        std::vector<uint16_t> t((size_t)K * N);
+        const int ks_n = K / 32;
+        for (int nt = 0; nt < N / 16; nt++)
+            for (int ks = 0; ks < ks_n; ks++)
+                for (int l = 0; l < 64; l++)
+                    for (int e = 0; e < 8; e++) {
+                        const int n = 16 * nt + (l & 15), k = 32 * ks + 8 * (l >> 4) + e;
+                        t[(((size_t)nt * ks_n + ks) * 64 + l) * 8 + e] = host_bf16(w[(size_t)k * N + n]);
+                    }
+        return put(t.data(), t.size());
+    }
 };
 
 // Static bounds of the FP8 products' inputs (so their per-tensor scales
@@ -316,6 +333,12 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
             L.wp_t = up.bf16_t(get(w, p + "proj.weight").data, d, d);
             L.w1_t = up.bf16_t(get(w, p + "ff1.weight").data, d, ff);
             L.w2_t = up.bf16_t(get(w, p + "ff2.weight").data, ff, d);
+            if (rows_fused_ok(d, ff)) {
+                L.wqkv_f = up.bf16_frag(wqkv, d, 3 * d);
+                L.wp_f = up.bf16_frag(get(w, p + "proj.weight").data, d, d);
+                L.w1_f = up.bf16_frag(get(w, p + "ff1.weight").data, d, ff);
+                L.w2_f = up.bf16_frag(get(w, p + "ff2.weight").data, ff, d);
+            }
         }
         m->L.push_back(L);
     }
@@ -421,9 +444,9 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
         }
         if (rows) {
             RowsArgs a;
-            a.W1 = L.w1_t;
+            a.W1 = L.w1_f;
             a.b1 = L.b1;
-            a.W2 = L.w2_t;
+            a.W2 = L.w2_f;
             a.b2 = L.b2;
             a.X = X;
             if (next) {
@@ -434,12 +457,12 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
             double bytes = (double)M * d * (sizeof(Act) + 8.0);
             if (fused) {
                 a.O = O;
-                a.Wp = L.wp_t;
+                a.Wp = L.wp_f;
                 a.bp = L.bp;
                 a.n2g = L.n2g;
                 a.n2b = L.n2b;
                 if (next) {
-                    a.Wqkv = next->wqkv_t;
+                    a.Wqkv = next->wqkv_f;
                     a.bqkv = next->bqkv;
                     a.QKV = QKV;
                     bytes += (double)M * 3 * d * sizeof(Act);

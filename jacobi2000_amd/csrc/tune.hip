@@ -135,9 +135,9 @@ void tune_bucket(Model& m, int b, Events& ev) {
         const LayerW& N1 = m.L[1];
         const int bq = GEMM_BM[m.tune.pick[TO_GEMM_QKV][b]], bp = GEMM_BM[m.tune.pick[TO_GEMM_PROJ][b]];
         RowsArgs r;
-        r.W1 = L.w1_t;
+        r.W1 = L.w1_f;
         r.b1 = L.b1;
-        r.W2 = L.w2_t;
+        r.W2 = L.w2_f;
         r.b2 = L.b2;
         r.X = X;
         r.n1g = N1.n1g;
@@ -150,15 +150,15 @@ void tune_bucket(Model& m, int b, Events& ev) {
             rows_fused(s, (int)M, d, ff, split, a);
             gemm(3 * d, d, Hh, N1.wqkv_t, N1.bqkv, QKV, EPI_STORE, nullptr, nullptr, bq);
         });
-        for (int shape : {ROWS_4W, ROWS_8W, ROWS_4W_F64})
+        for (int shape : {ROWS_4W, ROWS_8W, ROWS_4W_F64, ROWS_2X2})
             m.tune.us[TO_ROWS][b][shape] = time_us(s, ev, [&] {
                 RowsArgs a = r;
                 a.O = O;
-                a.Wp = L.wp_t;
+                a.Wp = L.wp_f;
                 a.bp = L.bp;
                 a.n2g = L.n2g;
                 a.n2b = L.n2b;
-                a.Wqkv = N1.wqkv_t;
+                a.Wqkv = N1.wqkv_f;
                 a.bqkv = N1.bqkv;
                 a.QKV = QKV;
                 rows_fused(s, (int)M, d, ff, split, a, shape);
@@ -191,7 +191,7 @@ int Tuning::choice(int op, long M) const {
 }
 
 void force_tuning(Model& m, const char* spec) {
-    // "gemm_bm=64|128", "attn_kv_mode=0|1|2", "rows_fused=0|1|2|3", comma separated; every bucket
+    // "gemm_bm=64|128", "attn_kv_mode=0|1|2", "rows_fused=0..4", comma separated; every bucket
     int bm = -1, mode = -1, rows = -1;
     std::string t(spec);
     size_t pos = 0;
@@ -205,7 +205,7 @@ void force_tuning(Model& m, const char* spec) {
             const int v = std::atoi(kv.c_str() + eq + 1);
             if (k == "gemm_bm" && (v == 64 || v == 128)) bm = v == 64 ? 0 : 1;
             else if (k == "attn_kv_mode" && (v == KV_JOINT || v == KV_SPLIT || v == KV_PLANE)) mode = v;
-            else if (k == "rows_fused" && v >= 0 && v <= ROWS_4W_F64) rows = v;
+            else if (k == "rows_fused" && v >= 0 && v <= ROWS_2X2) rows = v;
             else fail(J2K_ERR_INVALID, "J2K_TUNE_FORCE: unknown setting '" + kv + "'");
         }
         pos = end + 1;
@@ -249,7 +249,7 @@ std::string tuning_json(const Model& m) {
     out += ", \"buckets\": [";
     bool first = true;
     for (int b = B_MIN; b <= m.tune.tuned_to; b++) {
-        char buf[160];
+        char buf[200];
         std::snprintf(buf, sizeof buf, "%s{\"tokens\": %ld", first ? "" : ", ", 1L << b);
         out += buf;
         first = false;
@@ -258,9 +258,9 @@ std::string tuning_json(const Model& m) {
             if (c < 0) continue;
             const int v = op == TO_ATTN || op == TO_ROWS ? c : GEMM_BM[c];
             if (op == TO_ROWS)
-                std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f, %.2f]", names[op], v,
-                              names[op], m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2],
-                              m.tune.us[op][b][3]);
+                std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f, %.2f, %.2f]", names[op],
+                              v, names[op], m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2],
+                              m.tune.us[op][b][3], m.tune.us[op][b][4]);
             else if (op == TO_ATTN)
                 std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f]", names[op], v, names[op],
                               m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2]);

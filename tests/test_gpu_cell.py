@@ -186,7 +186,7 @@ def _model_env(st, cj, precision, **env):
 
 SPECS = ["gemm_bm=64,attn_kv_mode=0,rows_fused=1", "gemm_bm=128,attn_kv_mode=1,rows_fused=0",
          "gemm_bm=64,attn_kv_mode=2,rows_fused=0", "gemm_bm=128,attn_kv_mode=2,rows_fused=2",
-         "gemm_bm=64,attn_kv_mode=1,rows_fused=3"]
+         "gemm_bm=64,attn_kv_mode=1,rows_fused=3", "gemm_bm=128,attn_kv_mode=0,rows_fused=4"]
 
 
 @pytest.mark.parametrize("precision", ["f32", "bf16", "bf16x2"])
@@ -235,7 +235,7 @@ def test_autotune_table_and_results(model20_cell):
     assert [b["tokens"] for b in rep["buckets"]] == [4096, 8192, 16384]
     for b in rep["buckets"]:
         assert b["gemm_qkv_bm"] in (64, 128) and b["gemm_proj_bm"] in (64, 128)
-        assert b["attn_kv_mode"] in (0, 1, 2) and b["rows_fused"] in (0, 1, 2, 3)
+        assert b["attn_kv_mode"] in (0, 1, 2) and b["rows_fused"] in (0, 1, 2, 3, 4)
         assert min(b["gemm_qkv_bm_us"]) > 0 and min(b["attn_kv_mode_us"]) > 0
     plain = _model_env(st, cj, "bf16x2", J2K_AUTOTUNE="0")
     assert plain.autotune(0)["buckets"] == []

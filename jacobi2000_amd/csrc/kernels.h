@@ -64,6 +64,9 @@ bool rows_fused_ok(int dim, int ff);
 // over 32 tokens and 64 columns each.
 enum { ROWS_4W = 1, ROWS_8W = 2, ROWS_4W_F64 = 3, ROWS_2X2 = 4 };
 void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int shape = ROWS_4W);
+// QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
+// arithmetic: the first layer's product after the embedding.
+void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

@@ -312,6 +312,45 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     }
 }
 
+// The first layer's qkv product on its own (the embedding wrote H): the
+// row-panel kernel's last stage, QKV = H @ Wqkv + bqkv for 64 tokens a block,
+// with the same arithmetic.
+template <bool SPLIT>
+__global__ __launch_bounds__(256, 2) void k_rows_qkv(int M, RowsArgs a) {
+    constexpr int J = 2, P = SPLIT ? 2 : 1;
+    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, wn = wave * 32, rows = min(RB, M - m0);
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
+    bf8v w[4][J];
+    load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
+    stage_rows<SPLIT, 256>(a.Hin, m0, rows, Hh, Hl, tid);
+    __syncthreads();
+    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * RD * (SPLIT ? 4 : 2),
+                              rows * 3 * RD * (SPLIT ? 4 : 2));
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        f4v acc[4][J];
+        zero(acc);
+        mma_panel<SPLIT, 4, J>(acc, Hh, Hl, w, r16, kq);
+        if (c < 2) load_w(w, a.Wqkv, RD, (c + 1) * RD, 0, wn, lane);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * RD + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                f4v v;
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
+                const int o = ((i * 16 + r16) * 3 * RD + c * RD + wn + j * 16 + c4) * 4;
+                if (SPLIT) bstore4(rQ, o, v);
+                else bstore4(rQ, o / 2, bf4(v));
+            }
+        }
+    }
+}
+
 }  // namespace
 
 bool rows_fused_ok(int dim, int ff) { return dim == RD && ff % RD == 0; }
@@ -343,6 +382,15 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
     J2K_ROWS(false, false, false, false)
 #undef J2K_ROWS
     fail(J2K_ERR_INVALID, "fused row kernel: unsupported stage combination");
+}
+
+void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {
+    if (M <= 0) return;
+    if (dim != RD) fail(J2K_ERR_LIMIT, "row-panel qkv kernel needs dim == 128");
+    if (!a.Hin || !a.Wqkv || !a.bqkv || !a.QKV) fail(J2K_ERR_INVALID, "row-panel qkv kernel: missing operand");
+    if (split) hipLaunchKernelGGL(k_rows_qkv<true>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, a);
+    else hipLaunchKernelGGL(k_rows_qkv<false>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, a);
+    LAUNCHCHK("rows_qkv");
 }
 
 }  // namespace j2k

@@ -433,7 +433,17 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
         else layernorm(s, M, d, X, lg, lb, nullptr, (uint16_t*)Hh);
     };
     if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
-    gemm(TO_GEMM_QKV, 3 * d, d, Hh, m.L[0].wqkv_t, m.L[0].bqkv, QKV, EPI_STORE, nullptr, nullptr);
+    if (fused) {
+        TScope t(m, s, KC_GEMM, g * 3 * d);
+        RowsArgs a;
+        a.Hin = Hh;
+        a.Wqkv = m.L[0].wqkv_f;
+        a.bqkv = m.L[0].bqkv;
+        a.QKV = QKV;
+        rows_qkv(s, M, d, split, a);
+    } else {
+        gemm(TO_GEMM_QKV, 3 * d, d, Hh, m.L[0].wqkv_t, m.L[0].bqkv, QKV, EPI_STORE, nullptr, nullptr);
+    }
     for (int l = 0; l < nl; l++) {
         const LayerW& L = m.L[l];
         const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;

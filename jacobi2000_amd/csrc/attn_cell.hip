@@ -1,10 +1,12 @@
 // Stencil attention of the cell_st form with K/V staged in LDS.
 //
-// A block owns an 8 x 8 tile of cells in one time slice of one segment and
-// loops over the heads. For each head it stages the K and V rows of the
-// tile's neighbourhood (the tile plus a periodic halo of RS cells in y and x,
-// over the clipped time window t-RT .. t+RT) into LDS once, so every key row
-// is read from L2 once per tile instead of once per query. Rows are loaded
+// A block owns an 8 x 8 tile of cells in two consecutive time slices of one
+// segment (four lanes per query, 512 threads) and loops over the heads. For
+// each head it stages the K and V rows of the tile's neighbourhood (the tile
+// plus a periodic halo of RS cells in y and x, over the clipped time window
+// t-RT .. t+1+RT) into LDS once, so every key row is read from L2 once per
+// tile instead of once per query, and the two slices share the planes
+// between them. Rows are loaded
 // into registers one phase ahead of their use, so the L2 round trips overlap
 // the arithmetic. Tiles are dealt to blocks XCD by XCD (xcd_remap):
 // consecutive tiles, and so the time slices whose K/V rows they share, sit
@@ -29,7 +31,8 @@
 
 namespace j2k {
 
-constexpr int CT_Y = 8, CT_X = 8;
+constexpr int CT_Y = 8, CT_X = 8, CT_T = 2;  // a tile: 8 x 8 cells of CT_T consecutive slices
+constexpr int CNT = 256 * CT_T;               // threads of a block: four lanes per query
 
 // A native 16-byte vector: copies of it stay in registers (HIP's uint4 is a
 // struct whose copies lower to memcpy and pin arrays of it in scratch).
@@ -103,18 +106,20 @@ __device__ __forceinline__ int wrap(int v, int n) {
     return v;
 }
 
-// Waves per SIMD the LDS footprint allows (two blocks per CU up to 80 KB);
-// at one, the register file holds twice as much and nothing spills.
+// Waves per SIMD for a block of CNT threads: two blocks per CU when the LDS
+// footprint allows it (up to 80 KB) and the scores are those of a 27-key
+// stencil, else one, whose register file then holds twice as much.
 template <int HD, int RS, int RT, class T, int MODE>
 constexpr int cell_occupancy() {
-    return (MODE == KV_JOINT ? 2 : 1) * (2 * RT + 1) * (CT_Y + 2 * RS) * (CT_X + 2 * RS) * HD * (int)sizeof(T) >
-                   80 * 1024
-               ? 1
-               : 2;
+    return (MODE == KV_JOINT ? 2 : 1) * (2 * RT + CT_T) * (CT_Y + 2 * RS) * (CT_X + 2 * RS) * HD * (int)sizeof(T) >
+                           80 * 1024 ||
+                   (2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 32
+               ? CNT / 256
+               : 2 * CNT / 256;
 }
 
 template <int HD, int RS, int RT, class T, int MODE>
-__global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void k_attn_cell(
+__global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void k_attn_cell(
     const T* __restrict__ qkv, T* __restrict__ out, int dim, int heads, const Seg* __restrict__ segs,
     const CellTile* __restrict__ tiles) {
     constexpr int D4 = HD / 4;               // dims per lane
@@ -123,11 +128,12 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
     constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
     constexpr int KS = (2 * RS + 1) * (2 * RS + 1), KMAX = KS * (2 * RT + 1);
     constexpr int ROWS = MODE == KV_JOINT ? 2 : 1;  // K/V rows per cell in one staging phase
-    constexpr int NV = ((2 * RT + 1) * PLANE * ROWS * VPR + 255) / 256;  // staged vectors per thread
+    constexpr int NV = ((2 * RT + CT_T) * PLANE * ROWS * VPR + CNT - 1) / CNT;  // staged vectors per thread
     extern __shared__ __align__(16) unsigned char smem[];
     const CellTile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
     const Seg sg = segs[tl.seg];
-    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + RT, sg.T - 1), nt = t_hi - t_lo + 1;
+    // planes of the tile's slices and their time neighbours
+    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + CT_T - 1 + RT, sg.T - 1), nt = t_hi - t_lo + 1;
     const int ncell = nt * PLANE, nvec = ncell * ROWS * VPR;
     T* Ks = reinterpret_cast<T*>(smem);
     T* Vs = MODE == KV_JOINT ? Ks + (size_t)ncell * HD : Ks;
@@ -150,47 +156,52 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
     // ahead; the rest (wide stencils) are loaded when the phase is staged. PF
     // is sized so that the prefetch (5 registers a vector), the scores and
     // the q / k / accumulator rows fit the register file without spilling.
-    constexpr int PF_FIT = ((cell_occupancy<HD, RS, RT, T, MODE>() == 2 ? 160 : 200) - KMAX - 3 * D4) / 5;
+    constexpr int PF_FIT = (512 / cell_occupancy<HD, RS, RT, T, MODE>() * 7 / 10 - KMAX - 3 * D4) / 5;
     constexpr int PF = NV < PF_FIT ? NV : (PF_FIT < 4 ? 4 : PF_FIT);
     int grow[PF];
 #pragma unroll
-    for (int i = 0; i < PF; i++) grow[i] = row_of(tid + 256 * i);
+    for (int i = 0; i < PF; i++) grow[i] = row_of(tid + CNT * i);
     // Loads of phase (head h, rows `w`: 0 = K, 1 = V, -1 = both) and their
     // stores into LDS; written out in place rather than as lambdas so that
     // `pre` stays in registers.
 #define J2K_WHICH(v, w) ((w) < 0 ? ((v) / VPR) & 1 : (w))
 #define J2K_FETCH(h, w)                                                                                            \
     _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                               \
-        const int v = tid + 256 * i, e = v % VPR;                                                                  \
+        const int v = tid + CNT * i, e = v % VPR;                                                                  \
         pre[i] = *reinterpret_cast<const u32x4*>(qkv + (long)grow[i] * ld + (1 + J2K_WHICH(v, w)) * dim + (h) * HD + \
                                                  e * VEC);                                                         \
     }
 #define J2K_STAGE(h, w)                                                                                            \
     {                                                                                                              \
         _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                           \
-            const int v = tid + 256 * i, c = v / (ROWS * VPR), e = v % VPR;                                        \
+            const int v = tid + CNT * i, c = v / (ROWS * VPR), e = v % VPR;                                        \
             if (v < nvec) *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = pre[i]; \
         }                                                                                                          \
         _Pragma("unroll") for (int i0 = PF; i0 < NV; i0 += 8) {                                                    \
             u32x4 tmp[8];                                                                                          \
             _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
-                const int i = i0 + j, v = tid + 256 * i, e = v % VPR;                                              \
+                const int i = i0 + j, v = tid + CNT * i, e = v % VPR;                                              \
                 if (i < NV)                                                                                        \
                     tmp[j] = *reinterpret_cast<const u32x4*>(qkv + (long)row_of(v) * ld + (1 + J2K_WHICH(v, w)) * dim + \
                                                              (h) * HD + e * VEC);                                  \
             }                                                                                                      \
             _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
-                const int i = i0 + j, v = tid + 256 * i, c = v / (ROWS * VPR), e = v % VPR;                        \
+                const int i = i0 + j, v = tid + CNT * i, c = v / (ROWS * VPR), e = v % VPR;                        \
                 if (i < NV && v < nvec)                                                                            \
                     *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = tmp[j];    \
             }                                                                                                      \
         }                                                                                                          \
     }
     u32x4 pre[PF];
-    const int q = tid >> 2, sub = tid & 3;
+    // the lane's query: cell q of the tile in slice tq (a slice past the
+    // segment's end repeats the tile's first and stores nothing); its keys
+    // are planes poff .. poff + ntq - 1 of the staged ones
+    const int q = (tid >> 2) & 63, sub = tid & 3, ts = tid >> 8;
     const int qy = tl.y0 + (q >> 3), qx = tl.x0 + (q & 7);
-    const bool live = qy < sg.ny && qx < sg.nx;
-    const long qrow = (long)sg.base + (long)tl.t * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
+    const bool live = qy < sg.ny && qx < sg.nx && tl.t + ts < sg.T;
+    const int tq = tl.t + ts < sg.T ? tl.t + ts : tl.t;
+    const int tq_lo = max(tq - RT, 0), ntq = min(tq + RT, sg.T - 1) - tq_lo + 1, poff = tq_lo - t_lo;
+    const long qrow = (long)sg.base + (long)tq * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
     const float scale = 1.0f / sqrtf((float)HD);
 
     if (MODE == KV_JOINT) {
@@ -218,12 +229,12 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
         float mx = -INFINITY;
 #pragma unroll
         for (int ct = 0; ct < 2 * RT + 1; ct++) {
-            if (ct >= nt) break;
+            if (ct >= ntq) break;
 #pragma unroll
             for (int dy = -RS; dy <= RS; dy++)
 #pragma unroll
                 for (int dx = -RS; dx <= RS; dx++) {
-                    const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
+                    const int c = ((poff + ct) * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
                     row_f32<D4>(Ks + (size_t)c * HD + sub * D4, kv);
                     float dot = 0.f;
 #pragma unroll
@@ -248,12 +259,12 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
         float l = 0.f;
 #pragma unroll
         for (int ct = 0; ct < 2 * RT + 1; ct++) {
-            if (ct >= nt) break;
+            if (ct >= ntq) break;
 #pragma unroll
             for (int dy = -RS; dy <= RS; dy++)
 #pragma unroll
                 for (int dx = -RS; dx <= RS; dx++) {
-                    const int c = (ct * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
+                    const int c = ((poff + ct) * HH + (q >> 3) + RS + dy) * HW + (q & 7) + RS + dx;
                     const float p = __expf(s[ct * KS + (dy + RS) * (2 * RS + 1) + dx + RS] - mx);
                     l += p;
                     row_f32<D4>(Vs + (size_t)c * HD + sub * D4, kv);
@@ -274,26 +285,27 @@ __global__ __launch_bounds__(256, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
 }
 
 // KV_PLANE: one time plane of K, then of V, at a time. Per head the phases
-// are K of slices t_lo .. t_hi (scores), then V of the same slices (weights),
+// are K of slices t_lo .. t_hi (scores of the queries whose window holds the
+// slice), then V of the same slices (weights),
 // each plane (tile + halo, one head) double-buffered in LDS with the next
 // phase's rows already in flight in registers: one barrier per phase and a
 // small LDS footprint, so more blocks share a CU. The arithmetic, and its
 // order, is that of the other modes.
-// Three waves per SIMD for the 27-key stencils; the scores of wider ones
-// need the registers of two.
+// Four waves per SIMD (two blocks) for the 27-key stencils; the scores of
+// wider ones need the registers of two.
 template <int HD, int RS, int RT, class T>
-__global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 32 || HD > 32 ? 2 : 3)) void k_attn_cell_plane(const T* __restrict__ qkv, T* __restrict__ out, int dim,
+__global__ __launch_bounds__(CNT, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 32 || HD > 32 ? 2 : 4)) void k_attn_cell_plane(const T* __restrict__ qkv, T* __restrict__ out, int dim,
                                                            int heads, const Seg* __restrict__ segs,
                                                            const CellTile* __restrict__ tiles) {
     constexpr int D4 = HD / 4, VEC = 16 / sizeof(T), VPR = HD / VEC;
     constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
     constexpr int KS = (2 * RS + 1) * (2 * RS + 1), KMAX = KS * (2 * RT + 1);
-    constexpr int NV = (PLANE * VPR + 255) / 256;  // staged vectors per thread per plane
+    constexpr int NV = (PLANE * VPR + CNT - 1) / CNT;  // staged vectors per thread per plane
     extern __shared__ __align__(16) unsigned char smem[];  // two planes
     T* const planes = reinterpret_cast<T*>(smem);
     const CellTile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
     const Seg sg = segs[tl.seg];
-    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + RT, sg.T - 1), nt = t_hi - t_lo + 1;
+    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + CT_T - 1 + RT, sg.T - 1), nt = t_hi - t_lo + 1;
     const long ld = 3L * dim;
     const int n = sg.ny * sg.nx;
     const int tid = threadIdx.x;
@@ -304,7 +316,7 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
     int coff[NV];
 #pragma unroll
     for (int i = 0; i < NV; i++) {
-        const int v = tid + 256 * i, c = v / VPR, cy = c / HW, cx = c - cy * HW;
+        const int v = tid + CNT * i, c = v / VPR, cy = c / HW, cx = c - cy * HW;
         coff[i] = v < PLANE * VPR ? wrap(tl.y0 - RS + cy, sg.ny) * sg.nx + wrap(tl.x0 - RS + cx, sg.nx) : 0;
     }
     u32x4 pre[NV];
@@ -314,15 +326,18 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
         const int which_ = (f) < nt ? 0 : 1, t_ = t_lo + ((f) < nt ? (f) : (f) - nt);                            \
         const T* src_ = qkv + ((long)sg.base + (long)t_ * n) * ld + (1 + which_) * dim + (h) * HD;              \
         _Pragma("unroll") for (int i = 0; i < NV; i++) {                                                          \
-            const int v = tid + 256 * i, e = v % VPR;                                                            \
+            const int v = tid + CNT * i, e = v % VPR;                                                            \
             pre[i] = *reinterpret_cast<const u32x4*>(src_ + (long)coff[i] * ld + e * VEC);                       \
         }                                                                                                        \
     }
 
-    const int q = tid >> 2, sub = tid & 3;
+    // the lane's query, as in k_attn_cell: its key planes are poff .. poff + ntq - 1
+    const int q = (tid >> 2) & 63, sub = tid & 3, ts = tid >> 8;
     const int qy = tl.y0 + (q >> 3), qx = tl.x0 + (q & 7);
-    const bool live = qy < sg.ny && qx < sg.nx;
-    const long qrow = (long)sg.base + (long)tl.t * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
+    const bool live = qy < sg.ny && qx < sg.nx && tl.t + ts < sg.T;
+    const int tq = tl.t + ts < sg.T ? tl.t + ts : tl.t;
+    const int tq_lo = max(tq - RT, 0), ntq = min(tq + RT, sg.T - 1) - tq_lo + 1, poff = tq_lo - t_lo;
+    const long qrow = (long)sg.base + (long)tq * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
     const float scale = 1.0f / sqrtf((float)HD);
     const int phases = 2 * nt;
 
@@ -338,7 +353,7 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
             T* P = planes + (size_t)b * PLANE * HD;
 #pragma unroll
             for (int i = 0; i < NV; i++) {
-                const int v = tid + 256 * i, c = v / VPR, e = v % VPR;
+                const int v = tid + CNT * i, c = v / VPR, e = v % VPR;
                 if (v < PLANE * VPR) *reinterpret_cast<u32x4*>(P + (size_t)c * HD + e * VEC) = pre[i];
             }
             __syncthreads();  // the plane is in; every thread is past the phase before the last
@@ -351,7 +366,7 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
                 // scores of slice t_lo + f
 #pragma unroll
                 for (int ct = 0; ct < 2 * RT + 1; ct++) {
-                    if (ct != f) continue;
+                    if (ct != f - poff || ct >= ntq) continue;
 #pragma unroll
                     for (int dy = -RS; dy <= RS; dy++)
 #pragma unroll
@@ -372,7 +387,7 @@ __global__ __launch_bounds__(256, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
                 // weights of slice t_lo + f - nt
 #pragma unroll
                 for (int ct = 0; ct < 2 * RT + 1; ct++) {
-                    if (ct != f - nt) continue;
+                    if (ct != f - nt - poff || ct >= ntq) continue;
 #pragma unroll
                     for (int dy = -RS; dy <= RS; dy++)
 #pragma unroll
@@ -448,13 +463,13 @@ void attn_cell_prepare() {
 
 size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode) {
     if (mode == KV_PLANE) return 2ull * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;  // two planes of one head
-    return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + 1) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
+    return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + CT_T) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
 }
 
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles) {
     tiles.clear();
     for (size_t i = 0; i < segs.size(); i++)
-        for (int t = 0; t < segs[i].T; t++)
+        for (int t = 0; t < segs[i].T; t += CT_T)
             for (int y = 0; y < segs[i].ny; y += CT_Y)
                 for (int x = 0; x < segs[i].nx; x += CT_X) tiles.push_back(CellTile{(int)i, t, y, x});
 }
@@ -468,7 +483,7 @@ static void launch(hipStream_t s, const T* qkv, T* out, int dim, int heads, int 
     const size_t smem = attn_cell_smem(hd, rs, rt, sizeof(T), mode);
     if (!f || smem > 160 * 1024) fail(J2K_ERR_LIMIT, "attention: no LDS-tiled kernel for this head size / stencil");
     void* args[] = {(void*)&qkv, (void*)&out, (void*)&dim, (void*)&heads, (void*)&segs, (void*)&tiles};
-    HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(256), args, smem, s));
+    HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(CNT), args, smem, s));
 }
 
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,

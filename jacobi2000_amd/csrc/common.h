@@ -40,21 +40,41 @@ __device__ __forceinline__ float swz_xor(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, x), 0x1F | (M << 10)));
 }
 
+// x of lane ^ o for o = 1 .. 16 (DPP or swizzle), or 32 (a permute).
+__device__ __forceinline__ float lane_xor(float x, int o) {
+    switch (o) {
+        case 1: return quad_xor1(x);
+        case 2: return quad_xor2(x);
+        case 4: return swz_xor<4>(x);
+        case 8: return swz_xor<8>(x);
+        case 16: return swz_xor<16>(x);
+        default: return __shfl_xor(x, o, WAVE);
+    }
+}
+__device__ __forceinline__ double lane_xor(double x, int o) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const float lo = lane_xor(__uint_as_float((unsigned)b), o), hi = lane_xor(__uint_as_float((unsigned)(b >> 32)), o);
+    return __builtin_bit_cast(double, (unsigned long long)__float_as_uint(lo) |
+                                          ((unsigned long long)__float_as_uint(hi) << 32));
+}
+
+// Butterfly over the wave, lanes xor 32, 16, ..., 1: every lane ends with
+// the total.
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    for (int o = 32; o > 0; o >>= 1) v += lane_xor(v, o);
     return v;
 }
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, WAVE));
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, lane_xor(v, o));
     return v;
 }
 
 __device__ __forceinline__ double wave_sum_d(double v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    for (int o = 32; o > 0; o >>= 1) v += lane_xor(v, o);
     return v;
 }
 

@@ -175,10 +175,115 @@ void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0,
     LAUNCHCHK("cell_targets");
 }
 
+// k_cell_project for a slice of at most CPT*K cells: the same passes in the
+// same order, with each thread's cells (tid, tid + CPT, ...) held in
+// registers from the first load to the one store, so a pass costs its
+// arithmetic and one block reduction instead of a round trip to memory.
+template <int K>
+__device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __restrict__ x,
+                                                  const float4* __restrict__ p, double mass, double eref,
+                                                  unsigned* __restrict__ resid, const Phys& ph,
+                                                  double (*sh)[CPT / 64], float* shm) {
+    const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
+    const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
+    float4 c[K];
+    bool in[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int i = threadIdx.x + CPT * k;
+        in[k] = i < S.n;
+        c[k] = in[k] ? x[i] : make_float4(0.f, 0.f, 0.f, 1.f);
+    }
+    double v[2] = {0, 0};
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        if (in[k]) {
+            v[0] += (double)c[k].x;
+            v[1] += c[k].w != 0.f ? 0.0 : 1.0;
+        }
+    block_sum<2, CPT>(v, sh);
+    const double nb = v[1];
+    if (nb > 0) {
+        const double delta = (mass * (double)S.n - v[0]) / nb;
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (c[k].w == 0.f) c[k].x = (float)((double)c[k].x + delta);
+        double e[5] = {0, 0, 0, 0, 0};  // pk, kk, pb, kb, sum of blank h
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!in[k]) continue;
+            double h, u, w;
+            phys_huv(c[k], ph, h, u, w);
+            const double pp = 0.5 * g * h * h, kq = 0.5 * h * (u * u + w * w);
+            if (c[k].w != 0.f) {
+                e[0] += pp;
+                e[1] += kq;
+            } else {
+                e[2] += pp;
+                e[3] += kq;
+                e[4] += h;
+            }
+        }
+        block_sum<5, CPT>(e, sh);
+        const double bound = (double)ph.r_max * eref;
+        if (e[0] + e[1] + e[2] + e[3] > bound) {
+            const double target = bound * (1.0 - 1e-7);
+            const double budget = target - e[0] - e[1] - e[2];
+            double su = 0.0;
+            if (budget >= 0) su = fmin(sqrt(budget / fmax(e[3], 1e-12)), 1.0);
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (c[k].w != 0.f) continue;
+                const double u = (double)c[k].y * s1 + m1, w = (double)c[k].z * s2 + m2;
+                c[k].y = (float)((u * su - m1) / s1);
+                c[k].z = (float)((w * su - m2) / s2);
+            }
+            if (budget < 0) {
+                const double hbar = e[4] / nb;
+                double a[1] = {0};
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (c[k].w != 0.f) continue;
+                    const double d = (double)c[k].x * s0 + m0 - hbar;
+                    a[0] += d * d;
+                }
+                block_sum<1, CPT>(a, sh);
+                const double avail = target - e[0] - e[1] - 0.5 * g * nb * hbar * hbar;
+                double lam = 0.0;
+                if (avail > 0 && a[0] > 0) lam = fmin(sqrt(avail / (0.5 * g * a[0])), 1.0);
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (c[k].w != 0.f) continue;
+                    const double h = (double)c[k].x * s0 + m0;
+                    c[k].x = (float)(((hbar + lam * (h - hbar)) - m0) / s0);
+                }
+            }
+        }
+    }
+    float r = 0.f;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        if (!in[k]) continue;
+        x[threadIdx.x + CPT * k] = c[k];
+        const float4 b = p[threadIdx.x + CPT * k];
+        if (c[k].w != 0.f) continue;
+        r = fmaxf(r, fmaxf(fabsf(c[k].x - b.x), fmaxf(fabsf(c[k].y - b.y), fabsf(c[k].z - b.z))));
+    }
+    r = wave_max(r);
+    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = r;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int w = 0; w < CPT / 64; w++) t = fmaxf(t, shm[w]);
+        atomicMax(resid + S.req, __float_as_uint(t));
+    }
+}
+
 // Blank-cell projection of one slice: mass shift, then scale blank (u, v)
 // down to r_max*Eref*(1-1e-7); if potential alone is over it, zero blank
 // (u, v) and shrink blank h anomalies about their mean (mass unchanged).
-// Then the slice's max |xn - xc| over blank cells.
+// Then the slice's max |xn - xc| over blank cells. Slices of up to 4096
+// cells keep their cells in registers (cell_project_regs).
 __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
                                                      const float4* __restrict__ xc, const double* __restrict__ mass_t,
                                                      const double* __restrict__ eref_t, unsigned* __restrict__ resid,
@@ -187,6 +292,10 @@ __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ 
     __shared__ float shm[CPT / 64];
     const Slice S = sl[blockIdx.x];
     float4* x = xn + S.start;
+    if (S.n <= 4 * CPT) {
+        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, ph, sh, shm);
+        return;
+    }
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
     const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
     {

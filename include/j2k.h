@@ -183,8 +183,9 @@ int j2k_fill_blank(const float* grid, const uint8_t* known, int T, int ny, int n
  * Writes the final iterate to grid_io; iters, resid, violation may be NULL. */
 int j2k_iterate(void* model, float* grid_io, const uint8_t* known, const float* meta, int T, int ny, int nx,
                 int max_iter, float tol, int* iters, float* resid, double* violation);
-/* Device-pointer form; synchronises only for the per-iteration convergence
- * test. d_grid [3,T,ny,nx], d_known [T,ny,nx] u8, d_meta [meta_dim].
+/* Device-pointer form; synchronises for the per-iteration convergence test
+ * and once at the end (the violation; the grid is written on return).
+ * d_grid [3,T,ny,nx], d_known [T,ny,nx] u8, d_meta [meta_dim].
  * From the second call with the same grid shape on, both forms replay each
  * iteration from a HIP graph (captured on that call; results bit-identical
  * to the eager launches) unless per-kernel timing is on or the environment

@@ -215,9 +215,8 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     };
     while (it < max_iter) {
         iteration(xa, xb);
-        unsigned bits = 0;
-        d2h(&bits, p.resid, sizeof(unsigned), s);
-        std::memcpy(&r, &bits, 4);
+        d2h(m.h_words, p.resid, sizeof(unsigned), s);
+        std::memcpy(&r, m.h_words, 4);
         std::swap(xa, xb);
         it++;
         if (r < tol) break;
@@ -233,9 +232,10 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
     cell_violation(s, p.n_slices, p.slices, xa, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
                    m.cfg.norm_depth, m.cfg.energy_r_max);
-    double v = 0;
-    d2h(&v, vb, sizeof(double), s);
     cell_unpack(s, p.M, xa, d_grid);
+    double v = 0;
+    d2h(m.h_words + 2, vb, sizeof(double), s);  // waits for the unpack too
+    std::memcpy(&v, m.h_words + 2, sizeof(double));
     if (iters) *iters = it;
     if (resid) *resid = r;
     if (violation) *violation = v;

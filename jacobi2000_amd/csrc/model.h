@@ -132,6 +132,9 @@ struct Model {
         hipGraphExec_t g[2] = {nullptr, nullptr};
     } cell_graph;
     unsigned graph_epoch = 0;
+    // pinned host words for the per-iteration residual and the violation
+    // (device -> host copies into pageable memory take a staging detour)
+    unsigned* h_words = nullptr;
     bool graphs_off = false;  // J2K_GRAPHS=0 at creation, or a capture failed: run eagerly
     void drop_cell_graphs();
 

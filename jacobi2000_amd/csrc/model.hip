@@ -33,6 +33,7 @@ Model::~Model() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     drop_cell_graphs();
+    if (h_words) (void)hipHostFree(h_words);
     for (void* p : owned) (void)hipFree(p);
     for (int i = 0; i < S_COUNT; i++)
         if (slot[i]) (void)hipFree(slot[i]);
@@ -265,6 +266,7 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
     if (const char* e = std::getenv("J2K_GRAPHS")) m->graphs_off = std::strcmp(e, "0") == 0;
     m->device = device;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
+    hip_check(hipHostMalloc((void**)&m->h_words, 4 * sizeof(unsigned)), "hipHostMalloc", J2K_ERR_OOM);
     if (cfg.form == J2K_FORM_CELL_ST) attn_cell_prepare();
     Uploader up{*m};
     const int d = cfg.dim, ff = cfg.ff_inner;

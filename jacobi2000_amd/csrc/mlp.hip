@@ -81,25 +81,29 @@ __device__ __forceinline__ void load_w(bf8v (&w)[K4][J], const uint16_t* Wf, int
 template <bool SPLIT, int MI, int J, int K4 = 4, int LD = RLD>
 __device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah, const uint16_t* Al,
                                           const bf8v (&w)[K4][J], int r16, int kq) {
+    // the operands of step k4 + 1 are read while step k4 multiplies
+    bf8v a[2][MI], l[2][MI];
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        a[0][i] = frag(&Ah[(i * 16 + r16) * LD + kq]);
+        if (SPLIT) l[0][i] = frag(&Al[(i * 16 + r16) * LD + kq]);
+    }
 #pragma unroll
     for (int k4 = 0; k4 < K4; k4++) {
-        bf8v a[MI], l[MI];
+        const int b = k4 & 1;
+        if (k4 + 1 < K4)
 #pragma unroll
-        for (int i = 0; i < MI; i++) {
-            a[i] = frag(&Ah[(i * 16 + r16) * LD + k4 * 32 + kq]);
-            if (SPLIT) l[i] = frag(&Al[(i * 16 + r16) * LD + k4 * 32 + kq]);
-        }
+            for (int i = 0; i < MI; i++) {
+                a[b ^ 1][i] = frag(&Ah[(i * 16 + r16) * LD + (k4 + 1) * 32 + kq]);
+                if (SPLIT) l[b ^ 1][i] = frag(&Al[(i * 16 + r16) * LD + (k4 + 1) * 32 + kq]);
+            }
 #pragma unroll
         for (int i = 0; i < MI; i++)
 #pragma unroll
-            for (int j = 0; j < J; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], a[i], acc[i][j], 0, 0, 0);
-        if (SPLIT) {
-#pragma unroll
-            for (int i = 0; i < MI; i++)
-#pragma unroll
-                for (int j = 0; j < J; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], l[i], acc[i][j], 0, 0, 0);
-        }
+            for (int j = 0; j < J; j++) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], a[b][i], acc[i][j], 0, 0, 0);
+                if (SPLIT) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], l[b][i], acc[i][j], 0, 0, 0);
+            }
     }
 }
 

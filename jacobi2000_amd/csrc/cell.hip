@@ -33,6 +33,22 @@ void check_cell_dims(const Model& m, const CellDims& d) {
 static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X) {
     const int d = m.cfg.dim;
     const LayerW& L0 = m.L.at(0);
+    if ((m.bf16() || m.split()) && rows_fused_ok(d, m.cfg.ff_inner) && L0.wqkv_f) {
+        // the embedding, LN1 and the first qkv product in one launch
+        RowsArgs a;
+        a.X = X;
+        a.n1g = L0.n1g;
+        a.n1b = L0.n1b;
+        a.Wqkv = L0.wqkv_f;
+        a.bqkv = L0.bqkv;
+        a.QKV = m.get<char>(S_QKV, (size_t)p.M * 3 * d * (m.split() ? 4 : 2));
+        {
+            TScope t(m, s, KC_GEMM, 2.0 * p.M * d * 3 * d);
+            rows_embed_qkv(s, p.M, d, m.split(), x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, a);
+        }
+        trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p));
+        return;
+    }
     if (m.bf16()) {
         uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, nullptr, Hb);
@@ -40,7 +56,7 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         float* H = m.get<float>(S_H, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, H, nullptr);
     }
-    trunk(m, s, p.M, X, true, plan_of(p));
+    trunk(m, s, p.M, X, TRUNK_H, plan_of(p));
 }
 
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn) {

@@ -67,6 +67,11 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
 // QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
 // arithmetic: the first layer's product after the embedding.
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);
+// The cell_st embedding (x4 [M] float4, we [4][128], be, mvec per request
+// through tok_req when not NULL) into X, the first block's LN1 (n1g, n1b)
+// and QKV = LN1(X) @ Wqkv + bqkv, in one launch (dim == 128).
+void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, const float* we, const float* be,
+                    const float* mvec, const int* tok_req, const RowsArgs& a);
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.

@@ -316,6 +316,95 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     }
 }
 
+// The cell_st embedding, the first block's LN1 and its qkv product in one
+// launch (d = 128; the served forms): X = embed(x4) + bias + meta vector
+// (the per-element arithmetic of k_cell_embed_ln), LN1 with the row-panel
+// kernel's statistics, H in LDS only, QKV = H @ Wqkv + bqkv as in
+// k_rows_qkv. 64 tokens a block; four waves of 32 columns.
+template <bool SPLIT>
+__global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* __restrict__ x4,
+                                                           const float* __restrict__ we, const float* __restrict__ be,
+                                                           const float* __restrict__ mvec,
+                                                           const int* __restrict__ tok_req, RowsArgs a) {
+    constexpr int J = 2, P = SPLIT ? 2 : 1;
+    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
+    __shared__ float mrow[RB], irow[RB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, wn = wave * 32, rows = min(RB, M - m0);
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
+    bf8v w[4][J];
+    load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
+    const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
+    f4v xv[4][J];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = m0 + i * 16 + r16, mm = m < M ? m : M - 1;
+        const float4 f = x4[mm];
+        const float* mv = mvec + (tok_req ? (long)tok_req[mm] * RD : 0);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int d0 = wn + j * 16 + c4;
+            const f4v bv = *reinterpret_cast<const f4v*>(be + d0);
+            const f4v w0 = *reinterpret_cast<const f4v*>(we + d0);
+            const f4v w1 = *reinterpret_cast<const f4v*>(we + RD + d0);
+            const f4v w2 = *reinterpret_cast<const f4v*>(we + 2 * RD + d0);
+            const f4v w3 = *reinterpret_cast<const f4v*>(we + 3 * RD + d0);
+            const f4v mw = *reinterpret_cast<const f4v*>(mv + d0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float v = bv[k];
+                if (f.x != 0.f) v += f.x * w0[k];
+                if (f.y != 0.f) v += f.y * w1[k];
+                if (f.z != 0.f) v += f.z * w2[k];
+                if (f.w != 0.f) v += f.w * w3[k];
+                v += mw[k];
+                xv[i][j][k] = v;
+            }
+            bstore4(rX, ((i * 16 + r16) * RD + d0) * 4, xv[i][j]);
+        }
+    }
+    ln_row_stats_t<RB, 4, J>(xv, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Hs), mrow, irow);
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const f4v g = *reinterpret_cast<const f4v*>(a.n1g + wn + j * 16 + c4);
+        const f4v b = *reinterpret_cast<const f4v*>(a.n1b + wn + j * 16 + c4);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int row = i * 16 + r16;
+            const float mu = mrow[row], iv = irow[row];
+            f4v h;
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[r] = (xv[i][j][r] - mu) * iv * g[r] + b[r];
+            put_a4<SPLIT>(Hh, Hl, row, wn + j * 16 + c4, h);
+        }
+    }
+    __syncthreads();
+    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * RD * (SPLIT ? 4 : 2),
+                              rows * 3 * RD * (SPLIT ? 4 : 2));
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        asm volatile("" ::: "memory");  // H's fragments read per chunk, not held across all three
+        f4v acc[4][J];
+        zero(acc);
+        mma_panel<SPLIT, 4, J>(acc, Hh, Hl, w, r16, kq);
+        if (c < 2) load_w(w, a.Wqkv, RD, (c + 1) * RD, 0, wn, lane);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * RD + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                f4v v;
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
+                const int o = ((i * 16 + r16) * 3 * RD + c * RD + wn + j * 16 + c4) * 4;
+                if (SPLIT) bstore4(rQ, o, v);
+                else bstore4(rQ, o / 2, bf4(v));
+            }
+        }
+    }
+}
+
 // The first layer's qkv product on its own (the embedding wrote H): the
 // row-panel kernel's last stage, QKV = H @ Wqkv + bqkv for 64 tokens a block,
 // with the same arithmetic.
@@ -386,6 +475,21 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
     J2K_ROWS(false, false, false, false)
 #undef J2K_ROWS
     fail(J2K_ERR_INVALID, "fused row kernel: unsupported stage combination");
+}
+
+void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, const float* we, const float* be,
+                    const float* mvec, const int* tok_req, const RowsArgs& a) {
+    if (M <= 0) return;
+    if (dim != RD) fail(J2K_ERR_LIMIT, "row-panel embedding kernel needs dim == 128");
+    if (!a.X || !a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV)
+        fail(J2K_ERR_INVALID, "row-panel embedding kernel: missing operand");
+    if (split)
+        hipLaunchKernelGGL(k_rows_embed_qkv<true>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, (const float4*)x4, we, be,
+                           mvec, tok_req, a);
+    else
+        hipLaunchKernelGGL(k_rows_embed_qkv<false>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, (const float4*)x4, we, be,
+                           mvec, tok_req, a);
+    LAUNCHCHK("rows_embed_qkv");
 }
 
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {

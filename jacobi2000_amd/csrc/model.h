@@ -162,9 +162,12 @@ struct DeviceGuard {
 Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision);
 
 // The transformer trunk (cpu.rs:287-313) over M packed tokens: X [M, dim] is
-// the residual stream; H must hold LN1 of block 0 when h_ready, else it is
-// computed. Leaves the stream un-normalised (callers apply the final norm).
-void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap);
+// the residual stream. `ready` says what the caller has made of the first
+// block's input: X only (LN1 is computed here), X and H = LN1(X), or X and
+// the first block's QKV (the served bf16 forms). Leaves the stream
+// un-normalised (callers apply the final norm).
+enum { TRUNK_X = 0, TRUNK_H = 1, TRUNK_QKV = 2 };
+void trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap);
 
 // patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
 // add_state the result is state + forward (cpu.rs:84-88).

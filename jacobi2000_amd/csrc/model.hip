@@ -408,7 +408,7 @@ void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPl
 // qkv GEMMs run on their own and the row kernel does the MLP and LN1'.
 // Other widths use the tile GEMMs with LayerNorm epilogues or launches.
 template <class Act>
-void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
+void trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
     constexpr bool split = sizeof(Act) == 4;
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
@@ -434,8 +434,9 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
         if (split) layernorm(s, M, d, X, lg, lb, (float*)Hh, nullptr);
         else layernorm(s, M, d, X, lg, lb, nullptr, (uint16_t*)Hh);
     };
-    if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
-    if (fused) {
+    if (ready == TRUNK_X) norm(m.L[0].n1g, m.L[0].n1b);
+    if (ready == TRUNK_QKV) {
+    } else if (fused) {
         TScope t(m, s, KC_GEMM, g * 3 * d);
         RowsArgs a;
         a.Hin = Hh;
@@ -511,7 +512,7 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, bool h_ready, const At
 // products on E4M3 codes (tile GEMM quantizing its A operand), attention,
 // LayerNorm and residual in f32. d = 128 takes the LayerNorms in the GEMM
 // epilogues.
-void trunk_f8(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
+void trunk_f8(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     float* Hh = m.get<float>(S_H, (size_t)M * d);
     float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
@@ -528,7 +529,7 @@ void trunk_f8(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Attn
         TScope t(m, s, KC_NORM, ln_b);
         layernorm(s, M, d, X, lg, lb, Hh, nullptr);
     };
-    if (!h_ready) norm(m.L[0].n1g, m.L[0].n1b);
+    if (ready == TRUNK_X) norm(m.L[0].n1g, m.L[0].n1b);
     for (int l = 0; l < nl; l++) {
         const LayerW& L = m.L[l];
         const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;
@@ -557,7 +558,9 @@ void trunk_f8(Model& m, hipStream_t s, int M, float* X, bool h_ready, const Attn
 
 }  // namespace
 
-void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPlan& ap) {
+void trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
+    if (ready == TRUNK_QKV && (m.cfg.precision == J2K_PREC_F32 || m.fp8()))
+        fail(J2K_ERR_INVALID, "trunk: a ready QKV needs a served bf16 form");
     const int d = m.cfg.dim, ff = m.cfg.ff_inner;
     if (m.cfg.precision == J2K_PREC_F32) {
         float* Hh = m.get<float>(S_H, (size_t)M * d);
@@ -567,7 +570,7 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPla
         const double ln_b = (double)M * d * 8, at_b = 16.0 * M * d, g = 2.0 * M * d;
         for (int l = 0; l < m.cfg.layers; l++) {
             const LayerW& L = m.L[l];
-            if (l > 0 || !h_ready) {
+            if (l > 0 || ready == TRUNK_X) {
                 TScope t(m, s, KC_NORM, ln_b);
                 layernorm(s, M, d, X, L.n1g, L.n1b, Hh, nullptr);
             }
@@ -598,9 +601,9 @@ void trunk(Model& m, hipStream_t s, int M, float* X, bool h_ready, const AttnPla
         }
         return;
     }
-    if (m.fp8()) trunk_f8(m, s, M, X, h_ready, ap);
-    else if (m.split()) trunk_lowp<float>(m, s, M, X, h_ready, ap);
-    else trunk_lowp<uint16_t>(m, s, M, X, h_ready, ap);
+    if (m.fp8()) trunk_f8(m, s, M, X, ready, ap);
+    else if (m.split()) trunk_lowp<float>(m, s, M, X, ready, ap);
+    else trunk_lowp<uint16_t>(m, s, M, X, ready, ap);
 }
 
 void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state) {
@@ -631,7 +634,7 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
     ap.segs = static_cast<const Seg*>(m.slot[S_SEGS]);
     ap.tiles = static_cast<const Tile*>(m.slot[S_TILES]);
     ap.n_tiles = m.p_tiles;
-    trunk(m, s, M, X, false, ap);
+    trunk(m, s, M, X, TRUNK_X, ap);
 
     // Final norm and CellHead (cpu.rs:315-349).
     float* xn = m.get<float>(S_XN, (size_t)M * d);

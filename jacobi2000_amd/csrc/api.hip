@@ -213,10 +213,21 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
         if (replay && cg.g[parity]) HIPCHK(hipGraphLaunch(cg.g[parity], s));
         else cell_iteration(m, s, p, x0, xc, xn);
     };
+    // Each iteration also finishes its iterate (energy violation, unpack into
+    // d_grid) before the one synchronisation that reads the residual, so a
+    // request that stops there needs no second round trip; a later
+    // iteration overwrites both.
+    unsigned long long* vb = reinterpret_cast<unsigned long long*>(p.resid + 2);
+    double v = 0;
     while (it < max_iter) {
         iteration(xa, xb);
-        d2h(m.h_words, p.resid, sizeof(unsigned), s);
+        fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
+        cell_violation(s, p.n_slices, p.slices, xb, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
+                       m.cfg.norm_depth, m.cfg.energy_r_max);
+        cell_unpack(s, p.M, xb, d_grid);
+        d2h(m.h_words, p.resid, 4 * sizeof(unsigned), s);  // the residual word, then the violation
         std::memcpy(&r, m.h_words, 4);
+        std::memcpy(&v, m.h_words + 2, sizeof(double));
         std::swap(xa, xb);
         it++;
         if (r < tol) break;
@@ -228,14 +239,14 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
         if (cg.g[0] || cg.g[1]) m.drop_cell_graphs();
         else cg.epoch = m.graph_epoch;
     }
-    unsigned long long* vb = reinterpret_cast<unsigned long long*>(p.resid + 2);
-    fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
-    cell_violation(s, p.n_slices, p.slices, xa, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
-                   m.cfg.norm_depth, m.cfg.energy_r_max);
-    cell_unpack(s, p.M, xa, d_grid);
-    double v = 0;
-    d2h(m.h_words + 2, vb, sizeof(double), s);  // waits for the unpack too
-    std::memcpy(&v, m.h_words + 2, sizeof(double));
+    if (it == 0) {  // no iteration ran: the packed start, unpacked
+        fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
+        cell_violation(s, p.n_slices, p.slices, xa, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
+                       m.cfg.norm_depth, m.cfg.energy_r_max);
+        cell_unpack(s, p.M, xa, d_grid);
+        d2h(m.h_words + 2, vb, sizeof(double), s);
+        std::memcpy(&v, m.h_words + 2, sizeof(double));
+    }
     if (iters) *iters = it;
     if (resid) *resid = r;
     if (violation) *violation = v;

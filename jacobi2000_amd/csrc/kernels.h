@@ -174,8 +174,9 @@ void add_inplace(hipStream_t s, long n, float* x, const float* y);
 struct Slice {
     int start, n, req;
 };
-// [3,T,ny,nx] planes <-> token-major float4 (h, u, v, known).
-void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4);
+// [3,T,ny,nx] planes <-> token-major float4 (h, u, v, known); the pack also
+// writes `copy` when it is not NULL.
+void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4, float* copy = nullptr);
 void cell_unpack(hipStream_t s, int M, const float* x4, float* grid);
 // X = embed([h,u,v,known]) + mvec (mvec per request), H = LN1_0(X).
 void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* we, const float* be,

@@ -146,14 +146,17 @@ void add_inplace(hipStream_t s, long n, float* x, const float* y) {
 
 // ---- cell_st per-token kernels ---------------------------------------------
 
-__global__ void k_cell_pack(int M, const float* __restrict__ g, const uint8_t* __restrict__ kn, float4* __restrict__ x4) {
+__global__ void k_cell_pack(int M, const float* __restrict__ g, const uint8_t* __restrict__ kn, float4* __restrict__ x4,
+                            float4* __restrict__ copy) {
     int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= M) return;
-    x4[m] = make_float4(g[m], g[(long)M + m], g[2 * (long)M + m], kn[m] ? 1.0f : 0.0f);
+    const float4 v = make_float4(g[m], g[(long)M + m], g[2 * (long)M + m], kn[m] ? 1.0f : 0.0f);
+    x4[m] = v;
+    if (copy) copy[m] = v;
 }
 
-void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4) {
-    hipLaunchKernelGGL(k_cell_pack, dim3(cdiv(M, 256)), dim3(256), 0, s, M, grid, known, (float4*)x4);
+void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4, float* copy) {
+    hipLaunchKernelGGL(k_cell_pack, dim3(cdiv(M, 256)), dim3(256), 0, s, M, grid, known, (float4*)x4, (float4*)copy);
     LAUNCHCHK("cell_pack");
 }
 

@@ -182,6 +182,12 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     auto at = [&](int i, int j, int ld) { return ((wm + i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
     bf8v w1[4][J];
 
+    // With the register budget of the four-wave 128-column shape the
+    // projected residual stays in registers through the MLP; the other
+    // shapes park it in X and read it back.
+    constexpr bool KEEP = PROJ && WM == 1 && WN == 4 && FC == 128;
+    f4v xk[KEEP ? MI : 1][J];
+
     // ---- H, the MLP's input: LN2 of the projected residual, or Hin
     stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hs, Hs + (SPLIT ? RB * RLD : 0), tid);
     if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, lane);
@@ -202,7 +208,8 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
             for (int i = 0; i < MI; i++) {
 #pragma unroll
                 for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc[i][j][r] + b[r];
-                bstore4(rX, at(i, j, RD), xm[i][j]);  // read back after the MLP
+                if constexpr (KEEP) xk[i][j] = xm[i][j];
+                else bstore4(rX, at(i, j, RD), xm[i][j]);  // read back after the MLP
             }
         }
         // its barriers also retire every read of O
@@ -257,7 +264,8 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
         const f4v b = *reinterpret_cast<const f4v*>(a.b2 + wn + j * 16 + c4);
 #pragma unroll
         for (int i = 0; i < MI; i++) {
-            xm[i][j] = bload4(rX, at(i, j, RD));
+            if constexpr (KEEP) xm[i][j] = xk[i][j];
+            else xm[i][j] = bload4(rX, at(i, j, RD));
 #pragma unroll
             for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc2[i][j][r] + b[r];
             bstore4(rX, at(i, j, RD), xm[i][j]);

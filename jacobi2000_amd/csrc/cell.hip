@@ -30,7 +30,8 @@ void check_cell_dims(const Model& m, const CellDims& d) {
     if ((long)d.T * d.ny * d.nx > (1L << 28)) fail(J2K_ERR_SHAPE, "grid too large (over 2^28 cells)");
 }
 
-static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X) {
+// The trunk over the pack, ending in the readout `ro`.
+static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X, const Readout& ro) {
     const int d = m.cfg.dim;
     const LayerW& L0 = m.L.at(0);
     if ((m.bf16() || m.split()) && rows_fused_ok(d, m.cfg.ff_inner) && L0.wqkv_f) {
@@ -46,7 +47,8 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
             TScope t(m, s, KC_GEMM, 2.0 * p.M * d * 3 * d);
             rows_embed_qkv(s, p.M, d, m.split(), x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, a);
         }
-        trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p));
+        if (!trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p), &ro))
+            cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
         return;
     }
     if (m.bf16()) {
@@ -56,14 +58,21 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         float* H = m.get<float>(S_H, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, H, nullptr);
     }
-    trunk(m, s, p.M, X, TRUNK_H, plan_of(p));
+    if (!trunk(m, s, p.M, X, TRUNK_H, plan_of(p), &ro))
+        cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
 }
 
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn) {
     const j2k_config& c = m.cfg;
     float* X = m.get<float>(S_X, (size_t)p.M * c.dim);
-    embed_and_trunk(m, s, p, xc, X);
-    cell_readout(s, p.M, c.dim, X, m.norm_g, m.norm_b, m.ro_w, m.ro_b, x0, xn, nullptr);
+    Readout ro;
+    ro.g = m.norm_g;
+    ro.b = m.norm_b;
+    ro.w = m.ro_w;
+    ro.bias = m.ro_b;
+    ro.x0 = x0;
+    ro.xn = xn;
+    embed_and_trunk(m, s, p, xc, X, ro);
     fill_u32(s, p.resid, p.n_req, 0u);
     TScope t(m, s, KC_PROJ, 48.0 * p.M);  // read xn and xc, write xn: three float4 per cell
     cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
@@ -73,8 +82,13 @@ void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0,
 void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y) {
     const j2k_config& c = m.cfg;
     float* X = m.get<float>(S_X, (size_t)p.M * c.dim);
-    embed_and_trunk(m, s, p, x4, X);
-    cell_readout(s, p.M, c.dim, X, m.norm_g, m.norm_b, m.ro_w, m.ro_b, nullptr, nullptr, y);
+    Readout ro;
+    ro.g = m.norm_g;
+    ro.b = m.norm_b;
+    ro.w = m.ro_w;
+    ro.bias = m.ro_b;
+    ro.y = y;
+    embed_and_trunk(m, s, p, x4, X, ro);
 }
 
 }  // namespace j2k

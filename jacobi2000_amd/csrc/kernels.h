@@ -42,6 +42,15 @@ void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, float sa, 
 // X += ff2(GELU(ff1(H))); then, if Wqkv, QKV = LN1'(X) @ Wqkv + bqkv, else
 // if n1g, Hout = LN1'(X). Activations (O, Hin, Hout, QKV) are bf16, or f32
 // with split; weights are the bf16 [N, K] copies.
+// The cell_st readout of the served forms (final LayerNorm g, b, then
+// y = LN(X) @ w [128][3] + bias): with xn set, xn = known ? x0 : x0 + y
+// (float4 per token, known in .w); with y set, y [M][3].
+struct Readout {
+    const float *g = nullptr, *b = nullptr, *w = nullptr, *bias = nullptr;
+    const float* x0 = nullptr;
+    float* xn = nullptr;
+    float* y = nullptr;
+};
 // Weights Wp, W1, W2, Wqkv: bf16 in MFMA fragment order (LayerW::*_f).
 struct RowsArgs {
     const void* O = nullptr;
@@ -56,6 +65,7 @@ struct RowsArgs {
     const float* bqkv = nullptr;
     void* QKV = nullptr;
     void* Hout = nullptr;
+    Readout ro;  // last block with PROJ only: the readout instead of storing X
 };
 bool rows_fused_ok(int dim, int ff);
 // Launch shapes of the row-panel kernel (same results in all): four waves of
@@ -67,6 +77,9 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
 // QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
 // arithmetic: the first layer's product after the embedding.
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);
+// The readout of X [M][128] alone, with the row-panel kernel's fused
+// readout arithmetic (its order fixed by the LayerNorm statistics' tree).
+void rows_readout(hipStream_t s, int M, int dim, const float* X, const Readout& ro);
 // The cell_st embedding (x4 [M] float4, we [4][128], be, mvec per request
 // through tok_req when not NULL) into X, the first block's LN1 (n1g, n1b)
 // and QKV = LN1(X) @ Wqkv + bqkv, in one launch (dim == 128).

@@ -107,4 +107,32 @@ __device__ __forceinline__ void ln_row_stats_t(const lnf4v (&acc)[MI][J], int wm
     }
 }
 
+// Row totals over the 128 columns of values held as in ln_row_stats_t, in
+// the same tree (the first pass of the statistics without the division):
+// out[row] for every row of the block. Synchronises the block twice.
+template <int BM, int MI, int J>
+__device__ __forceinline__ void row_sums_t(const lnf4v (&val)[MI][J], int wm, int wn, int lane, float (*red)[BM][4],
+                                           float* out) {
+#pragma clang fp contract(off)
+    const int r16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < MI; i++)
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const lnf4v x = val[i][j];
+            red[wn / 16 + j][wm + i * 16 + r16][g] = (x[0] + x[1]) + (x[2] + x[3]);
+        }
+    __syncthreads();
+    if ((int)threadIdx.x < BM) {
+        float tot = 0.f;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            const lnf4v q = *reinterpret_cast<const lnf4v*>(red[t][threadIdx.x]);
+            tot += (q[0] + q[1]) + (q[2] + q[3]);
+        }
+        out[threadIdx.x] = tot;
+    }
+    __syncthreads();
+}
+
 }  // namespace j2k

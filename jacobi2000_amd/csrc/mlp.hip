@@ -150,6 +150,61 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
     }
 }
 
+// The readout of the block's tokens whose final residual xm is held as in
+// the row-panel kernel (row wm + 16i + r16, columns wn + 16j + 4(lane/16) + r):
+// LN with the statistics' tree, then each of the three outputs as a row
+// total of the products h_d w[d][c] in the same tree, plus the bias. scratch:
+// 8 KB of idle LDS; ys: [3][RB] LDS.
+template <int MI, int J>
+__device__ __forceinline__ void readout_rows(const f4v (&xm)[MI][J], int wm, int wn, int lane, const Readout& ro,
+                                             int m0, int rows, void* scratch, float* mrow, float* irow,
+                                             float (*ys)[RB]) {
+#pragma clang fp contract(off)
+    float(*red)[RB][4] = reinterpret_cast<float(*)[RB][4]>(scratch);
+    const int c4 = (lane >> 4) * 4, r16 = lane & 15;
+    ln_row_stats_t<RB, MI, J>(xm, wm, wn, lane, red, mrow, irow);
+    f4v h[MI][J];
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const f4v g = *reinterpret_cast<const f4v*>(ro.g + wn + j * 16 + c4);
+        const f4v b = *reinterpret_cast<const f4v*>(ro.b + wn + j * 16 + c4);
+#pragma unroll
+        for (int i = 0; i < MI; i++) {
+            const int row = wm + i * 16 + r16;
+            const float mu = mrow[row], iv = irow[row];
+#pragma unroll
+            for (int r = 0; r < 4; r++) h[i][j][r] = (xm[i][j][r] - mu) * iv * g[r] + b[r];
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        f4v p[MI][J];
+#pragma unroll
+        for (int j = 0; j < J; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const float w = ro.w[(wn + j * 16 + c4 + r) * 3 + c];
+#pragma unroll
+                for (int i = 0; i < MI; i++) p[i][j][r] = h[i][j][r] * w;
+            }
+        row_sums_t<RB, MI, J>(p, wm, wn, lane, red, ys[c]);
+    }
+    const int t = threadIdx.x;
+    if (t < rows) {
+        const long m = (long)m0 + t;
+        const float y0 = ys[0][t] + ro.bias[0], y1 = ys[1][t] + ro.bias[1], y2 = ys[2][t] + ro.bias[2];
+        if (ro.y) {
+            ro.y[m * 3] = y0;
+            ro.y[m * 3 + 1] = y1;
+            ro.y[m * 3 + 2] = y2;
+        }
+        if (ro.xn) {
+            const float4 a = reinterpret_cast<const float4*>(ro.x0)[m];
+            reinterpret_cast<float4*>(ro.xn)[m] = a.w != 0.f ? a : make_float4(a.x + y0, a.y + y1, a.z + y2, 0.f);
+        }
+    }
+}
+
 // The block's waves form a WM x WN grid over (tokens, product columns): a
 // wave owns 64/WM tokens and 128/WN columns of every 128-column product.
 // WM = 1, WN = 4 (32 columns a wave) reads the whole 64-token panel from LDS
@@ -160,9 +215,10 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
 // The arithmetic of an output element is the same in every shape. The
 // LayerNorm row sums borrow LDS that is idle at the time: F's before the
 // MLP, H's after it.
-template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WM, int WN, int FC>
+template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WM, int WN, int FC, bool RO = false>
 __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM * WN / 2) void k_rows(int M, int ff,
                                                                                                      RowsArgs a) {
+    static_assert(!RO || (PROJ && !QKV && !LNOUT), "the readout ends the last block");
     constexpr int WAVES = WM * WN, MI = 4 / WM, J = 8 / WN, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
     constexpr int FLD = FC + 8, JF = FC / (16 * WN), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
     static_assert(JF >= 1 && FC % 32 == 0, "each wave owns at least one 16-column tile of an MLP chunk");
@@ -170,6 +226,7 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
     __shared__ __align__(16) uint16_t Fs[P * RB * FLD];
     __shared__ float mrow[RB], irow[RB];
+    __shared__ float ys[RO ? 3 : 1][RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave / WN) * 16 * MI, wnx = wave % WN;  // the wave's first token, its column slot
     const int m0 = blockIdx.x * RB, wn = wnx * 16 * J, wf = wnx * (FC / WN), rows = min(RB, M - m0);
@@ -268,8 +325,13 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
             else xm[i][j] = bload4(rX, at(i, j, RD));
 #pragma unroll
             for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc2[i][j][r] + b[r];
-            bstore4(rX, at(i, j, RD), xm[i][j]);
+            if constexpr (!RO) bstore4(rX, at(i, j, RD), xm[i][j]);
         }
+    }
+    if constexpr (RO) {
+        // every ff1 read of H is done: its LDS holds the sums
+        readout_rows<MI, J>(xm, wm, wn, lane, a.ro, m0, rows, Hs, mrow, irow, ys);
+        return;
     }
     if (!QKV && !LNOUT) return;
 
@@ -413,6 +475,23 @@ __global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* 
     }
 }
 
+// The readout alone (X [M][128] in memory), with readout_rows' arithmetic.
+__global__ __launch_bounds__(256, 2) void k_rows_readout(int M, const float* __restrict__ X, Readout ro) {
+    constexpr int J = 2;
+    __shared__ __align__(16) float scratch[8 * RB * 4];
+    __shared__ float mrow[RB], irow[RB], ys[3][RB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, wn = wave * 32, rows = min(RB, M - m0);
+    const int r16 = lane & 15, c4 = (lane >> 4) * 4;
+    const auto rX = rows_rsrc(X + (long)m0 * RD, rows * RD * 4);
+    f4v xm[4][J];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < J; j++) xm[i][j] = bload4(rX, ((i * 16 + r16) * RD + wn + j * 16 + c4) * 4);
+    readout_rows<4, J>(xm, 0, wn, lane, ro, m0, rows, scratch, mrow, irow, ys);
+}
+
 // The first layer's qkv product on its own (the embedding wrote H): the
 // row-panel kernel's last stage, QKV = H @ Wqkv + bqkv for 64 tokens a block,
 // with the same arithmetic.
@@ -464,6 +543,14 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
     if (shape < ROWS_4W || shape > ROWS_2X2) fail(J2K_ERR_INVALID, "fused row kernel: unknown launch shape");
     const int waves = shape == ROWS_8W ? 8 : 4;
     dim3 grid(cdiv(M, RB)), block(64 * waves);
+    const bool ro = a.ro.w != nullptr;
+    if (ro && !(proj && !qkv && !lnout)) fail(J2K_ERR_INVALID, "fused row kernel: the readout ends the last block");
+    if (ro) {
+        if (split) hipLaunchKernelGGL((k_rows<true, true, false, false, 1, 4, 128, true>), grid, dim3(256), 0, s, M, ff, a);
+        else hipLaunchKernelGGL((k_rows<false, true, false, false, 1, 4, 128, true>), grid, dim3(256), 0, s, M, ff, a);
+        LAUNCHCHK("rows_fused");
+        return;
+    }
 #define J2K_ROWS(S, P, Q, L)                                                                                          \
     if (split == S && proj == P && qkv == Q && lnout == L) {                                                          \
         if (shape == ROWS_4W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 1, 4, 128>), grid, block, 0, s, M, ff, a);      \
@@ -498,6 +585,15 @@ void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, 
         hipLaunchKernelGGL(k_rows_embed_qkv<false>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, (const float4*)x4, we, be,
                            mvec, tok_req, a);
     LAUNCHCHK("rows_embed_qkv");
+}
+
+void rows_readout(hipStream_t s, int M, int dim, const float* X, const Readout& ro) {
+    if (M <= 0) return;
+    if (dim != RD) fail(J2K_ERR_LIMIT, "row-panel readout needs dim == 128");
+    if (!X || !ro.g || !ro.b || !ro.w || !ro.bias || !(ro.y || (ro.xn && ro.x0)))
+        fail(J2K_ERR_INVALID, "row-panel readout: missing operand");
+    hipLaunchKernelGGL(k_rows_readout, dim3(cdiv(M, RB)), dim3(256), 0, s, M, X, ro);
+    LAUNCHCHK("rows_readout");
 }
 
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {

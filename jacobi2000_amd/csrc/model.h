@@ -165,9 +165,11 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
 // the residual stream. `ready` says what the caller has made of the first
 // block's input: X only (LN1 is computed here), X and H = LN1(X), or X and
 // the first block's QKV (the served bf16 forms). Leaves the stream
-// un-normalised (callers apply the final norm).
+// un-normalised (callers apply the final norm), unless `ro` asks for the
+// cell_st readout and the served d = 128 trunk takes it (then X's final
+// value may never be stored): returns whether it did.
 enum { TRUNK_X = 0, TRUNK_H = 1, TRUNK_QKV = 2 };
-void trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap);
+bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro = nullptr);
 
 // patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
 // add_state the result is state + forward (cpu.rs:84-88).

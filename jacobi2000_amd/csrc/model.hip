@@ -408,7 +408,7 @@ void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPl
 // qkv GEMMs run on their own and the row kernel does the MLP and LN1'.
 // Other widths use the tile GEMMs with LayerNorm epilogues or launches.
 template <class Act>
-void trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
+bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro) {
     constexpr bool split = sizeof(Act) == 4;
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
@@ -486,6 +486,7 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
                 a.Hout = Hh;
                 if (next) bytes += (double)M * d * sizeof(Act);
             }
+            if (!next && ro && fused) a.ro = *ro;
             TScope t(m, s, a.QKV ? KC_ROWS : KC_ROWS_TAIL, bytes);
             rows_fused(s, M, d, ff, split, a, rows_shape);
             continue;
@@ -506,6 +507,10 @@ void trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
             norm(next->n1g, next->n1b);
         }
     }
+    if (!ro || !rows) return false;
+    // the fused readout's arithmetic whichever launch shape ran the blocks
+    if (!fused) rows_readout(s, M, d, X, *ro);
+    return true;
 }
 
 // The trunk in the FP8 form: activations f32 in HBM, the four block
@@ -558,7 +563,7 @@ void trunk_f8(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPla
 
 }  // namespace
 
-void trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
+bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro) {
     if (ready == TRUNK_QKV && (m.cfg.precision == J2K_PREC_F32 || m.fp8()))
         fail(J2K_ERR_INVALID, "trunk: a ready QKV needs a served bf16 form");
     const int d = m.cfg.dim, ff = m.cfg.ff_inner;
@@ -599,11 +604,12 @@ void trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
                 gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID);
             }
         }
-        return;
+        return false;
     }
     if (m.fp8()) trunk_f8(m, s, M, X, ready, ap);
-    else if (m.split()) trunk_lowp<float>(m, s, M, X, ready, ap);
-    else trunk_lowp<uint16_t>(m, s, M, X, ready, ap);
+    else if (m.split()) return trunk_lowp<float>(m, s, M, X, ready, ap, ro);
+    else return trunk_lowp<uint16_t>(m, s, M, X, ready, ap, ro);
+    return false;
 }
 
 void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float* d_meta, float* d_out, bool add_state) {

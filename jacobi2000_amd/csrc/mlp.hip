@@ -88,6 +88,7 @@ __device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah,
         a[0][i] = frag(&Ah[(i * 16 + r16) * LD + kq]);
         if (SPLIT) l[0][i] = frag(&Al[(i * 16 + r16) * LD + kq]);
     }
+    __builtin_amdgcn_s_setprio(1);  // the other wave on the SIMD fills the MFMA gaps
 #pragma unroll
     for (int k4 = 0; k4 < K4; k4++) {
         const int b = k4 & 1;
@@ -105,6 +106,7 @@ __device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah,
                 if (SPLIT) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[k4][j], l[b][i], acc[i][j], 0, 0, 0);
             }
     }
+    __builtin_amdgcn_s_setprio(0);
 }
 
 template <int MI, int J>

@@ -121,12 +121,27 @@ def note(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_symbol(cls, prec):
-    """Name prefix of the kernel a timing class measures, as the profiler shows it."""
+# launch shape of the row-panel kernel (the autotuner's rows_fused pick) ->
+# the shape's template arguments: waves along tokens, along columns, MLP chunk
+ROWS_SHAPES = {1: "1, 4, 128", 2: "1, 8, 128", 3: "1, 4, 64", 4: "2, 2, 128"}
+
+
+def kernel_symbol(cls, prec, rows_pick=1):
+    """Name (a substring of it) of the kernel a timing class measures, as the
+    profiler shows it; rows_pick is the launch shape the autotuner chose."""
     split = "true" if prec == "bf16x2" else "false"
-    return {"gemm": "j2k::k_gemm_f32" if prec == "f32" else "j2k::k_tile_bf16",
+    shape = ROWS_SHAPES.get(rows_pick, ROWS_SHAPES[1])
+    return {"gemm": "j2k::k_gemm_f32" if prec == "f32" else "k_rows_embed_qkv",
             "attention": "j2k::k_attn_cell", "layernorm": "j2k::k_layernorm", "projection": "j2k::k_cell_project",
-            "rows": f"k_rows<{split}, true, true, false>", "rows_tail": "k_rows<"}[cls]
+            "rows": f"k_rows<{split}, true, true, false, {shape}, false>",
+            "rows_tail": f"k_rows<{split}, true, false, false, 1, 4, 128, true>"}[cls]
+
+
+def rows_pick_of(res):
+    """The autotuner's row-panel launch shape at the largest tuned bucket."""
+    t = res.get("autotune") or {}
+    b = t.get("buckets") or []
+    return b[-1].get("rows_fused", 1) if b else 1
 
 
 def roofline(res, prec, traffic=None):
@@ -139,7 +154,7 @@ def roofline(res, prec, traffic=None):
     per_launch_s = d["ms_total"] / max(d["launches"], 1) / 1e3
     work_per_launch = d["work"] / max(d["launches"], 1)
     achieved = work_per_launch / per_launch_s
-    out = {"kernel": kernel_symbol(dom, prec), "class": dom, "avg_launch_us": per_launch_s * 1e6,
+    out = {"kernel": kernel_symbol(dom, prec, rows_pick_of(res)), "class": dom, "avg_launch_us": per_launch_s * 1e6,
            "launches": d["launches"], "work_per_launch": work_per_launch}
     if dom == "gemm":
         peak = PEAK["mfma_f32" if prec == "f32" else "mfma_bf16"]
@@ -151,7 +166,7 @@ def roofline(res, prec, traffic=None):
     return out
 
 
-def measure_traffic(args, symbol):
+def measure_traffic(args, symbol, force=None):
     """HBM bytes per launch of `symbol` at the workload's size: two
     rocprofv3 --pmc runs of this script (FETCH_SIZE, then WRITE_SIZE, KiB per
     dispatch) on the headline form, each scaled by what the same counter reads
@@ -174,8 +189,10 @@ def measure_traffic(args, symbol):
         with tempfile.TemporaryDirectory(dir="/tmp") as td:
             cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", td, "-o", "run", "--"] + child
             try:
-                r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"), capture_output=True,
-                                   timeout=240)
+                env = dict(os.environ, TMPDIR="/tmp")
+                if force:  # the launch shape the measured run chose
+                    env["J2K_TUNE_FORCE"] = force
+                r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, timeout=240)
             except subprocess.TimeoutExpired:
                 return None, f"rocprofv3 --pmc {ctr} timed out"
             files = sorted(Path(td).rglob("*counter_collection.csv"))
@@ -711,8 +728,9 @@ def main():
     if not args.no_traffic:
         k = head["kernels"]
         dom = max(k, key=lambda n: k[n]["ms_total"])
-        note(f"traffic: two rocprofv3 --pmc runs for {kernel_symbol(dom, args.headline)}")
-        t_bytes, t_detail = measure_traffic(args, kernel_symbol(dom, args.headline))
+        sym = kernel_symbol(dom, args.headline, rows_pick_of(head))
+        note(f"traffic: two rocprofv3 --pmc runs for {sym}")
+        t_bytes, t_detail = measure_traffic(args, sym, f"rows_fused={rows_pick_of(head)}")
         traffic = t_bytes
         traffic_detail = t_detail
     summary = lambda r: {k: r[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "iterations", "resid",

@@ -161,6 +161,8 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     if (m.tune.enabled) autotune(m, (long)T * ny * nx);  // before the tables: it reuses their slots
     CellRun run;
     cell_setup(m, s, CellDims{T, ny, nx}, d_meta, run);
+    unsigned long long* vb = reinterpret_cast<unsigned long long*>(run.pack.resid + 2);
+    run.pack.viol = vb;  // the projection leaves the violation of each iterate
     const CellPack& p = run.pack;
     float* x0 = m.get<float>(S_X0, 4 * (size_t)p.M);
     float* xa = m.get<float>(S_XA, 4 * (size_t)p.M);
@@ -212,17 +214,13 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
         if (replay && cg.g[parity]) HIPCHK(hipGraphLaunch(cg.g[parity], s));
         else cell_iteration(m, s, p, x0, xc, xn);
     };
-    // Each iteration also finishes its iterate (energy violation, unpack into
-    // d_grid) before the one synchronisation that reads the residual, so a
-    // request that stops there needs no second round trip; a later
-    // iteration overwrites both.
-    unsigned long long* vb = reinterpret_cast<unsigned long long*>(p.resid + 2);
+    // Each iteration also finishes its iterate (the projection leaves its
+    // energy violation; the unpack into d_grid) before the one
+    // synchronisation that reads the residual, so a request that stops
+    // there needs no second round trip; a later iteration overwrites both.
     double v = 0;
     while (it < max_iter) {
         iteration(xa, xb);
-        fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
-        cell_violation(s, p.n_slices, p.slices, xb, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
-                       m.cfg.norm_depth, m.cfg.energy_r_max);
         cell_unpack(s, p.M, xb, d_grid);
         d2h(m.h_words, p.resid, 4 * sizeof(unsigned), s);  // the residual word, then the violation
         std::memcpy(&r, m.h_words, 4);

@@ -27,6 +27,9 @@ struct CellPack {
     double* mass_t = nullptr;
     double* eref_t = nullptr;
     unsigned* resid = nullptr;  // [n_req] float bits of max |dx|
+    // one request only: the iteration also leaves the energy violation of
+    // its result here (double bits), zeroed with the residual
+    unsigned long long* viol = nullptr;
 };
 
 // Host-side tables for a list of requests (packed in order).

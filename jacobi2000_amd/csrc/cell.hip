@@ -73,10 +73,11 @@ void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0,
     ro.x0 = x0;
     ro.xn = xn;
     embed_and_trunk(m, s, p, xc, X, ro);
-    fill_u32(s, p.resid, p.n_req, 0u);
+    if (p.viol) fill_u32(s, p.resid, 4, 0u);  // the residual word and the violation after it
+    else fill_u32(s, p.resid, p.n_req, 0u);
     TScope t(m, s, KC_PROJ, 48.0 * p.M);  // read xn and xc, write xn: three float4 per cell
     cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
-                 c.norm_depth, c.energy_r_max);
+                 c.norm_depth, c.energy_r_max, p.viol);
 }
 
 void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y) {

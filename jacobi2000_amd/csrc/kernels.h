@@ -202,10 +202,11 @@ void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g,
 void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
                   float g, const float* mean, const float* stdv, float depth);
 // Projection of every slice on its blank cells, then max|xn - xc| over blank
-// cells into resid_bits[req] (float bits, atomic max; caller zeroes it).
+// cells into resid_bits[req] (float bits, atomic max; caller zeroes it) and,
+// when viol_bits is not NULL, cell_violation of the result into it.
 void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
-                  float depth, float r_max);
+                  float depth, float r_max, unsigned long long* viol_bits = nullptr);
 // Per-slice relative energy violation, max into viol_bits[req] (double bits, atomic max).
 void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x4, const double* eref_t,
                     unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,

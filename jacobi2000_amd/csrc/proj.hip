@@ -182,8 +182,8 @@ void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0,
 template <int K>
 __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __restrict__ x,
                                                   const float4* __restrict__ p, double mass, double eref,
-                                                  unsigned* __restrict__ resid, const Phys& ph,
-                                                  double (*sh)[CPT / 64], float* shm) {
+                                                  unsigned* __restrict__ resid, unsigned long long* __restrict__ viol,
+                                                  const Phys& ph, double (*sh)[CPT / 64], float* shm) {
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
     const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
     float4 c[K];
@@ -260,6 +260,19 @@ __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __rest
             }
         }
     }
+    if (viol) {  // k_cell_violation's sum, in its order, on the final values
+        double v[1] = {0};
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            if (!in[k]) continue;
+            double h, u, w;
+            phys_huv(c[k], ph, h, u, w);
+            v[0] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
+        }
+        block_sum<1, CPT>(v, sh);
+        if (threadIdx.x == 0)
+            atomicMax(viol + S.req, (unsigned long long)__double_as_longlong(fmax((v[0] - (double)ph.r_max * eref) / eref, 0.0)));
+    }
     float r = 0.f;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -287,13 +300,13 @@ __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __rest
 __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
                                                      const float4* __restrict__ xc, const double* __restrict__ mass_t,
                                                      const double* __restrict__ eref_t, unsigned* __restrict__ resid,
-                                                     Phys ph) {
+                                                     unsigned long long* __restrict__ viol, Phys ph) {
     __shared__ double sh[5][CPT / 64];
     __shared__ float shm[CPT / 64];
     const Slice S = sl[blockIdx.x];
     float4* x = xn + S.start;
     if (S.n <= 4 * CPT) {
-        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, ph, sh, shm);
+        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, viol, ph, sh, shm);
         return;
     }
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
@@ -367,6 +380,19 @@ __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ 
             }
         }
     }
+    if (viol) {  // k_cell_violation's sum on the final values (each thread reads what it wrote)
+        double v[1] = {0};
+        for (int i = threadIdx.x; i < S.n; i += CPT) {
+            double h, u, w;
+            phys_huv(x[i], ph, h, u, w);
+            v[0] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
+        }
+        block_sum<1, CPT>(v, sh);
+        if (threadIdx.x == 0) {
+            const double e = eref_t[blockIdx.x];
+            atomicMax(viol + S.req, (unsigned long long)__double_as_longlong(fmax((v[0] - (double)ph.r_max * e) / e, 0.0)));
+        }
+    }
     // Residual over blank cells (all three channels), max is order-free.
     float r = 0.f;
     const float4* p = xc + S.start;
@@ -387,10 +413,10 @@ __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ 
 
 void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
-                  float depth, float r_max) {
+                  float depth, float r_max, unsigned long long* viol_bits) {
     if (n_slices <= 0) return;
     hipLaunchKernelGGL(k_cell_project, dim3(n_slices), dim3(CPT), 0, s, sl, (float4*)xn, (const float4*)xc, mass_t,
-                       eref_t, resid_bits, phys_of(g, mean, stdv, depth, r_max));
+                       eref_t, resid_bits, viol_bits, phys_of(g, mean, stdv, depth, r_max));
     LAUNCHCHK("cell_project");
 }
 

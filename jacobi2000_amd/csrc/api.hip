@@ -167,9 +167,8 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     float* x0 = m.get<float>(S_X0, 4 * (size_t)p.M);
     float* xa = m.get<float>(S_XA, 4 * (size_t)p.M);
     float* xb = m.get<float>(S_XB, 4 * (size_t)p.M);
-    cell_pack(s, p.M, d_grid, d_known, x0, xa);
-    cell_targets(s, p.n_slices, p.slices, x0, p.mass_t, p.eref_t, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
-                 m.cfg.norm_depth);
+    cell_pack_targets(s, p.n_slices, p.slices, p.M, d_grid, d_known, x0, xa, p.mass_t, p.eref_t, m.cfg.g,
+                      m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth);
     int it = 0;
     float r = 0.f;
     StallWatch watch;

@@ -198,6 +198,11 @@ void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* 
 // y = LN(X) @ Wr + br; xn = known ? x0 : x0 + y (either output may be null).
 void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g, const float* b, const float* wr,
                   const float* br, const float* x0, float* xn, float* y);
+// One request (its slices in grid order): cell_pack into x0 and x1 and
+// cell_targets of the result, in one launch (the same values).
+void cell_pack_targets(hipStream_t s, int n_slices, const Slice* sl, int M, const float* grid, const uint8_t* known,
+                       float* x0, float* x1, double* mass_t, double* eref_t, float g, const float* mean,
+                       const float* stdv, float depth);
 // Per-slice targets from x0: mass (mean normalized h) and energy (f64).
 void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
                   float g, const float* mean, const float* stdv, float depth);

@@ -167,6 +167,43 @@ __global__ __launch_bounds__(CPT) void k_cell_targets(const Slice* __restrict__ 
     }
 }
 
+// One request's pack and targets in one launch: block b packs slice b's
+// tokens (grid planes [3][M] and the known mask in token order) into x0 and
+// x1 and sums them as k_cell_targets does, in its order.
+__global__ __launch_bounds__(CPT) void k_cell_pack_targets(const Slice* __restrict__ sl, int M,
+                                                          const float* __restrict__ grid,
+                                                          const uint8_t* __restrict__ known, float4* __restrict__ x0,
+                                                          float4* __restrict__ x1, double* __restrict__ mass_t,
+                                                          double* __restrict__ eref_t, Phys ph) {
+    __shared__ double sh[2][CPT / 64];
+    const Slice S = sl[blockIdx.x];
+    double v[2] = {0, 0};
+    for (int i = threadIdx.x; i < S.n; i += CPT) {
+        const long m = (long)S.start + i;
+        const float4 c = make_float4(grid[m], grid[M + m], grid[2L * M + m], known[m] ? 1.0f : 0.0f);
+        x0[m] = c;
+        x1[m] = c;
+        double h, u, w;
+        phys_huv(c, ph, h, u, w);
+        v[0] += (double)c.x;
+        v[1] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
+    }
+    block_sum<2, CPT>(v, sh);
+    if (threadIdx.x == 0) {
+        mass_t[blockIdx.x] = v[0] / (double)S.n;
+        eref_t[blockIdx.x] = v[1];
+    }
+}
+
+void cell_pack_targets(hipStream_t s, int n_slices, const Slice* sl, int M, const float* grid, const uint8_t* known,
+                       float* x0, float* x1, double* mass_t, double* eref_t, float g, const float* mean,
+                       const float* stdv, float depth) {
+    if (n_slices <= 0) return;
+    hipLaunchKernelGGL(k_cell_pack_targets, dim3(n_slices), dim3(CPT), 0, s, sl, M, grid, known, (float4*)x0,
+                       (float4*)x1, mass_t, eref_t, phys_of(g, mean, stdv, depth, 1.0f));
+    LAUNCHCHK("cell_pack_targets");
+}
+
 void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
                   float g, const float* mean, const float* stdv, float depth) {
     if (n_slices <= 0) return;

@@ -9,8 +9,8 @@
 // between them. Rows are loaded
 // into registers one phase ahead of their use, so the L2 round trips overlap
 // the arithmetic. Tiles are dealt to blocks XCD by XCD (xcd_remap):
-// consecutive tiles, and so the time slices whose K/V rows they share, sit
-// in one XCD's L2.
+// consecutive tiles (a band of tile rows through every slice), and so the
+// time slices whose K/V rows they share, sit in one XCD's L2.
 //
 // Three staging modes, chosen by the autotuner (the arithmetic, and so the
 // result, is the same in all):
@@ -466,11 +466,15 @@ size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode) {
     return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + CT_T) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
 }
 
+// Tiles in the order rows of tiles (y), then slices, then x: the kernels
+// deal consecutive tiles to one XCD, so an XCD holds bands of rows through
+// all slices, and the time neighbours whose K/V rows a tile stages sit in
+// its own L2 (only the rows next to a band come from another XCD).
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles) {
     tiles.clear();
     for (size_t i = 0; i < segs.size(); i++)
-        for (int t = 0; t < segs[i].T; t += CT_T)
-            for (int y = 0; y < segs[i].ny; y += CT_Y)
+        for (int y = 0; y < segs[i].ny; y += CT_Y)
+            for (int t = 0; t < segs[i].T; t += CT_T)
                 for (int x = 0; x < segs[i].nx; x += CT_X) tiles.push_back(CellTile{(int)i, t, y, x});
 }
 

@@ -6,14 +6,16 @@
 //          X += ff2(GELU(ff1(H)))                    (always)
 //   QKV    QKV = LN1'(X) @ Wqkv' + bqkv'             (next block's norm and qkv)
 //   LNOUT  Hout = LN1'(X)                            (when the qkv GEMM runs apart)
+//   RO     the cell_st readout of X (last block)     (X itself not stored)
 //
 // With PROJ and QKV a layer is two launches: this kernel and the attention.
 // The same arithmetic, in the same order, as the tile GEMM epilogues, so
 // every combination gives bit-identical results and the autotuner may pick
 // any of them.
 //
-// A block owns 64 tokens; its four waves split every 128-column product (32
-// columns each, 2 x 4 tiles of v_mfma_f32_16x16x32_bf16, f32 accumulate).
+// A block owns 64 tokens; in the main launch shape its four waves split
+// every 128-column product (32 columns each, 2 x 4 tiles of
+// v_mfma_f32_16x16x32_bf16, f32 accumulate; the other shapes below).
 // The products run transposed: the MFMA's first operand is a 16 x 32
 // fragment of W (K-contiguous bf16 [N, K], so output columns are its rows),
 // the second a 32 x 16 fragment of the panel (tokens) from LDS, as bf16 or,
@@ -23,12 +25,12 @@
 // one token: residual, biases and results move as 16-byte vectors (raw
 // buffer operations bounded by the panel's rows, so a ragged last panel
 // needs no branches), and the next product's operand is written as 8-byte
-// LDS stores. Weights stream from L2 as 16-byte fragments, loaded a whole
-// 128 x 128 chunk ahead of their use. The ff-wide intermediate is made 128
-// columns at a time and consumed at once as the next K chunk of ff2. The
-// residual after the projection goes to X and is read back after the MLP,
-// so it holds no registers while the MLP runs. Row statistics of the
-// LayerNorms in the order every served kernel uses (lnstats.h).
+// LDS stores. Weights (in fragment order, 1 KB per wave load) stream from
+// L2 a whole chunk ahead of their use. The ff-wide intermediate is made 128
+// (or 64) columns at a time and consumed at once as the next K chunk of ff2.
+// The residual after the projection stays in registers through the MLP in
+// the main shape; the others park it in X and read it back. Row statistics
+// of the LayerNorms in the order every served kernel uses (lnstats.h).
 #include "common.h"
 #include "kernels.h"
 #include "lnstats.h"

@@ -186,10 +186,13 @@ int j2k_iterate(void* model, float* grid_io, const uint8_t* known, const float* 
 /* Device-pointer form; synchronises for the per-iteration convergence test
  * and once at the end (the violation; the grid is written on return).
  * d_grid [3,T,ny,nx], d_known [T,ny,nx] u8, d_meta [meta_dim].
- * From the second call with the same grid shape on, both forms replay each
- * iteration from a HIP graph (captured on that call; results bit-identical
- * to the eager launches) unless per-kernel timing is on or the environment
- * held J2K_GRAPHS=0 when the model was created. */
+ * From the second call with the same grid shape and the same d_grid, d_known
+ * and d_meta pointers on, each iteration (the first with the request's
+ * set-up, the unpack and the residual copy) replays from a HIP graph
+ * (captured on that call; results bit-identical to the eager launches)
+ * unless per-kernel timing is on or the environment held J2K_GRAPHS=0 when
+ * the model was created. j2k_iterate uploads into the model's own buffers,
+ * so it replays from its second call with a grid shape. */
 int j2k_iterate_dev(void* model, float* d_grid, const uint8_t* d_known, const float* d_meta, int T, int ny, int nx,
                     int max_iter, float tol, int* iters, float* resid, double* violation, void* stream);
 /* One model evaluation y = f(x) of the cell_st form: y [T*ny*nx, 3]. */

@@ -109,7 +109,8 @@ struct CellRun {
 };
 
 // The tables of one request depend on its grid only: built and uploaded on
-// the first call with a grid, reused while the next calls keep it.
+// the first call with a grid, reused while the next calls keep it. With
+// d_meta, also the request's meta vector (else the caller launches it).
 void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta, CellRun& run) {
     check_cell_dims(m, d);
     CellPack& p = run.pack;
@@ -146,7 +147,7 @@ void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta,
     p.ctiles = static_cast<CellTile*>(m.slot[S_CTILES]);
     p.tok_req = nullptr;
     p.mvec = m.get<float>(S_MVEC, m.cfg.dim);
-    meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
+    if (d_meta) meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
     double* targ = m.get<double>(S_TARG, 2 * (size_t)p.n_slices);
     p.mass_t = targ;
     p.eref_t = targ + p.n_slices;
@@ -160,68 +161,80 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     if (tol < 0) tol = m.cfg.tol;
     if (m.tune.enabled) autotune(m, (long)T * ny * nx);  // before the tables: it reuses their slots
     CellRun run;
-    cell_setup(m, s, CellDims{T, ny, nx}, d_meta, run);
+    cell_setup(m, s, CellDims{T, ny, nx}, nullptr, run);
     unsigned long long* vb = reinterpret_cast<unsigned long long*>(run.pack.resid + 2);
     run.pack.viol = vb;  // the projection leaves the violation of each iterate
     const CellPack& p = run.pack;
     float* x0 = m.get<float>(S_X0, 4 * (size_t)p.M);
     float* xa = m.get<float>(S_XA, 4 * (size_t)p.M);
     float* xb = m.get<float>(S_XB, 4 * (size_t)p.M);
-    cell_pack_targets(s, p.n_slices, p.slices, p.M, d_grid, d_known, x0, xa, p.mass_t, p.eref_t, m.cfg.g,
-                      m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth);
+    // the request's meta vector, packed start and projection targets
+    auto prologue = [&] {
+        meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
+        cell_pack_targets(s, p.n_slices, p.slices, p.M, d_grid, d_known, x0, xa, p.mass_t, p.eref_t, m.cfg.g,
+                          m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth);
+    };
     int it = 0;
     float r = 0.f;
     StallWatch watch;
-    // Replay the iteration from a graph once this grid and these buffers
+    // Replay the iterations from graphs once this grid and these buffers
     // have run eagerly (not while per-kernel timing records events).
     auto& cg = m.cell_graph;
-    const float* xa0 = xa;
-    if (cg.T != T || cg.ny != ny || cg.nx != nx || cg.x0 != x0 || cg.xa != xa || cg.xb != xb ||
-        cg.epoch != m.graph_epoch) {
+    if (cg.T != T || cg.ny != ny || cg.nx != nx || cg.x0 != x0 || cg.xa != xa || cg.xb != xb || cg.grid != d_grid ||
+        cg.known != d_known || cg.meta != d_meta || cg.epoch != m.graph_epoch) {
         m.drop_cell_graphs();
         cg.T = T, cg.ny = ny, cg.nx = nx, cg.x0 = x0, cg.xa = xa, cg.xb = xb, cg.epoch = m.graph_epoch;
+        cg.grid = d_grid, cg.known = d_known, cg.meta = d_meta;
     }
     if (cg.calls < 2) cg.calls++;
     const bool replay = !m.timer.on && !m.graphs_off && cg.calls >= 2;
-    auto iteration = [&](float* xc, float* xn) {
-        const int parity = xc == xa0 ? 0 : 1;
-        if (replay && !cg.g[parity]) {
+    // One iteration (the first also sets the request up), then the unpack of
+    // its iterate into d_grid and the copy of the residual word and the
+    // violation after it (the projection leaves both) to h_words, so a
+    // request that stops there needs no second round trip; a later
+    // iteration overwrites both.
+    auto iteration = [&](int kind, float* xc, float* xn) {
+        auto body = [&] {
+            if (kind == 0) prologue();
+            cell_iteration(m, s, p, x0, xc, xn);
+            cell_unpack(s, p.M, xn, d_grid);
+            hip_check(hipMemcpyAsync(m.h_words, p.resid, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s),
+                      "device->host copy", J2K_ERR_MAP);
+        };
+        if (replay && !cg.g[kind]) {
             hipGraph_t g = nullptr;
             const unsigned before = m.graph_epoch;
             bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
                 m.timer.capturing = true;
                 try {
-                    cell_iteration(m, s, p, x0, xc, xn);
+                    body();
                 } catch (const Error&) {
                     ok = false;
                 }
                 m.timer.capturing = false;
                 const hipError_t e = hipStreamEndCapture(s, &g);
                 ok = ok && e == hipSuccess && g &&
-                     hipGraphInstantiate(&cg.g[parity], g, nullptr, nullptr, 0) == hipSuccess;
+                     hipGraphInstantiate(&cg.g[kind], g, nullptr, nullptr, 0) == hipSuccess;
                 if (g) (void)hipGraphDestroy(g);
             }
             if (!ok || m.graph_epoch != before) {  // a capture must not reallocate
                 m.drop_cell_graphs();
                 m.graphs_off = !ok;
                 (void)hipGetLastError();
-                cell_iteration(m, s, p, x0, xc, xn);
+                body();
+                sync(s);
                 return;
             }
         }
-        if (replay && cg.g[parity]) HIPCHK(hipGraphLaunch(cg.g[parity], s));
-        else cell_iteration(m, s, p, x0, xc, xn);
+        if (replay && cg.g[kind]) HIPCHK(hipGraphLaunch(cg.g[kind], s));
+        else body();
+        sync(s);
     };
-    // Each iteration also finishes its iterate (the projection leaves its
-    // energy violation; the unpack into d_grid) before the one
-    // synchronisation that reads the residual, so a request that stops
-    // there needs no second round trip; a later iteration overwrites both.
     double v = 0;
+    const float* xa0 = xa;
     while (it < max_iter) {
-        iteration(xa, xb);
-        cell_unpack(s, p.M, xb, d_grid);
-        d2h(m.h_words, p.resid, 4 * sizeof(unsigned), s);  // the residual word, then the violation
+        iteration(it == 0 ? 0 : xa == xa0 ? 1 : 2, xa, xb);
         std::memcpy(&r, m.h_words, 4);
         std::memcpy(&v, m.h_words + 2, sizeof(double));
         std::swap(xa, xb);
@@ -232,10 +245,11 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     // growth during an eager call is adopted before any capture; after
     // one, it retires the graphs (they point into the old slots)
     if (cg.epoch != m.graph_epoch) {
-        if (cg.g[0] || cg.g[1]) m.drop_cell_graphs();
+        if (cg.g[0] || cg.g[1] || cg.g[2]) m.drop_cell_graphs();
         else cg.epoch = m.graph_epoch;
     }
     if (it == 0) {  // no iteration ran: the packed start, unpacked
+        prologue();
         fill_u32(s, reinterpret_cast<unsigned*>(vb), 2, 0u);
         cell_violation(s, p.n_slices, p.slices, xa, p.eref_t, vb, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
                        m.cfg.norm_depth, m.cfg.energy_r_max);

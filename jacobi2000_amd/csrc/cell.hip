@@ -30,8 +30,10 @@ void check_cell_dims(const Model& m, const CellDims& d) {
     if ((long)d.T * d.ny * d.nx > (1L << 28)) fail(J2K_ERR_SHAPE, "grid too large (over 2^28 cells)");
 }
 
-// The trunk over the pack, ending in the readout `ro`.
-static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X, const Readout& ro) {
+// The trunk over the pack, ending in the readout `ro`; clears zero[0 ..
+// n_zero) on the way.
+static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* X, const Readout& ro,
+                            unsigned* zero = nullptr, int n_zero = 0) {
     const int d = m.cfg.dim;
     const LayerW& L0 = m.L.at(0);
     if ((m.bf16() || m.split()) && rows_fused_ok(d, m.cfg.ff_inner) && L0.wqkv_f) {
@@ -43,6 +45,9 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         a.Wqkv = L0.wqkv_f;
         a.bqkv = L0.bqkv;
         a.QKV = m.get<char>(S_QKV, (size_t)p.M * 3 * d * (m.split() ? 4 : 2));
+        a.zero = zero;
+        a.n_zero = p.M > 0 ? n_zero : 0;
+        if (p.M <= 0 && n_zero > 0) fill_u32(s, zero, n_zero, 0u);
         {
             TScope t(m, s, KC_GEMM, 2.0 * p.M * d * 3 * d);
             rows_embed_qkv(s, p.M, d, m.split(), x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, a);
@@ -51,6 +56,7 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
             cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
         return;
     }
+    if (n_zero > 0) fill_u32(s, zero, n_zero, 0u);
     if (m.bf16()) {
         uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, nullptr, Hb);
@@ -72,9 +78,9 @@ void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0,
     ro.bias = m.ro_b;
     ro.x0 = x0;
     ro.xn = xn;
-    embed_and_trunk(m, s, p, xc, X, ro);
-    if (p.viol) fill_u32(s, p.resid, 4, 0u);  // the residual word and the violation after it
-    else fill_u32(s, p.resid, p.n_req, 0u);
+    // the residual words (and the violation after a single request's word)
+    // start at zero for the projection's maxima
+    embed_and_trunk(m, s, p, xc, X, ro, p.resid, p.viol ? 4 : p.n_req);
     TScope t(m, s, KC_PROJ, 48.0 * p.M);  // read xn and xc, write xn: three float4 per cell
     cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
                  c.norm_depth, c.energy_r_max, p.viol);

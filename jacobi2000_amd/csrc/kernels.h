@@ -66,6 +66,10 @@ struct RowsArgs {
     void* QKV = nullptr;
     void* Hout = nullptr;
     Readout ro;  // last block with PROJ only: the readout instead of storing X
+    // the embedding kernel's first block also clears these words (the
+    // residual and violation words a later projection accumulates into)
+    unsigned* zero = nullptr;
+    int n_zero = 0;
 };
 bool rows_fused_ok(int dim, int ff);
 // Launch shapes of the row-panel kernel (same results in all): four waves of

@@ -394,7 +394,8 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
 // launch (d = 128; the served forms): X = embed(x4) + bias + meta vector
 // (the per-element arithmetic of k_cell_embed_ln), LN1 with the row-panel
 // kernel's statistics, H in LDS only, QKV = H @ Wqkv + bqkv as in
-// k_rows_qkv. 64 tokens a block; four waves of 32 columns.
+// k_rows_qkv. 64 tokens a block; four waves of 32 columns. Block 0 also
+// clears a.zero[0 .. a.n_zero).
 template <bool SPLIT>
 __global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* __restrict__ x4,
                                                            const float* __restrict__ we, const float* __restrict__ be,
@@ -409,6 +410,8 @@ __global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* 
     uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
     bf8v w[4][J];
     load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
+    if (blockIdx.x == 0)
+        for (int i = tid; i < a.n_zero; i += 256) a.zero[i] = 0u;
     const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
     f4v xv[4][J];
 #pragma unroll
@@ -580,7 +583,7 @@ void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, 
                     const float* mvec, const int* tok_req, const RowsArgs& a) {
     if (M <= 0) return;
     if (dim != RD) fail(J2K_ERR_LIMIT, "row-panel embedding kernel needs dim == 128");
-    if (!a.X || !a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV)
+    if (!a.X || !a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV || (a.n_zero > 0 && !a.zero))
         fail(J2K_ERR_INVALID, "row-panel embedding kernel: missing operand");
     if (split)
         hipLaunchKernelGGL(k_rows_embed_qkv<true>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, (const float4*)x4, we, be,

@@ -119,17 +119,22 @@ struct Model {
     } cell_tables;
     int p_tiles = 0;
     Tuning tune;
-    // Replayed single-request cell_st iterations (j2k_iterate*): one graph
-    // per ping-pong parity, for the grid and buffers it was captured with.
+    // Replayed single-request cell_st iterations (j2k_iterate_dev): the
+    // first iteration with the request's set-up (meta vector, pack,
+    // targets), then one graph per ping-pong parity for the later ones;
+    // each ends with the unpack into the caller's grid and the copy of the
+    // residual and violation words to h_words. Keyed by the grid's shape,
+    // the caller's buffers and the scratch slots they were captured with.
     // graph_epoch moves whenever a scratch slot is reallocated or the tuning
     // changes, which retires them; the first call with a key runs eagerly so
     // that every slot has its size before a capture.
     struct {
         int T = -1, ny = 0, nx = 0;
         const void *x0 = nullptr, *xa = nullptr, *xb = nullptr;
+        const void *grid = nullptr, *known = nullptr, *meta = nullptr;
         unsigned epoch = 0;
         int calls = 0;
-        hipGraphExec_t g[2] = {nullptr, nullptr};
+        hipGraphExec_t g[3] = {nullptr, nullptr, nullptr};  // first, later from xa, later from xb
     } cell_graph;
     unsigned graph_epoch = 0;
     // pinned host words for the per-iteration residual and the violation

@@ -24,6 +24,7 @@ void Model::drop_cell_graphs() {
             g = nullptr;
         }
     cell_graph.T = -1;
+    cell_graph.grid = cell_graph.known = cell_graph.meta = nullptr;
     cell_graph.calls = 0;
 }
 

@@ -325,6 +325,37 @@ def test_iteration_graph_replays_match_eager_launches(model20_cell, precision):
             assert np.array_equal(y.view(np.uint32), gpu_forward(eager, x0, k).view(np.uint32)), i
 
 
+def test_device_iterations_replay_per_caller_buffers(model20_cell):
+    """The device form's graphs hold the caller's grid, mask and meta
+    pointers (the first iteration packs from them, every iteration unpacks
+    into the grid). Alternating two sets of buffers of one shape, each call
+    refilled with a new request, must agree bit for bit with a model that
+    never captures: a change of buffers runs eagerly and captures again."""
+    st, cj = model20_cell
+    eager = _model_env(st, cj, "bf16x2", J2K_GRAPHS="0")
+    graphs = _model_env(st, cj, "bf16x2")
+    dev = torch.device("cuda:0")
+    T, ny, nx = 4, 16, 16
+    bufs = [(torch.empty((3, T, ny, nx), dtype=torch.float32, device=dev),
+             torch.empty((T, ny, nx), dtype=torch.uint8, device=dev),
+             torch.empty(META.shape, dtype=torch.float32, device=dev)) for _ in range(2)]
+    for i, b in enumerate([0, 0, 0, 1, 1, 1, 0, 0]):
+        g = ck.drops_grid(T, ny, nx, 2, 70 + i)
+        k = ck.mask_of(("forward", "rewind", "reconstruction")[i % 3], T, ny, nx, seed=i)
+        x0 = O.fill_blank(g, k)
+        meta = (META * (1.0 + 0.1 * i)).astype(np.float32)
+        want = eager.iterate(x0, k, meta, max_iter=3, tol=0.0)
+        dg, dk, dm = bufs[b]
+        dg.copy_(torch.from_numpy(np.ascontiguousarray(x0)))
+        dk.copy_(torch.from_numpy(np.ascontiguousarray(k, dtype=np.uint8)))
+        dm.copy_(torch.from_numpy(meta))
+        torch.cuda.synchronize()
+        it, r, v = graphs.iterate_dev(dg.data_ptr(), dk.data_ptr(), dm.data_ptr(), T, ny, nx, 3, 0.0)
+        torch.cuda.synchronize()
+        assert (it, r, v) == want[1:], i
+        assert np.array_equal(dg.cpu().numpy().view(np.uint32), want[0].view(np.uint32)), i
+
+
 @pytest.mark.parametrize("shape", ["model20", "wide"])
 def test_fp8_form_error_vs_f64_oracle(model20_cell, shape):
     """The FP8 form (E4M3 block products, per-tensor scales, f32 accumulate)

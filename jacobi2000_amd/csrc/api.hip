@@ -170,9 +170,9 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     float* xb = m.get<float>(S_XB, 4 * (size_t)p.M);
     // the request's meta vector, packed start and projection targets
     auto prologue = [&] {
-        meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
         cell_pack_targets(s, p.n_slices, p.slices, p.M, d_grid, d_known, x0, xa, p.mass_t, p.eref_t, m.cfg.g,
-                          m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth);
+                          m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth, m.cfg.meta_dim, m.cfg.dim, d_meta,
+                          m.meta_w, m.meta_b, p.mvec);
     };
     int it = 0;
     float r = 0.f;
@@ -196,8 +196,7 @@ void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known,
     auto iteration = [&](int kind, float* xc, float* xn) {
         auto body = [&] {
             if (kind == 0) prologue();
-            cell_iteration(m, s, p, x0, xc, xn);
-            cell_unpack(s, p.M, xn, d_grid);
+            cell_iteration(m, s, p, x0, xc, xn, d_grid);  // the projection also unpacks into d_grid
             hip_check(hipMemcpyAsync(m.h_words, p.resid, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, s),
                       "device->host copy", J2K_ERR_MAP);
         };

@@ -70,7 +70,8 @@ inline AttnPlan plan_of(const CellPack& p) {
 }
 
 // One iteration: xn = P(clamp_known(x0 + f(xc))), resid[r] = max|xn - xc| over blank cells.
-void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn);
+void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn,
+                    float* grid = nullptr);
 // y = f(x) only (no clamp, no projection): y [M, 3].
 void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y);
 

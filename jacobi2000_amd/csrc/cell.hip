@@ -68,7 +68,8 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
 }
 
-void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn) {
+void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn,
+                    float* grid) {
     const j2k_config& c = m.cfg;
     float* X = m.get<float>(S_X, (size_t)p.M * c.dim);
     Readout ro;
@@ -81,9 +82,10 @@ void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0,
     // the residual words (and the violation after a single request's word)
     // start at zero for the projection's maxima
     embed_and_trunk(m, s, p, xc, X, ro, p.resid, p.viol ? 4 : p.n_req);
-    TScope t(m, s, KC_PROJ, 48.0 * p.M);  // read xn and xc, write xn: three float4 per cell
+    // read xn and xc, write xn: three float4 per cell (and three floats of grid)
+    TScope t(m, s, KC_PROJ, (grid ? 60.0 : 48.0) * p.M);
     cell_project(s, p.n_slices, p.slices, xn, xc, p.mass_t, p.eref_t, p.resid, c.g, c.norm_mean, c.norm_std,
-                 c.norm_depth, c.energy_r_max, p.viol);
+                 c.norm_depth, c.energy_r_max, p.viol, grid, p.M);
 }
 
 void cell_model(Model& m, hipStream_t s, const CellPack& p, const float* x4, float* y) {

@@ -110,6 +110,14 @@ __device__ __forceinline__ uint32_t f2bf2(float a, float b) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
+// Output o of the meta linear (cpu.rs:26-45 for one row: bias first, then
+// the inputs in order).
+__device__ __forceinline__ float meta_out(int md, int dim, const float* meta, const float* w, const float* b, int o) {
+    float acc = b[o];
+    for (int i = 0; i < md; i++) acc += meta[i] * w[i * dim + o];
+    return acc;
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 // Workgroups are dealt round-robin over the 8 XCDs (block b runs on XCD

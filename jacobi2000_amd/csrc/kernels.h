@@ -203,19 +203,24 @@ void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* 
 void cell_readout(hipStream_t s, int M, int dim, const float* X, const float* g, const float* b, const float* wr,
                   const float* br, const float* x0, float* xn, float* y);
 // One request (its slices in grid order): cell_pack into x0 and x1 and
-// cell_targets of the result, in one launch (the same values).
+// cell_targets of the result, in one launch (the same values); with meta,
+// also meta_linear into mvec.
 void cell_pack_targets(hipStream_t s, int n_slices, const Slice* sl, int M, const float* grid, const uint8_t* known,
                        float* x0, float* x1, double* mass_t, double* eref_t, float g, const float* mean,
-                       const float* stdv, float depth);
+                       const float* stdv, float depth, int meta_dim = 0, int dim = 0, const float* meta = nullptr,
+                       const float* meta_w = nullptr, const float* meta_b = nullptr, float* mvec = nullptr);
 // Per-slice targets from x0: mass (mean normalized h) and energy (f64).
 void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0, double* mass_t, double* eref_t,
                   float g, const float* mean, const float* stdv, float depth);
 // Projection of every slice on its blank cells, then max|xn - xc| over blank
 // cells into resid_bits[req] (float bits, atomic max; caller zeroes it) and,
-// when viol_bits is not NULL, cell_violation of the result into it.
+// when viol_bits is not NULL, cell_violation of the result into it. With
+// grid (a single request of M cells), the result also goes to grid [3][M]
+// as cell_unpack writes it.
 void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
-                  float depth, float r_max, unsigned long long* viol_bits = nullptr);
+                  float depth, float r_max, unsigned long long* viol_bits = nullptr, float* grid = nullptr,
+                  int M = 0);
 // Per-slice relative energy violation, max into viol_bits[req] (double bits, atomic max).
 void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x4, const double* eref_t,
                     unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,

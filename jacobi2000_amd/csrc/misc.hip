@@ -74,9 +74,7 @@ void patchify(hipStream_t s, int B, int C, int ny, int nx, int p, const float* s
 __global__ void k_meta_linear(int md, int dim, const float* meta, const float* w, const float* b, float* out) {
     int o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= dim) return;
-    float acc = b[o];
-    for (int i = 0; i < md; i++) acc += meta[i] * w[i * dim + o];
-    out[o] = acc;
+    out[o] = meta_out(md, dim, meta, w, b, o);
 }
 
 void meta_linear(hipStream_t s, int meta_dim, int dim, const float* meta, const float* w, const float* b, float* out) {

@@ -169,13 +169,23 @@ __global__ __launch_bounds__(CPT) void k_cell_targets(const Slice* __restrict__ 
 
 // One request's pack and targets in one launch: block b packs slice b's
 // tokens (grid planes [3][M] and the known mask in token order) into x0 and
-// x1 and sums them as k_cell_targets does, in its order.
+// x1 and sums them as k_cell_targets does, in its order. With ml.meta, one
+// more block (the last) makes the request's meta vector as k_meta_linear.
+struct MetaLin {
+    int md = 0, dim = 0;
+    const float *meta = nullptr, *w = nullptr, *b = nullptr;
+    float* out = nullptr;
+};
 __global__ __launch_bounds__(CPT) void k_cell_pack_targets(const Slice* __restrict__ sl, int M,
                                                           const float* __restrict__ grid,
                                                           const uint8_t* __restrict__ known, float4* __restrict__ x0,
                                                           float4* __restrict__ x1, double* __restrict__ mass_t,
-                                                          double* __restrict__ eref_t, Phys ph) {
+                                                          double* __restrict__ eref_t, Phys ph, MetaLin ml) {
     __shared__ double sh[2][CPT / 64];
+    if (ml.meta && blockIdx.x == gridDim.x - 1) {
+        for (int o = threadIdx.x; o < ml.dim; o += CPT) ml.out[o] = meta_out(ml.md, ml.dim, ml.meta, ml.w, ml.b, o);
+        return;
+    }
     const Slice S = sl[blockIdx.x];
     double v[2] = {0, 0};
     for (int i = threadIdx.x; i < S.n; i += CPT) {
@@ -197,10 +207,13 @@ __global__ __launch_bounds__(CPT) void k_cell_pack_targets(const Slice* __restri
 
 void cell_pack_targets(hipStream_t s, int n_slices, const Slice* sl, int M, const float* grid, const uint8_t* known,
                        float* x0, float* x1, double* mass_t, double* eref_t, float g, const float* mean,
-                       const float* stdv, float depth) {
-    if (n_slices <= 0) return;
-    hipLaunchKernelGGL(k_cell_pack_targets, dim3(n_slices), dim3(CPT), 0, s, sl, M, grid, known, (float4*)x0,
-                       (float4*)x1, mass_t, eref_t, phys_of(g, mean, stdv, depth, 1.0f));
+                       const float* stdv, float depth, int meta_dim, int dim, const float* meta, const float* meta_w,
+                       const float* meta_b, float* mvec) {
+    MetaLin ml;
+    if (meta) ml = MetaLin{meta_dim, dim, meta, meta_w, meta_b, mvec};
+    if (n_slices <= 0 && !meta) return;
+    hipLaunchKernelGGL(k_cell_pack_targets, dim3(std::max(n_slices, 0) + (meta ? 1 : 0)), dim3(CPT), 0, s, sl, M, grid,
+                       known, (float4*)x0, (float4*)x1, mass_t, eref_t, phys_of(g, mean, stdv, depth, 1.0f), ml);
     LAUNCHCHK("cell_pack_targets");
 }
 
@@ -220,7 +233,8 @@ template <int K>
 __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __restrict__ x,
                                                   const float4* __restrict__ p, double mass, double eref,
                                                   unsigned* __restrict__ resid, unsigned long long* __restrict__ viol,
-                                                  const Phys& ph, double (*sh)[CPT / 64], float* shm) {
+                                                  const Phys& ph, double (*sh)[CPT / 64], float* shm,
+                                                  float* __restrict__ grid, long M) {
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
     const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
     float4 c[K];
@@ -315,6 +329,12 @@ __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __rest
     for (int k = 0; k < K; k++) {
         if (!in[k]) continue;
         x[threadIdx.x + CPT * k] = c[k];
+        if (grid) {
+            const long m = (long)S.start + threadIdx.x + CPT * k;
+            grid[m] = c[k].x;
+            grid[M + m] = c[k].y;
+            grid[2 * M + m] = c[k].z;
+        }
         const float4 b = p[threadIdx.x + CPT * k];
         if (c[k].w != 0.f) continue;
         r = fmaxf(r, fmaxf(fabsf(c[k].x - b.x), fmaxf(fabsf(c[k].y - b.y), fabsf(c[k].z - b.z))));
@@ -333,17 +353,20 @@ __device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __rest
 // down to r_max*Eref*(1-1e-7); if potential alone is over it, zero blank
 // (u, v) and shrink blank h anomalies about their mean (mass unchanged).
 // Then the slice's max |xn - xc| over blank cells. Slices of up to 4096
-// cells keep their cells in registers (cell_project_regs).
+// cells keep their cells in registers (cell_project_regs). With grid, the
+// projected cells are also written to grid [3][M] (the unpack).
 __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
                                                      const float4* __restrict__ xc, const double* __restrict__ mass_t,
                                                      const double* __restrict__ eref_t, unsigned* __restrict__ resid,
-                                                     unsigned long long* __restrict__ viol, Phys ph) {
+                                                     unsigned long long* __restrict__ viol, Phys ph,
+                                                     float* __restrict__ grid, long M) {
     __shared__ double sh[5][CPT / 64];
     __shared__ float shm[CPT / 64];
     const Slice S = sl[blockIdx.x];
     float4* x = xn + S.start;
     if (S.n <= 4 * CPT) {
-        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, viol, ph, sh, shm);
+        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, viol, ph, sh, shm,
+                             grid, M);
         return;
     }
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
@@ -435,6 +458,12 @@ __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ 
     const float4* p = xc + S.start;
     for (int i = threadIdx.x; i < S.n; i += CPT) {
         const float4 a = x[i], b = p[i];
+        if (grid) {
+            const long m = (long)S.start + i;
+            grid[m] = a.x;
+            grid[M + m] = a.y;
+            grid[2 * M + m] = a.z;
+        }
         if (a.w != 0.f) continue;
         r = fmaxf(r, fmaxf(fabsf(a.x - b.x), fmaxf(fabsf(a.y - b.y), fabsf(a.z - b.z))));
     }
@@ -450,10 +479,10 @@ __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ 
 
 void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
-                  float depth, float r_max, unsigned long long* viol_bits) {
+                  float depth, float r_max, unsigned long long* viol_bits, float* grid, int M) {
     if (n_slices <= 0) return;
     hipLaunchKernelGGL(k_cell_project, dim3(n_slices), dim3(CPT), 0, s, sl, (float4*)xn, (const float4*)xc, mass_t,
-                       eref_t, resid_bits, viol_bits, phys_of(g, mean, stdv, depth, r_max));
+                       eref_t, resid_bits, viol_bits, phys_of(g, mean, stdv, depth, r_max), grid, (long)M);
     LAUNCHCHK("cell_project");
 }
 

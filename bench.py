@@ -32,7 +32,7 @@ sys.path.insert(0, str(ROOT))
 T, NY, NX = 16, 64, 64
 CELLS = T * NY * NX
 PEAK = {"mfma_f32": 157.3e12, "mfma_bf16": 2.5e15, "hbm": 8.0e12}
-CLASSES = ["gemm", "attention", "layernorm", "projection", "rows", "rows_tail"]
+CLASSES = ["gemm", "attention", "layernorm", "projection", "rows", "rows_tail", "rows_embed"]
 
 
 def workload():
@@ -133,8 +133,9 @@ def kernel_symbol(cls, prec, rows_pick=1):
     shape = ROWS_SHAPES.get(rows_pick, ROWS_SHAPES[1])
     return {"gemm": "j2k::k_gemm_f32" if prec == "f32" else "k_rows_embed_qkv",
             "attention": "j2k::k_attn_cell", "layernorm": "j2k::k_layernorm", "projection": "j2k::k_cell_project",
-            "rows": f"k_rows<{split}, true, true, false, {shape}, false>",
-            "rows_tail": f"k_rows<{split}, true, false, false, 1, 4, 128, true>"}[cls]
+            "rows": f"k_rows<{split}, true, true, false, {shape}, false, false>",
+            "rows_tail": f"k_rows<{split}, true, false, false, 1, 4, 128, true, false>",
+            "rows_embed": f"k_rows<{split}, true, true, false, 1, 4, 128, false, true>"}[cls]
 
 
 def rows_pick_of(res):

@@ -119,10 +119,11 @@ int j2k_model_autotune(void* model, long long max_tokens, char* report, size_t r
 /* Profiling hook, no reference counterpart (the reference timed passes with
  * GPU timestamp queries it did not commit): enable = 1 / 0 turns HIP-event
  * timing of the model's launches on or off and clears the counters (-1 keeps
- * the state). `out` (may be NULL) receives 18 doubles, per kernel class
+ * the state). `out` (may be NULL) receives 21 doubles, per kernel class
  * (GEMM, attention, LayerNorm, projection, row-panel kernel making the next
- * QKV, other row-panel launches): total ms, total algorithmic work (FLOPs
- * for GEMM, bytes otherwise), launches. Synchronises the model's stream. */
+ * QKV, other row-panel launches, the first block's row-panel launch made
+ * from the embedding): total ms, total algorithmic work (FLOPs for GEMM,
+ * bytes otherwise), launches. Synchronises the model's stream. */
 int j2k_model_timing(void* model, int enable, double* out);
 
 /* ---- patch2d form (the reference forward) ----------------------------- */

@@ -38,8 +38,10 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
     const LayerW& L0 = m.L.at(0);
     if ((m.bf16() || m.split()) && rows_fused_ok(d, m.cfg.ff_inner) && L0.wqkv_f) {
         // the embedding, LN1 and the first qkv product in one launch
+        // the first row-panel launch remakes the embedding rather than read X
+        const bool emb_in = rows_embed_in_trunk(m, p.M);
         RowsArgs a;
-        a.X = X;
+        a.X = emb_in ? nullptr : X;
         a.n1g = L0.n1g;
         a.n1b = L0.n1b;
         a.Wqkv = L0.wqkv_f;
@@ -52,7 +54,13 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
             TScope t(m, s, KC_GEMM, 2.0 * p.M * d * 3 * d);
             rows_embed_qkv(s, p.M, d, m.split(), x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, a);
         }
-        if (!trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p), &ro))
+        RowsEmbed e;
+        e.x4 = x4;
+        e.we = m.embed_w;
+        e.be = m.embed_b;
+        e.mvec = p.mvec;
+        e.tok_req = p.tok_req;
+        if (!trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p), &ro, emb_in ? &e : nullptr))
             cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
         return;
     }

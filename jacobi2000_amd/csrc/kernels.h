@@ -52,6 +52,13 @@ struct Readout {
     float* y = nullptr;
 };
 // Weights Wp, W1, W2, Wqkv: bf16 in MFMA fragment order (LayerW::*_f).
+// The cell_st embedding's operands (X = embed(x4) + bias + meta vector of
+// the token's request, k_cell_embed_ln's per-element arithmetic).
+struct RowsEmbed {
+    const float* x4 = nullptr;  // [M][4] token features
+    const float *we = nullptr, *be = nullptr, *mvec = nullptr;
+    const int* tok_req = nullptr;  // the token's request (its row of mvec); NULL: one request
+};
 struct RowsArgs {
     const void* O = nullptr;
     const uint16_t* Wp = nullptr;
@@ -66,6 +73,9 @@ struct RowsArgs {
     void* QKV = nullptr;
     void* Hout = nullptr;
     Readout ro;  // last block with PROJ only: the readout instead of storing X
+    // with emb.x4 (PROJ and QKV, four-wave shape): the residual entering the
+    // block is the embedding, made here instead of read from X
+    RowsEmbed emb;
     // the embedding kernel's first block also clears these words (the
     // residual and violation words a later projection accumulates into)
     unsigned* zero = nullptr;
@@ -86,7 +96,9 @@ void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);
 void rows_readout(hipStream_t s, int M, int dim, const float* X, const Readout& ro);
 // The cell_st embedding (x4 [M] float4, we [4][128], be, mvec per request
 // through tok_req when not NULL) into X, the first block's LN1 (n1g, n1b)
-// and QKV = LN1(X) @ Wqkv + bqkv, in one launch (dim == 128).
+// and QKV = LN1(X) @ Wqkv + bqkv, in one launch (dim == 128). With a.X
+// NULL the embedding is not stored (the next row-panel launch remakes it
+// through RowsArgs::emb).
 void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, const float* we, const float* be,
                     const float* mvec, const int* tok_req, const RowsArgs& a);
 

@@ -154,6 +154,41 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
     }
 }
 
+// The cell_st embedding of tile rows m0 + i*16 + r16 (i < 4), columns
+// wn + 16j + c4 .. +4 (j < J): bias, then the four features in order
+// (zero features skipped, as cpu.rs:26-45), then the request's meta vector.
+// Rows past M repeat row M - 1 (never stored).
+template <int J>
+__device__ __forceinline__ void embed_rows(f4v (&xv)[4][J], const RowsEmbed& e, int M, int m0, int wn, int r16,
+                                           int c4) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int m = m0 + i * 16 + r16, mm = m < M ? m : M - 1;
+        const float4 f = reinterpret_cast<const float4*>(e.x4)[mm];
+        const float* mv = e.mvec + (e.tok_req ? (long)e.tok_req[mm] * RD : 0);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const int d0 = wn + j * 16 + c4;
+            const f4v bv = *reinterpret_cast<const f4v*>(e.be + d0);
+            const f4v w0 = *reinterpret_cast<const f4v*>(e.we + d0);
+            const f4v w1 = *reinterpret_cast<const f4v*>(e.we + RD + d0);
+            const f4v w2 = *reinterpret_cast<const f4v*>(e.we + 2 * RD + d0);
+            const f4v w3 = *reinterpret_cast<const f4v*>(e.we + 3 * RD + d0);
+            const f4v mw = *reinterpret_cast<const f4v*>(mv + d0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                float v = bv[k];
+                if (f.x != 0.f) v += f.x * w0[k];
+                if (f.y != 0.f) v += f.y * w1[k];
+                if (f.z != 0.f) v += f.z * w2[k];
+                if (f.w != 0.f) v += f.w * w3[k];
+                v += mw[k];
+                xv[i][j][k] = v;
+            }
+        }
+    }
+}
+
 // The readout of the block's tokens whose final residual xm is held as in
 // the row-panel kernel (row wm + 16i + r16, columns wn + 16j + 4(lane/16) + r):
 // LN with the statistics' tree, then each of the three outputs as a row
@@ -219,10 +254,11 @@ __device__ __forceinline__ void readout_rows(const f4v (&xm)[MI][J], int wm, int
 // The arithmetic of an output element is the same in every shape. The
 // LayerNorm row sums borrow LDS that is idle at the time: F's before the
 // MLP, H's after it.
-template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WM, int WN, int FC, bool RO = false>
+template <bool SPLIT, bool PROJ, bool QKV, bool LNOUT, int WM, int WN, int FC, bool RO = false, bool EMB = false>
 __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM * WN / 2) void k_rows(int M, int ff,
                                                                                                      RowsArgs a) {
     static_assert(!RO || (PROJ && !QKV && !LNOUT), "the readout ends the last block");
+    static_assert(!EMB || (PROJ && QKV && WM == 1 && WN == 4), "the embedding enters the first block");
     constexpr int WAVES = WM * WN, MI = 4 / WM, J = 8 / WN, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
     constexpr int FLD = FC + 8, JF = FC / (16 * WN), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
     static_assert(JF >= 1 && FC % 32 == 0, "each wave owns at least one 16-column tile of an MLP chunk");
@@ -255,10 +291,14 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     __syncthreads();
     if (PROJ) {
         f4v xm[MI][J];
+        if constexpr (EMB) {
+            embed_rows<J>(xm, a.emb, M, m0, wn, r16, c4);
+        } else {
 #pragma unroll
-        for (int i = 0; i < MI; i++)
+            for (int i = 0; i < MI; i++)
 #pragma unroll
-            for (int j = 0; j < J; j++) xm[i][j] = bload4(rX, at(i, j, RD));
+                for (int j = 0; j < J; j++) xm[i][j] = bload4(rX, at(i, j, RD));
+        }
         f4v acc[MI][J];
         zero(acc);
         mma_panel<SPLIT, MI, J>(acc, Hh, Hl, w1, r16, kq);
@@ -412,35 +452,20 @@ __global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* 
     load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
     if (blockIdx.x == 0)
         for (int i = tid; i < a.n_zero; i += 256) a.zero[i] = 0u;
-    const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
+    // no X: a zero-sized buffer drops the stores
+    const auto rX = rows_rsrc(a.X ? a.X + (long)m0 * RD : nullptr, a.X ? rows * RD * 4 : 0);
     f4v xv[4][J];
+    RowsEmbed e;
+    e.x4 = reinterpret_cast<const float*>(x4);
+    e.we = we;
+    e.be = be;
+    e.mvec = mvec;
+    e.tok_req = tok_req;
+    embed_rows<J>(xv, e, M, m0, wn, r16, c4);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int m = m0 + i * 16 + r16, mm = m < M ? m : M - 1;
-        const float4 f = x4[mm];
-        const float* mv = mvec + (tok_req ? (long)tok_req[mm] * RD : 0);
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int j = 0; j < J; j++) {
-            const int d0 = wn + j * 16 + c4;
-            const f4v bv = *reinterpret_cast<const f4v*>(be + d0);
-            const f4v w0 = *reinterpret_cast<const f4v*>(we + d0);
-            const f4v w1 = *reinterpret_cast<const f4v*>(we + RD + d0);
-            const f4v w2 = *reinterpret_cast<const f4v*>(we + 2 * RD + d0);
-            const f4v w3 = *reinterpret_cast<const f4v*>(we + 3 * RD + d0);
-            const f4v mw = *reinterpret_cast<const f4v*>(mv + d0);
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                float v = bv[k];
-                if (f.x != 0.f) v += f.x * w0[k];
-                if (f.y != 0.f) v += f.y * w1[k];
-                if (f.z != 0.f) v += f.z * w2[k];
-                if (f.w != 0.f) v += f.w * w3[k];
-                v += mw[k];
-                xv[i][j][k] = v;
-            }
-            bstore4(rX, ((i * 16 + r16) * RD + d0) * 4, xv[i][j]);
-        }
-    }
+        for (int j = 0; j < J; j++) bstore4(rX, ((i * 16 + r16) * RD + wn + j * 16 + c4) * 4, xv[i][j]);
     ln_row_stats_t<RB, 4, J>(xv, 0, wn, lane, reinterpret_cast<float(*)[RB][4]>(Hs), mrow, irow);
 #pragma unroll
     for (int j = 0; j < J; j++) {
@@ -558,6 +583,17 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
         LAUNCHCHK("rows_fused");
         return;
     }
+    if (a.emb.x4) {
+        if (!(proj && qkv && shape == ROWS_4W) || !a.emb.we || !a.emb.be || !a.emb.mvec)
+            fail(J2K_ERR_INVALID, "fused row kernel: the embedding enters the four-wave first block only");
+        if (split)
+            hipLaunchKernelGGL((k_rows<true, true, true, false, 1, 4, 128, false, true>), grid, dim3(256), 0, s, M, ff, a);
+        else
+            hipLaunchKernelGGL((k_rows<false, true, true, false, 1, 4, 128, false, true>), grid, dim3(256), 0, s, M, ff,
+                               a);
+        LAUNCHCHK("rows_fused");
+        return;
+    }
 #define J2K_ROWS(S, P, Q, L)                                                                                          \
     if (split == S && proj == P && qkv == Q && lnout == L) {                                                          \
         if (shape == ROWS_4W) hipLaunchKernelGGL((k_rows<S, P, Q, L, 1, 4, 128>), grid, block, 0, s, M, ff, a);      \
@@ -583,7 +619,7 @@ void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, 
                     const float* mvec, const int* tok_req, const RowsArgs& a) {
     if (M <= 0) return;
     if (dim != RD) fail(J2K_ERR_LIMIT, "row-panel embedding kernel needs dim == 128");
-    if (!a.X || !a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV || (a.n_zero > 0 && !a.zero))
+    if (!a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV || (a.n_zero > 0 && !a.zero))
         fail(J2K_ERR_INVALID, "row-panel embedding kernel: missing operand");
     if (split)
         hipLaunchKernelGGL(k_rows_embed_qkv<true>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, (const float4*)x4, we, be,

@@ -34,7 +34,8 @@ enum Slot : int {
 // Kernel classes timed with HIP events when timing is on (the bench's
 // roofline figures): work is FLOPs for GEMM, algorithmic bytes otherwise.
 // The row-panel kernel counts its activations read and written once:
-// KC_ROWS when it also makes the next block's QKV, KC_ROWS_TAIL otherwise.
+// KC_ROWS when it also makes the next block's QKV, KC_ROWS_EMB when it
+// does so for the first block from the embedding, KC_ROWS_TAIL otherwise.
 enum KClass : int {
     KC_GEMM = 0,
     KC_ATTN = 1,
@@ -42,7 +43,8 @@ enum KClass : int {
     KC_PROJ = 3,
     KC_ROWS = 4,
     KC_ROWS_TAIL = 5,
-    KC_COUNT = 6
+    KC_ROWS_EMB = 6,
+    KC_COUNT = 7
 };
 
 struct KTimer {
@@ -173,8 +175,16 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
 // un-normalised (callers apply the final norm), unless `ro` asks for the
 // cell_st readout and the served d = 128 trunk takes it (then X's final
 // value may never be stored): returns whether it did.
+// With emb (ready == TRUNK_QKV, and only where rows_embed_in_trunk says
+// so), X holds nothing yet: the first block's row-panel launch makes the
+// embedding itself and stores X after its MLP.
 enum { TRUNK_X = 0, TRUNK_H = 1, TRUNK_QKV = 2 };
-bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro = nullptr);
+bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro = nullptr,
+           const RowsEmbed* emb = nullptr);
+// Whether the trunk over M tokens runs its first block in the row-panel
+// launch that can take the embedding (served bf16 forms, d = 128, more than
+// one block, the four-wave shape chosen for M).
+bool rows_embed_in_trunk(const Model& m, int M);
 
 // patch2d forward over B grids [B, 3, ny, nx] (cpu.rs:249-352); with
 // add_state the result is state + forward (cpu.rs:84-88).

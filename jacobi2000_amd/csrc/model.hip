@@ -409,7 +409,8 @@ void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPl
 // qkv GEMMs run on their own and the row kernel does the MLP and LN1'.
 // Other widths use the tile GEMMs with LayerNorm epilogues or launches.
 template <class Act>
-bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro) {
+bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro,
+                const RowsEmbed* emb) {
     constexpr bool split = sizeof(Act) == 4;
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     Act* Hh = m.get<Act>(split ? S_H : S_HB, (size_t)M * d);
@@ -421,6 +422,8 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
     const int rows_pick = rows ? m.tune.choice(TO_ROWS, M) : 0;
     const bool fused = rows_pick >= ROWS_4W;
     const int rows_shape = fused ? rows_pick : ROWS_4W;
+    if (emb && !(ready == TRUNK_QKV && fused && rows_shape == ROWS_4W && nl > 1))
+        fail(J2K_ERR_INVALID, "trunk: the embedding enters only a four-wave fused first block");
     auto gemm = [&](int op, int N, int K, const Act* A, const uint16_t* Bt, const float* bias, Act* C, int epi,
                     const float* lg, const float* lb) {
         TScope t(m, s, KC_GEMM, g / d * N * K);
@@ -488,7 +491,13 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
                 if (next) bytes += (double)M * d * sizeof(Act);
             }
             if (!next && ro && fused) a.ro = *ro;
-            TScope t(m, s, a.QKV ? KC_ROWS : KC_ROWS_TAIL, bytes);
+            int cls = a.QKV ? KC_ROWS : KC_ROWS_TAIL;
+            if (l == 0 && emb) {  // 16 bytes of features per token instead of X
+                a.emb = *emb;
+                bytes -= (double)M * d * 4 - 16.0 * M;
+                cls = KC_ROWS_EMB;
+            }
+            TScope t(m, s, cls, bytes);
             rows_fused(s, M, d, ff, split, a, rows_shape);
             continue;
         }
@@ -564,7 +573,14 @@ void trunk_f8(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPla
 
 }  // namespace
 
-bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro) {
+bool rows_embed_in_trunk(const Model& m, int M) {
+    return (m.bf16() || m.split()) && m.cfg.dim == 128 && rows_fused_ok(m.cfg.dim, m.cfg.ff_inner) &&
+           m.cfg.layers > 1 && !m.L.empty() && m.L[0].wqkv_f && m.tune.choice(TO_ROWS, M) == ROWS_4W;
+}
+
+bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap, const Readout* ro,
+           const RowsEmbed* emb) {
+    if (emb && !(m.bf16() || m.split())) fail(J2K_ERR_INVALID, "trunk: the embedding needs a served bf16 form");
     if (ready == TRUNK_QKV && (m.cfg.precision == J2K_PREC_F32 || m.fp8()))
         fail(J2K_ERR_INVALID, "trunk: a ready QKV needs a served bf16 form");
     const int d = m.cfg.dim, ff = m.cfg.ff_inner;
@@ -608,8 +624,8 @@ bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
         return false;
     }
     if (m.fp8()) trunk_f8(m, s, M, X, ready, ap);
-    else if (m.split()) return trunk_lowp<float>(m, s, M, X, ready, ap, ro);
-    else return trunk_lowp<uint16_t>(m, s, M, X, ready, ap, ro);
+    else if (m.split()) return trunk_lowp<float>(m, s, M, X, ready, ap, ro, emb);
+    else return trunk_lowp<uint16_t>(m, s, M, X, ready, ap, ro, emb);
     return false;
 }
 

@@ -223,18 +223,20 @@ SPECS = ["gemm_bm=64,attn_kv_mode=0,rows_fused=1", "gemm_bm=128,attn_kv_mode=1,r
 @pytest.mark.parametrize("spec", SPECS)
 def test_launch_choices_do_not_change_results(model20_cell, precision, spec):
     """Every launch shape the autotuner may pick (GEMM block height, K/V
-    staging mode of the attention) gives bit-identical iterates."""
+    staging mode of the attention, row-panel shape, the first block made
+    from the embedding) gives bit-identical iterates, also on a grid whose
+    token count is no multiple of the 64-token panel (3 x 20 x 22 = 1,320)."""
     st, cj = model20_cell
     base = _model_env(st, cj, precision, J2K_AUTOTUNE="0")
     forced = _model_env(st, cj, precision, J2K_TUNE_FORCE=spec)
-    T, ny, nx = 8, 32, 32
-    g = ck.drops_grid(T, ny, nx, 3, 17)
-    k = ck.mask_of("forward", T, ny, nx)
-    x0 = O.fill_blank(g, k)
-    xa, ia, ra, _ = base.iterate(x0, k, META, max_iter=2, tol=0.0)
-    xb, ib, rb, _ = forced.iterate(x0, k, META, max_iter=2, tol=0.0)
-    assert (ia, ra) == (ib, rb)
-    assert np.array_equal(xa, xb)
+    for T, ny, nx in ((8, 32, 32), (3, 20, 22)):
+        g = ck.drops_grid(T, ny, nx, 3, 17)
+        k = ck.mask_of("forward", T, ny, nx)
+        x0 = O.fill_blank(g, k)
+        xa, ia, ra, _ = base.iterate(x0, k, META, max_iter=2, tol=0.0)
+        xb, ib, rb, _ = forced.iterate(x0, k, META, max_iter=2, tol=0.0)
+        assert (ia, ra) == (ib, rb), (T, ny, nx)
+        assert np.array_equal(xa, xb), (T, ny, nx)
 
 
 @pytest.mark.parametrize("precision", ["f32", "bf16x2"])

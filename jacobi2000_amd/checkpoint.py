@@ -216,6 +216,32 @@ def drops_grid(T: int, ny: int, nx: int, drops: int, seed: int, centred: bool = 
     return g
 
 
+def moving_drops_grid(T: int, ny: int, nx: int, drops: int, seed: int, amp_range=(0.05, 0.15),
+                      sigma_range=(2.0, 5.0), speed: float = 0.75, flow: float = 0.02) -> np.ndarray:
+    """Synthetic request whose slices differ: Gaussian drops (amplitude and
+    width drawn as in drops_grid) whose centres move `speed` cells per slice
+    in seeded directions, wrapping periodically, plus a weak smooth flow
+    u = flow sin(2 pi (y/ny + t/T)), v = flow cos(2 pi (x/nx + t/T)).
+    Normalized grid [3, T, ny, nx] (identity normalizer, depth 1)."""
+    rng = np.random.default_rng(seed)
+    f = np.float32
+    ys, xs = np.meshgrid(np.arange(ny, dtype=np.float64), np.arange(nx, dtype=np.float64), indexing="ij")
+    g = np.zeros((3, T, ny, nx), np.float32)
+    for _ in range(drops):
+        cx, cy = rng.uniform(0, nx), rng.uniform(0, ny)
+        a, s = rng.uniform(*amp_range), rng.uniform(*sigma_range)
+        th = rng.uniform(0, 2 * np.pi)
+        for t in range(T):
+            px, py = cx + speed * t * np.cos(th), cy + speed * t * np.sin(th)
+            dx = (xs - px + nx / 2.0) % nx - nx / 2.0  # nearest periodic image
+            dy = (ys - py + ny / 2.0) % ny - ny / 2.0
+            g[0, t] += (a * np.exp(-(dx * dx + dy * dy) / (2.0 * s * s))).astype(f)
+    for t in range(T):
+        g[1, t] = (flow * np.sin(2 * np.pi * (ys / ny + t / T))).astype(f)
+        g[2, t] = (flow * np.cos(2 * np.pi * (xs / nx + t / T))).astype(f)
+    return g
+
+
 def mask_of(kind: str, T: int, ny: int, nx: int, seed: int = 0, frac: float = 0.02) -> np.ndarray:
     """Known-cell mask [T, ny, nx] (uint8): forward = slice 0 known, rewind =
     last slice known, reconstruction = `frac` of the (y, x) cells drawn

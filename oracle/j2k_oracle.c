@@ -110,6 +110,20 @@ double orc_energy_violation_c(const float* x, const orc_phys* p, int T, int n, c
     return worst;
 }
 
+/* bf16 activations: when on, the trunk rounds the values the served bf16
+ * form stores as bf16 (LN outputs, q, k, v, the attention output, the GELU
+ * output) to bf16, nearest even, through f32. */
+static int g_act_bf16 = 0;
+void orc_set_act_bf16(int on) { g_act_bf16 = on; }
+static float o_bf16f(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
 #define REAL float
 #define SFX _f32
 #define SQRT sqrtf

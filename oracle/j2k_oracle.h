@@ -90,6 +90,13 @@ int orc_iterate_c_f32(const orc_cfg_c* c, const float* const* w, const orc_phys*
                       const float* meta, int max_iter, float tol, float* resid_trace);
 int orc_iterate_c_f64(const orc_cfg_c* c, const double* const* w, const orc_phys* p, double* x_io, const uint8_t* known,
                       const double* meta, int max_iter, double tol, double* resid_trace);
+/* The same, also writing every iterate to states [max_iter][3][T][ny][nx] (may be NULL). */
+int orc_iterate_c_states_f32(const orc_cfg_c* c, const float* const* w, const orc_phys* p, float* x_io,
+                             const uint8_t* known, const float* meta, int max_iter, float tol, float* resid_trace,
+                             float* states);
+int orc_iterate_c_states_f64(const orc_cfg_c* c, const double* const* w, const orc_phys* p, double* x_io,
+                             const uint8_t* known, const double* meta, int max_iter, double tol, double* resid_trace,
+                             double* states);
 /* max over slices of max(0, (E_t - r_max*Eref_t)/Eref_t), energies in f64. */
 double orc_energy_violation_c(const float* x, const orc_phys* p, int T, int n, const double* eref_t);
 
@@ -100,6 +107,9 @@ float orc_gelu_f32(float x);
 
 /* Threads used by the parallel row loops (OpenMP); 0 = library default. */
 void orc_set_threads(int n);
+/* 1: the trunk rounds what the served bf16 form stores in bf16 (LN outputs,
+ * q, k, v, the attention output, the GELU output); 0 (default): no rounding. */
+void orc_set_act_bf16(int on);
 
 /* Shallow-water solver (crates/solvers/src/shallow_water.rs), sw_oracle.c. */
 typedef struct {

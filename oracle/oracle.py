@@ -117,6 +117,10 @@ def lib() -> ctypes.CDLL:
         _lib.orc_project_energy_f64.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         _lib.orc_iterate_c_f32.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_float, ctypes.c_void_p]
         _lib.orc_iterate_c_f64.argtypes = [ctypes.c_void_p] * 6 + [ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        _lib.orc_iterate_c_states_f32.argtypes = ([ctypes.c_void_p] * 6 +
+                                                  [ctypes.c_int, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p])
+        _lib.orc_iterate_c_states_f64.argtypes = ([ctypes.c_void_p] * 6 +
+                                                  [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p])
         _lib.orc_energy_violation_c.restype = ctypes.c_double
         _lib.orc_energy_violation_c.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_void_p]
@@ -228,9 +232,12 @@ def gelu(x: float) -> float:
 
 @dataclass
 class FormC:
-    """Form C model (per-cell space-time tokens): config dict + tensors."""
+    """Form C model (per-cell space-time tokens): config dict + tensors.
+    act_bf16 rounds the trunk's stored activations to bf16 (the served bf16
+    form's storage; see orc_set_act_bf16)."""
     cfg: dict
     w: dict
+    act_bf16: bool = False
 
     @staticmethod
     def load(dir_or_bytes, config_json: str | None = None) -> "FormC":
@@ -268,7 +275,11 @@ class FormC:
         m = np.ascontiguousarray(meta, dtype=dt)
         y = np.zeros((T * ny * nx, 3), dtype=dt)
         fn = lib().orc_forward_c_f64 if f64 else lib().orc_forward_c_f32
-        fn(ctypes.byref(c), _wptrs(ws), _ptr(g), _ptr(k), _ptr(m), _ptr(y))
+        lib().orc_set_act_bf16(int(self.act_bf16))
+        try:
+            fn(ctypes.byref(c), _wptrs(ws), _ptr(g), _ptr(k), _ptr(m), _ptr(y))
+        finally:
+            lib().orc_set_act_bf16(0)
         return y
 
     def iterate(self, x0: np.ndarray, known: np.ndarray, meta: np.ndarray, max_iter: int, tol: float,
@@ -284,8 +295,36 @@ class FormC:
         tr = np.zeros(max(max_iter, 1), dtype=dt)
         p = self.phys()
         fn = lib().orc_iterate_c_f64 if f64 else lib().orc_iterate_c_f32
-        its = fn(ctypes.byref(c), _wptrs(ws), ctypes.byref(p), _ptr(x), _ptr(k), _ptr(m), max_iter, tol, _ptr(tr))
+        lib().orc_set_act_bf16(int(self.act_bf16))
+        try:
+            its = fn(ctypes.byref(c), _wptrs(ws), ctypes.byref(p), _ptr(x), _ptr(k), _ptr(m), max_iter, tol, _ptr(tr))
+        finally:
+            lib().orc_set_act_bf16(0)
         return x, its, tr[:its]
+
+    def iterate_states(self, x0: np.ndarray, known: np.ndarray, meta: np.ndarray, max_iter: int, tol: float = 0.0,
+                       f64: bool = False):
+        """Like iterate, but returns every iterate: (states [its, 3, T, ny, nx],
+        iterations, residual trace)."""
+        _, T, ny, nx = x0.shape
+        c = self.c_cfg(T, ny, nx)
+        dt = np.float64 if f64 else np.float32
+        ws = self.weights(f64)
+        x = np.ascontiguousarray(x0, dtype=dt).copy()
+        k = np.ascontiguousarray(known, dtype=np.uint8)
+        m = np.ascontiguousarray(meta, dtype=dt)
+        n = max(max_iter, 1)
+        tr = np.zeros(n, dtype=dt)
+        states = np.zeros((n,) + x0.shape, dtype=dt)
+        p = self.phys()
+        fn = lib().orc_iterate_c_states_f64 if f64 else lib().orc_iterate_c_states_f32
+        lib().orc_set_act_bf16(int(self.act_bf16))
+        try:
+            its = fn(ctypes.byref(c), _wptrs(ws), ctypes.byref(p), _ptr(x), _ptr(k), _ptr(m), max_iter, tol,
+                     _ptr(tr), _ptr(states))
+        finally:
+            lib().orc_set_act_bf16(0)
+        return states[:its], its, tr[:its]
 
 
 def fill_blank(grid: np.ndarray, known: np.ndarray) -> np.ndarray:

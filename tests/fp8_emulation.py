@@ -1,4 +1,5 @@
-"""Numpy restatement of the FP8 form's arithmetic, for its GPU parity test:
+"""Numpy restatement of the FP8 form's arithmetic, for its GPU parity test
+(and, with bf16_act, of the bf16 form's storage rounding):
 the cell_st forward of the dense f64 restatement in test_oracle.py with the
 four block products taken on E4M3 codes. Weights: per-tensor scale
 amax / 448 (f32), round-to-nearest-even codes. Inputs: the static
@@ -26,8 +27,17 @@ def weight_codes(W):
     return ck.e4m3_decode(ck.e4m3_encode((W / sc).astype(np.float32))).astype(np.float64), float(sc)
 
 
-def cell_forward(weights, cfg, grid, known, meta, fp8=True):
-    """y = f(x) [T*ny*nx, 3] of the cell_st form; weights name -> array."""
+def bf16(a):
+    """Round to bf16 through f32, nearest even."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def cell_forward(weights, cfg, grid, known, meta, fp8=True, bf16_act=False):
+    """y = f(x) [T*ny*nx, 3] of the cell_st form; weights name -> array.
+    bf16_act (with fp8 False) rounds what the served bf16 form stores as
+    bf16: the LN outputs, q, k, v, the attention output, the GELU output."""
     w = {k: np.asarray(v, np.float64) for k, v in weights.items()}
     _, T, ny, nx = grid.shape
     t, y, x = np.meshgrid(np.arange(T), np.arange(ny), np.arange(nx), indexing="ij")
@@ -66,8 +76,9 @@ def cell_forward(weights, cfg, grid, known, meta, fp8=True):
             Wq, sw = weight_codes(W)
             return (quantize(a, sa) @ Wq) * sa * sw + b
 
-        h = ln(X, p("norm1.gamma"), p("norm1.beta"))
-        q, k, v = (lin(h, n, bh1 / 448) for n in "qkv")
+        r = bf16 if bf16_act else (lambda a: a)
+        h = r(ln(X, p("norm1.gamma"), p("norm1.beta")))
+        q, k, v = (r(lin(h, n, bh1 / 448)) for n in "qkv")
         out = np.zeros_like(X)
         for hh in range(H):
             sl = slice(hh * hd, (hh + 1) * hd)
@@ -75,7 +86,7 @@ def cell_forward(weights, cfg, grid, known, meta, fp8=True):
             s = np.where(allowed, s, -np.inf)
             s = np.exp(s - s.max(1, keepdims=True))
             out[:, sl] = (s / s.sum(1, keepdims=True)) @ v[:, sl]
-        X = X + lin(out, "proj", bo / 448)
-        h = ln(X, p("norm2.gamma"), p("norm2.beta"))
-        X = X + lin(gelu(lin(h, "ff1", bh2 / 448)), "ff2", max(bz, 0.17) / 448)
+        X = X + lin(r(out), "proj", bo / 448)
+        h = r(ln(X, p("norm2.gamma"), p("norm2.beta")))
+        X = X + lin(r(gelu(lin(h, "ff1", bh2 / 448))), "ff2", max(bz, 0.17) / 448)
     return ln(X, w["norm.gamma"], w["norm.beta"]) @ w["readout.weight"] + w["readout.bias"]

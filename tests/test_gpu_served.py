@@ -1,0 +1,407 @@
+"""GPU parity of the served d = 128 path through the C ABI, with checks that
+fail when its kernels are wrong (test_oracle_teeth.py shows each planted
+mistake lands outside these tolerances):
+
+  - the raw model output y = f(x) of every row-panel launch shape in bf16x2
+    and bf16, and of the f32 form, against the f64 oracle before any
+    projection, at the 64 x 64 x 16 grid of the single-request workload,
+    with the row-panel kernel classes confirmed launched;
+  - every iteration of the fixed point with an input-sensitive checkpoint
+    (N(0, 0.1), energy bound 1.05) at 64 x 64 x 16 for all three mask kinds;
+  - the 32 x 32 x 8 workload as configured (8 iterations, every iteration
+    checked), the 64 x 64 x 16 workload to convergence, and the six-layer
+    wide model;
+  - the server at model20 shape: each result the same alone and in the mix,
+    chained rollouts on 96 x 96 grids, and a served result against the oracle.
+
+The served forms multiply bf16 weights: they are compared with the f64
+oracle on those bf16-rounded weights, and their distance to the exact-weight
+oracle is printed beside it."""
+
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+import torch
+
+import jacobi2000_amd as j
+from jacobi2000_amd import _lib
+from jacobi2000_amd import checkpoint as ck
+from oracle import oracle as O
+from tests import served_checks as S
+
+pytestmark = pytest.mark.gpu
+
+META = S.META
+# launch shapes of the d = 128 token-local work (same results in all)
+SHAPES = {"4w_embed": "rows_fused=1", "8w": "rows_fused=2", "4w_64col": "rows_fused=3", "2x2": "rows_fused=4",
+          "apart": "rows_fused=0"}
+KC_GEMM, KC_ROWS, KC_ROWS_TAIL, KC_ROWS_EMB = 0, 4, 5, 6
+
+
+@pytest.fixture(scope="module")
+def ckpts():
+    out = {}
+    for name, (st, cj) in (("model20", S.model20()), ("teeth", S.teeth())):
+        cpu = O.FormC.load(st, cj)
+        out[name] = (st, cj, cpu, S.bf16_weights(cpu))
+    return out
+
+
+def _model(st, cj, precision, force=None):
+    """A model with fixed launch shapes (the tuning switches are read at creation)."""
+    env = {"J2K_TUNE_FORCE": force} if force else {"J2K_AUTOTUNE": "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return j.Model.from_bytes(st, cj, precision=precision)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def gpu_forward(model, grid, known, timing=False):
+    """y = f(x) through j2k_cell_forward_dev; with timing, also the launch
+    count of each kernel class."""
+    _, T, ny, nx = grid.shape
+    dev = torch.device("cuda:0")
+    g = torch.from_numpy(np.ascontiguousarray(grid)).to(dev)
+    k = torch.from_numpy(np.ascontiguousarray(known, dtype=np.uint8)).to(dev)
+    m = torch.from_numpy(META).to(dev)
+    y = torch.full((T * ny * nx, 3), float("nan"), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    if timing:
+        _lib.call("j2k_model_timing", model.handle, 1, None)
+    model.cell_forward_dev(g.data_ptr(), k.data_ptr(), m.data_ptr(), T, ny, nx, y.data_ptr())
+    torch.cuda.synchronize()  # the device-wide sync covers the model's own stream
+    counts = None
+    if timing:
+        out = (ctypes.c_double * 21)()
+        _lib.call("j2k_model_timing", model.handle, 0, out)
+        counts = [int(out[3 * c + 2]) for c in range(7)]
+    return y.cpu().numpy(), counts
+
+
+# ---------------------------------------------------------------- raw output
+
+_Y = {}
+
+
+def _raw_request(name):
+    # model20: the single-request workload's grid (five still drops, slice 0
+    # known); teeth: moving drops with the reconstruction mask, so the known
+    # flag varies in space
+    if name == "model20":
+        return S.request("forward", 16, 64, 64, moving=False)
+    return S.request("reconstruction", 16, 64, 64)
+
+
+def _oracle_y(ckpts, name, which):
+    """which: "f32" (the exact weights), "bf16x2" or "bf16" (the form's own arithmetic)."""
+    key = (name, which)
+    if key not in _Y:
+        x0, k = _raw_request(name)
+        _Y[key] = S.own_oracle(ckpts[name][2], which).forward(x0, k, META, f64=True)
+    return _Y[key]
+
+
+CASES = [("f32", None)] + [(p, s) for p in ("bf16x2", "bf16") for s in SHAPES]
+
+
+@pytest.mark.parametrize("name", ["model20", "teeth"])
+@pytest.mark.parametrize("precision,shape", CASES)
+def test_raw_output_vs_f64_oracle(ckpts, name, precision, shape):
+    """The model output before the clamp and the projection, per field,
+    against the f64 oracle on the form's own weights; the row-panel classes
+    the shape uses must have launched."""
+    st, cj = ckpts[name][:2]
+    model = _model(st, cj, precision, SHAPES.get(shape))
+    x0, k = _raw_request(name)
+    print(f"case {name} {precision} {shape}", flush=True)
+    y, counts = gpu_forward(model, x0, k, timing=True)
+    assert np.isfinite(y).all()
+    own = _oracle_y(ckpts, name, precision)
+    exact = _oracle_y(ckpts, name, "f32")
+    d_own, d_exact = S.rel_var(y, own), S.rel_var(y, exact)
+    print(f"{name} {precision} {shape}: rel_var vs own-weight oracle {d_own}, vs exact {d_exact}, "
+          f"max_abs {np.abs(y - own).max(0)}, launches per class {counts}")
+    assert d_own.max() < S.Y_TOL[precision]
+    if precision == "f32":
+        assert np.abs(y - own).max() < S.F32_MAX_ABS
+        return
+    rows_launched = counts[KC_ROWS] + counts[KC_ROWS_TAIL] + counts[KC_ROWS_EMB]
+    assert rows_launched == 4, counts  # one row-panel launch per block
+    if shape == "4w_embed":
+        assert counts[KC_ROWS_EMB] == 1 and counts[KC_ROWS] == 2 and counts[KC_ROWS_TAIL] == 1, counts
+    elif shape == "apart":
+        assert counts[KC_ROWS_TAIL] == 4, counts
+    else:
+        assert counts[KC_ROWS] == 3 and counts[KC_ROWS_TAIL] == 1, counts
+
+
+# ---------------------------------------------------------------- iterations
+
+_IT = {}
+ITERS = 4
+
+
+def _oracle_states(ckpts, kind, precision):
+    """The f32 oracle for the f32 form, the f64 oracle on the form's own
+    arithmetic for the served forms."""
+    key = (kind, precision)
+    if key not in _IT:
+        x0, k = S.request(kind, 16, 64, 64)
+        f64 = precision != "f32"
+        _IT[key] = S.own_oracle(ckpts["teeth"][2], precision).iterate_states(x0, k, META, ITERS, 0.0, f64=f64)
+    return _IT[key]
+
+
+@pytest.mark.parametrize("kind", S.KINDS)
+@pytest.mark.parametrize("precision", ["f32", "bf16x2", "bf16"])
+def test_iterations_vs_oracle_64x64x16(ckpts, kind, precision):
+    """Each of four iterations (tol 0) of the input-sensitive checkpoint at
+    64 x 64 x 16 against the oracle on the form's weights: the f32 form within
+    1e-4 max abs per field of the f32 oracle, every form within its bound of
+    the iterate's increment; the energy bound holds."""
+    st, cj = ckpts["teeth"][:2]
+    model = _model(st, cj, precision, None if precision == "f32" else SHAPES["4w_embed"])
+    x0, k = S.request(kind, 16, 64, 64)
+    ref, n, tr = _oracle_states(ckpts, kind, precision)
+    assert n == ITERS
+    for it in range(1, ITERS + 1):
+        print(f"case teeth {kind} {precision} iteration {it}", flush=True)
+        xg, itg, rg, vg = model.iterate(x0, k, META, max_iter=it, tol=0.0)
+        if precision == "bf16" and itg < it:  # retired at its noise floor (stall rule)
+            print(f"  bf16 stalled after {itg} iterations (resid {rg:.2e})")
+            break
+        assert itg == it
+        ri = S.rel_inc(xg, ref[it - 1], x0)
+        ma = np.abs(xg - ref[it - 1]).reshape(3, -1).max(1)
+        rl = S.rel_l2(xg, ref[it - 1])
+        print(f"  rel_inc per field {ri} max_abs {ma} rel L2 {rl:.2e} resid gpu {rg:.3e} oracle {tr[it - 1]:.3e} "
+              f"violation {vg:.1e}")
+        if precision == "bf16":
+            assert rl < S.BF16_STATE_TOL
+        else:
+            assert ri.max() < S.INC_TOL[precision]
+        if precision == "f32":
+            assert ma.max() < S.F32_MAX_ABS
+        assert vg <= 1e-6
+
+
+# ---------------------------------------------------------------- workloads as configured
+
+@pytest.mark.parametrize("kind", S.KINDS)
+def test_32x32x8_as_configured_every_iteration(ckpts, kind):
+    """model20 (N(0, 0.02), seed 0) on 32 x 32 x 8, one centred drop h = 1 +
+    0.1 exp(-r^2/18), 8 iterations with tol 0: every iteration of the f32
+    form within 1e-4 max abs per field of the f32 oracle, every iteration of
+    bf16x2 within 1e-3 relative L2 of the f64 oracle; energy bound."""
+    st, cj, cpu, _ = ckpts["model20"]
+    # a fixed count of iterations: no retirement at the noise floor
+    cfg = json.loads(cj)
+    cfg["stall_stop"] = False
+    cj = json.dumps(cfg)
+    g = ck.drops_grid(8, 32, 32, 1, 0, centred=True)
+    k = ck.mask_of(kind, 8, 32, 32, seed=1)
+    x0 = O.fill_blank(g, k)
+    r32, n32, _ = cpu.iterate_states(x0, k, META, 8, 0.0)
+    r64, n64, _ = cpu.iterate_states(x0, k, META, 8, 0.0, f64=True)
+    assert n32 == n64 == 8
+    f32 = _model(st, cj, "f32")
+    b2 = _model(st, cj, "bf16x2", SHAPES["4w_embed"])
+    for it in range(1, 9):
+        xg, itg, _, vg = f32.iterate(x0, k, META, max_iter=it, tol=0.0)
+        xb, itb, _, vb = b2.iterate(x0, k, META, max_iter=it, tol=0.0)
+        ma = np.abs(xg - r32[it - 1]).reshape(3, -1).max(1)
+        rb = S.rel_l2(xb, r64[it - 1])
+        print(f"{kind} iteration {it}: f32 max_abs per field {ma}, bf16x2 rel L2 {rb:.2e}, "
+              f"violations {vg:.1e} {vb:.1e}")
+        assert itg == it and ma.max() < S.F32_MAX_ABS
+        assert itb == it and rb < 1e-3
+        assert vg <= 1e-6 and vb <= 1e-6
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x2", "bf16"])
+def test_64x64x16_as_configured(ckpts, precision):
+    """The single-request workload itself: model20, 64 x 64 x 16, five drops
+    (amplitude U[0.05, 0.15], sigma U[2, 5]), slice 0 known, until max|dx| <
+    1e-4 or 32 iterations. f32 matches the f32 oracle's iteration count and
+    iterate (1e-4 max abs per field); bf16x2 stays within 1e-3 relative L2 of
+    the f64 oracle; bf16's error is reported."""
+    st, cj, cpu, _ = ckpts["model20"]
+    x0, k = S.request("forward", 16, 64, 64, moving=False)
+    model = _model(st, cj, precision, None if precision == "f32" else SHAPES["4w_embed"])
+    xg, itg, rg, vg = model.iterate(x0, k, META, max_iter=32, tol=1e-4)
+    if precision == "f32":
+        xc, itc, _ = cpu.iterate(x0, k, META, 32, 1e-4)
+        ma = np.abs(xg - xc).reshape(3, -1).max(1)
+        print(f"f32: iterations {itg} (oracle {itc}), max_abs per field {ma}")
+        assert itg == itc and ma.max() < S.F32_MAX_ABS
+    xd, _, _ = cpu.iterate(x0, k, META, itg, 0.0, f64=True)
+    rel = S.rel_l2(xg, xd)
+    print(f"{precision}: iterations {itg}, resid {rg:.2e}, rel L2 vs f64 oracle {rel:.2e}, violation {vg:.1e}")
+    assert vg <= 1e-6
+    assert rel < (1e-3 if precision != "bf16" else 2e-2)
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16x2", "fp8"])
+def test_wide_six_layers(precision):
+    """The wide configuration at full depth (d = 256, 6 layers, 8 x 32
+    heads, ff 1024, 75-key stencil, seed 1) on 4 x 20 x 20: the f32 form
+    within 1e-4 max abs per field of the f32 oracle on each of two
+    iterations; bf16x2's raw output within its bound of the oracle on its
+    bf16 weights; every form's error against the exact f64 oracle reported."""
+    st, cj = ck.random_cell_st(dim=256, layers=6, heads=8, ff=1024, seed=1, radius=2)
+    cpu = O.FormC.load(st, cj)
+    model = j.Model.from_bytes(st, cj, precision=precision)
+    x0, k = S.request("forward", 4, 20, 20)
+    if precision == "bf16x2":
+        y, _ = gpu_forward(model, x0, k)
+        d = S.rel_var(y, S.bf16_weights(cpu).forward(x0, k, META, f64=True))
+        print(f"wide bf16x2 raw output rel_var vs bf16-weight oracle {d}")
+        assert d.max() < S.Y_TOL["bf16x2"]
+    ref, _, _ = cpu.iterate_states(x0, k, META, 2, 0.0)
+    exact, _, _ = cpu.iterate_states(x0, k, META, 2, 0.0, f64=True)
+    for it in (1, 2):
+        xg, itg, _, vg = model.iterate(x0, k, META, max_iter=it, tol=0.0)
+        assert itg == it and vg <= 1e-6
+        print(f"wide {precision} iteration {it}: rel L2 vs f64 oracle {S.rel_l2(xg, exact[it - 1]):.2e}, "
+              f"rel_inc {S.rel_inc(xg, exact[it - 1], x0)}")
+        if precision == "f32":
+            assert np.abs(xg - ref[it - 1]).max() < S.F32_MAX_ABS
+
+
+# ---------------------------------------------------------------- serving at model20 shape
+
+def _server(model, budget=1 << 36, max_active=64):
+    h = ctypes.c_void_p()
+    _lib.call("j2k_server_create", model.handle, budget, max_active, ctypes.byref(h))
+    return h
+
+
+def _submit(h, grid, known):
+    _, T, ny, nx = grid.shape
+    rid = ctypes.c_uint64()
+    _lib.call("j2k_server_submit", h, grid.ctypes.data_as(ctypes.c_void_p), known.ctypes.data_as(ctypes.c_void_p),
+              META.ctypes.data_as(ctypes.c_void_p), T, ny, nx, ctypes.byref(rid))
+    return rid.value
+
+
+def _result(h, rid, shape):
+    out = np.empty(shape, np.float32)
+    res = _lib.Result()
+    _lib.call("j2k_server_poll", h, rid, ctypes.byref(res), out.ctypes.data_as(ctypes.c_void_p))
+    return res, out
+
+
+MIX = [(8, 32, 32, "forward"), (16, 48, 48, "rewind"), (8, 64, 64, "reconstruction"), (16, 32, 32, "forward"),
+       (8, 96, 96, "rewind"), (16, 64, 64, "reconstruction"), (8, 48, 48, "forward")]
+
+
+@pytest.mark.parametrize("name", ["model20", "teeth"])
+@pytest.mark.parametrize("precision", ["bf16x2", "bf16"])
+def test_server_model20_shape_same_alone_and_in_the_mix(ckpts, name, precision):
+    """Requests of the mixed stream's sizes ({32, 48, 64, 96}^2 x {8, 16})
+    and all three mask kinds, packed into ragged launches of the d = 128
+    row-panel path: each result equals the request run alone (same
+    iterations, 1e-6), and with the teeth checkpoint the first request's
+    result sits within bf16x2's bound of the oracle on its weights."""
+    st, cj, cpu, cpu_bf = ckpts[name]
+    model = _model(st, cj, precision, SHAPES["4w_embed"])
+    reqs = []
+    for i, (T, ny, nx, kind) in enumerate(MIX):
+        g = ck.moving_drops_grid(T, ny, nx, 3, 200 + i) if name == "teeth" else ck.drops_grid(T, ny, nx, 3, 200 + i)
+        reqs.append((g, ck.mask_of(kind, T, ny, nx, seed=i)))
+    alone = [model.iterate(j.fill_blank(g, k), k, META) for g, k in reqs]
+    h = _server(model)
+    try:
+        ids = []
+        for i, (g, k) in enumerate(reqs):
+            ids.append(_submit(h, g, k))
+            if i == 3:
+                time.sleep(0.02)  # later requests arrive while the first ones run
+        _lib.call("j2k_server_drain", h)
+        for (g, k), rid, (xa, ita, ra, va) in zip(reqs, ids, alone):
+            res, out = _result(h, rid, g.shape)
+            d = np.abs(out - xa).max()
+            print(f"{name} {precision} request {rid} {g.shape}: iters mix={res.iters} alone={ita} "
+                  f"max_abs={d:.2e} violation={res.violation:.1e}")
+            assert res.iters == ita and d <= 1e-6 and res.violation <= 1e-6
+        peak = ctypes.c_int()
+        _lib.call("j2k_server_stats", h, None, None, ctypes.byref(peak), None)
+        assert peak.value >= 2
+        if name == "teeth" and precision == "bf16x2":
+            g, k = reqs[0]
+            x0 = O.fill_blank(g, k)
+            res, out = _result(h, ids[0], g.shape)
+            ref, _, _ = cpu_bf.iterate(x0, k, META, res.iters, 0.0, f64=True)
+            ri = S.rel_inc(out, ref, x0)
+            print(f"served result vs oracle on bf16 weights after {res.iters} iterations: rel_inc {ri}")
+            assert ri.max() < S.INC_TOL["bf16x2"]
+    finally:
+        _lib.call("j2k_server_destroy", h)
+
+
+def _windows_alone(model, state, window, ticks):
+    _, ny, nx = state.shape
+    traj = np.zeros((3, ticks, ny, nx), np.float32)
+    traj[:, 0] = state
+    anchor, t0, iters = state, 0, []
+    while t0 < ticks - 1:
+        T = min(window, ticks - t0)
+        g = np.ascontiguousarray(np.repeat(anchor[:, None], T, axis=1))
+        k = np.zeros((T, ny, nx), np.uint8)
+        k[0] = 1
+        x, it, _, v = model.iterate(g, k, META)
+        assert v <= 1e-6
+        traj[:, t0 + 1:t0 + T] = x[:, 1:T]
+        anchor, t0 = x[:, T - 1], t0 + window - 1
+        iters.append(it)
+    return traj, iters
+
+
+@pytest.mark.parametrize("name", ["model20", "teeth"])
+def test_server_rollouts_96x96_model20_shape(ckpts, name):
+    """Chained rollouts of the long-rollout workload's shape (96 x 96 grids,
+    8-slice windows, here to 22 ticks: three windows), three at once with
+    plain requests in the mix, bf16x2: each trajectory equals its windows
+    run one by one, every window within the energy bound."""
+    st, cj = ckpts[name][:2]
+    model = _model(st, cj, "bf16x2", SHAPES["4w_embed"])
+    ny = nx = 96
+    W, ticks = 8, 22
+    states = []
+    for i in range(3):
+        g = ck.moving_drops_grid(1, ny, nx, 3, 300 + i) if name == "teeth" else ck.drops_grid(1, ny, nx, 3, 300 + i)
+        states.append(np.ascontiguousarray(g[:, 0]))
+    alone = [_windows_alone(model, s, W, ticks) for s in states]
+    h = _server(model)
+    try:
+        ids = []
+        for i, s in enumerate(states):
+            rid = ctypes.c_uint64()
+            _lib.call("j2k_server_submit_rollout", h, s.ctypes.data_as(ctypes.c_void_p),
+                      META.ctypes.data_as(ctypes.c_void_p), ny, nx, W, ticks, ctypes.byref(rid))
+            ids.append(rid.value)
+            T, my, mx, kind = MIX[i]
+            _submit(h, ck.drops_grid(T, my, mx, 2, 400 + i), ck.mask_of(kind, T, my, mx, seed=i))
+        _lib.call("j2k_server_drain", h)
+        for rid, (traj, iters) in zip(ids, alone):
+            res, out = _result(h, rid, (3, ticks, ny, nx))
+            n = ctypes.c_int()
+            lat, vio, its = (ctypes.c_double * 8)(), (ctypes.c_double * 8)(), (ctypes.c_int * 8)()
+            _lib.call("j2k_server_windows", h, rid, 8, ctypes.byref(n), lat, vio, its)
+            d = np.abs(out - traj).max()
+            print(f"{name} rollout {rid}: windows {n.value} iters {list(its)[:n.value]} alone {iters} max_abs {d:.2e}")
+            assert n.value == 3 and list(its)[:3] == iters
+            assert d <= 1e-6
+            assert all(vio[w] <= 1e-6 for w in range(3))
+    finally:
+        _lib.call("j2k_server_destroy", h)

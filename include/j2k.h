@@ -199,6 +199,20 @@ int j2k_iterate_dev(void* model, float* d_grid, const uint8_t* d_known, const fl
 /* One model evaluation y = f(x) of the cell_st form: y [T*ny*nx, 3]. */
 int j2k_cell_forward_dev(void* model, const float* d_grid, const uint8_t* d_known, const float* d_meta, int T, int ny,
                          int nx, float* d_y, void* stream);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * evaluation as j2k_cell_forward_dev, bit for bit, with every scratch buffer
+ * (tables, packed features, residual stream, activations) carved from a
+ * fixed device region of J2K_NOALLOC_MAX_BYTES that the library holds on each
+ * device it runs on until the process ends; it calls no hipMalloc or
+ * hipFree. A request whose scratch does not fit is refused with
+ * J2K_ERR_OOM before any device work (about 43,000 tokens in BF16X2 and
+ * 26,000 in F32 at d = 128, ff 512). Every call on one device shares the
+ * region, so calls that overlap on one device race; the call uploads the
+ * grid's tables with synchronous copies (not for use under stream capture). */
+#define J2K_NOALLOC_MAX_BYTES ((size_t)128 << 20)
+int j2k_cell_forward_dev_noalloc(void* model, const float* d_grid, const uint8_t* d_known, const float* d_meta, int T,
+                                 int ny, int nx, float* d_y, void* stream);
 
 /* ---- serving (continuous batching on one GPU) -------------------------- */
 

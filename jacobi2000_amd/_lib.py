@@ -93,6 +93,7 @@ PROTOS = {
     "j2k_iterate": [P, P, P, P, I, I, I, I, F, ctypes.POINTER(I), ctypes.POINTER(F), ctypes.POINTER(D)],
     "j2k_iterate_dev": [P, P, P, P, I, I, I, I, F, ctypes.POINTER(I), ctypes.POINTER(F), ctypes.POINTER(D), P],
     "j2k_cell_forward_dev": [P, P, P, P, I, I, I, P, P],
+    "j2k_cell_forward_dev_noalloc": [P, P, P, P, I, I, I, P, P],
     "j2k_server_create": [P, ctypes.c_size_t, I, PP],
     "j2k_server_submit": [P, P, P, P, I, I, I, ctypes.POINTER(ctypes.c_uint64)],
     "j2k_server_submit_rollout": [P, P, P, I, I, I, I, ctypes.POINTER(ctypes.c_uint64)],

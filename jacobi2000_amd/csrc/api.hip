@@ -103,57 +103,6 @@ double violation_of(const std::vector<double>& e, float r_max) {
 
 // ---- cell_st single request ---------------------------------------------------
 
-struct CellRun {
-    CellTables tab;
-    CellPack pack;
-};
-
-// The tables of one request depend on its grid only: built and uploaded on
-// the first call with a grid, reused while the next calls keep it. With
-// d_meta, also the request's meta vector (else the caller launches it).
-void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta, CellRun& run) {
-    check_cell_dims(m, d);
-    CellPack& p = run.pack;
-    auto& c = m.cell_tables;
-    m.p_tables_batch = -1;  // the patch2d tables share these slots
-    if (c.T != d.T || c.ny != d.ny || c.nx != d.nx) {
-        c.T = -1;
-        run.tab.build({d}, attn_tokens_per_block(m.cfg.heads), false);
-        sync(s);
-        Seg* segs = m.get<Seg>(S_SEGS, run.tab.segs.size());
-        Tile* tiles = m.get<Tile>(S_TILES, run.tab.tiles.size());
-        Slice* slices = m.get<Slice>(S_SLICES, run.tab.slices.size());
-        CellTile* ctiles = m.get<CellTile>(S_CTILES, run.tab.ctiles.size());
-        HIPCHK(hipMemcpy(ctiles, run.tab.ctiles.data(), run.tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(segs, run.tab.segs.data(), run.tab.segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(tiles, run.tab.tiles.data(), run.tab.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(slices, run.tab.slices.data(), run.tab.slices.size() * sizeof(Slice), hipMemcpyHostToDevice));
-        c.M = run.tab.M;
-        c.n_slices = (int)run.tab.slices.size();
-        c.n_tiles = (int)run.tab.tiles.size();
-        c.n_ctiles = (int)run.tab.ctiles.size();
-        c.T = d.T;
-        c.ny = d.ny;
-        c.nx = d.nx;
-    }
-    p.M = c.M;
-    p.n_req = 1;
-    p.n_slices = c.n_slices;
-    p.n_tiles = c.n_tiles;
-    p.n_ctiles = c.n_ctiles;
-    p.segs = static_cast<Seg*>(m.slot[S_SEGS]);
-    p.tiles = static_cast<Tile*>(m.slot[S_TILES]);
-    p.slices = static_cast<Slice*>(m.slot[S_SLICES]);
-    p.ctiles = static_cast<CellTile*>(m.slot[S_CTILES]);
-    p.tok_req = nullptr;
-    p.mvec = m.get<float>(S_MVEC, m.cfg.dim);
-    if (d_meta) meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
-    double* targ = m.get<double>(S_TARG, 2 * (size_t)p.n_slices);
-    p.mass_t = targ;
-    p.eref_t = targ + p.n_slices;
-    p.resid = m.get<unsigned>(S_RESID, 4);
-}
-
 void iterate_dev(Model& m, hipStream_t s, float* d_grid, const uint8_t* d_known, const float* d_meta, int T, int ny,
                  int nx, int max_iter, float tol, int* iters, float* resid, double* violation) {
     need_form(m, J2K_FORM_CELL_ST, "j2k_iterate");

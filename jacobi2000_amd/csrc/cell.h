@@ -45,6 +45,18 @@ struct CellTables {
 
 void check_cell_dims(const Model& m, const CellDims& d);
 
+// One request's tables and packed-pack view.
+struct CellRun {
+    CellTables tab;
+    CellPack pack;
+};
+
+// The tables of one request depend on its grid only: built and uploaded on
+// the first call with a grid (synchronising the stream), reused while the
+// next calls keep it (m.cell_tables.T = -1 forces a rebuild). With d_meta,
+// also the request's meta vector.
+void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta, CellRun& run);
+
 // Stall rule: when the residual fails to drop below 0.99x the best value
 // seen so far on two consecutive iterations, the iteration has reached the
 // noise floor of its arithmetic; it is retired there (reported as stalled,

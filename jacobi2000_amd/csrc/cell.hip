@@ -1,6 +1,8 @@
 // The cell_st fixed-point iteration over a packed set of requests.
 #include "cell.h"
 
+#include <string>
+
 namespace j2k {
 
 void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req) {
@@ -74,6 +76,52 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
     }
     if (!trunk(m, s, p.M, X, TRUNK_H, plan_of(p), &ro))
         cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
+}
+
+// The tables of one request depend on its grid only: built and uploaded on
+// the first call with a grid, reused while the next calls keep it. With
+// d_meta, also the request's meta vector (else the caller launches it).
+void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta, CellRun& run) {
+    check_cell_dims(m, d);
+    CellPack& p = run.pack;
+    auto& c = m.cell_tables;
+    m.p_tables_batch = -1;  // the patch2d tables share these slots
+    if (c.T != d.T || c.ny != d.ny || c.nx != d.nx) {
+        c.T = -1;
+        run.tab.build({d}, attn_tokens_per_block(m.cfg.heads), false);
+        HIPCHK(hipStreamSynchronize(s));
+        Seg* segs = m.get<Seg>(S_SEGS, run.tab.segs.size());
+        Tile* tiles = m.get<Tile>(S_TILES, run.tab.tiles.size());
+        Slice* slices = m.get<Slice>(S_SLICES, run.tab.slices.size());
+        CellTile* ctiles = m.get<CellTile>(S_CTILES, run.tab.ctiles.size());
+        HIPCHK(hipMemcpy(ctiles, run.tab.ctiles.data(), run.tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(segs, run.tab.segs.data(), run.tab.segs.size() * sizeof(Seg), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(tiles, run.tab.tiles.data(), run.tab.tiles.size() * sizeof(Tile), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(slices, run.tab.slices.data(), run.tab.slices.size() * sizeof(Slice), hipMemcpyHostToDevice));
+        c.M = run.tab.M;
+        c.n_slices = (int)run.tab.slices.size();
+        c.n_tiles = (int)run.tab.tiles.size();
+        c.n_ctiles = (int)run.tab.ctiles.size();
+        c.T = d.T;
+        c.ny = d.ny;
+        c.nx = d.nx;
+    }
+    p.M = c.M;
+    p.n_req = 1;
+    p.n_slices = c.n_slices;
+    p.n_tiles = c.n_tiles;
+    p.n_ctiles = c.n_ctiles;
+    p.segs = static_cast<Seg*>(m.current(S_SEGS));
+    p.tiles = static_cast<Tile*>(m.current(S_TILES));
+    p.slices = static_cast<Slice*>(m.current(S_SLICES));
+    p.ctiles = static_cast<CellTile*>(m.current(S_CTILES));
+    p.tok_req = nullptr;
+    p.mvec = m.get<float>(S_MVEC, m.cfg.dim);
+    if (d_meta) meta_linear(s, m.cfg.meta_dim, m.cfg.dim, d_meta, m.meta_w, m.meta_b, p.mvec);
+    double* targ = m.get<double>(S_TARG, 2 * (size_t)p.n_slices);
+    p.mass_t = targ;
+    p.eref_t = targ + p.n_slices;
+    p.resid = m.get<unsigned>(S_RESID, 4);
 }
 
 void cell_iteration(Model& m, hipStream_t s, const CellPack& p, const float* x0, const float* xc, float* xn,

@@ -145,8 +145,22 @@ struct Model {
     bool graphs_off = false;  // J2K_GRAPHS=0 at creation, or a capture failed: run eagerly
     void drop_cell_graphs();
 
+    // A fixed region that scratch() carves every slot from while `on`,
+    // instead of allocating (the allocation-free entry point): 256-byte
+    // aligned, first come first served, J2K_ERR_OOM once it is exhausted.
+    // The slots of the arena are separate from the model's own.
+    struct {
+        bool on = false;
+        char* base = nullptr;
+        size_t size = 0, used = 0;
+        void* slot[S_COUNT] = {};
+        size_t cap[S_COUNT] = {};
+    } arena;
+
     ~Model();
     void* scratch(int s, size_t bytes);
+    // The slot's current device buffer (the arena's while it is on).
+    void* current(int s) const { return arena.on ? arena.slot[s] : slot[s]; }
     template <class T>
     T* get(int s, size_t count) {
         return static_cast<T*>(scratch(s, count * sizeof(T)));
@@ -167,6 +181,11 @@ struct DeviceGuard {
 };
 
 Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision);
+
+// Looks up, once per device and with that device current, the device address
+// of the allocation-free entry point's fixed scratch region (a failure is
+// kept and reported by that entry point).
+void noalloc_init(int device);
 
 // The transformer trunk (cpu.rs:287-313) over M packed tokens: X [M, dim] is
 // the residual stream. `ready` says what the caller has made of the first

@@ -44,6 +44,17 @@ Model::~Model() {
 
 void* Model::scratch(int s, size_t bytes) {
     if (bytes == 0) bytes = 16;
+    if (arena.on) {  // carve from the caller's fixed region; never allocate
+        if (arena.cap[s] >= bytes) return arena.slot[s];
+        const size_t want = (bytes + 255) & ~(size_t)255;
+        if (want > arena.size - arena.used)
+            fail(J2K_ERR_OOM, "allocation-free scratch exhausted: " + std::to_string(arena.used) + " of " +
+                                  std::to_string(arena.size) + " bytes in use, " + std::to_string(want) + " more asked");
+        arena.slot[s] = arena.base + arena.used;
+        arena.cap[s] = want;
+        arena.used += want;
+        return arena.slot[s];
+    }
     if (cap[s] >= bytes) return slot[s];
     graph_epoch++;  // captured graphs may point into the old slot
     if (slot[s]) {
@@ -268,7 +279,10 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
     m->device = device;
     hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "hipStreamCreate", J2K_ERR_NO_DEVICE);
     hip_check(hipHostMalloc((void**)&m->h_words, 4 * sizeof(unsigned)), "hipHostMalloc", J2K_ERR_OOM);
-    if (cfg.form == J2K_FORM_CELL_ST) attn_cell_prepare();
+    if (cfg.form == J2K_FORM_CELL_ST) {
+        attn_cell_prepare();
+        noalloc_init(device);  // the allocation-free entry point's region on this device
+    }
     Uploader up{*m};
     const int d = cfg.dim, ff = cfg.ff_inner;
     if (cfg.dim / cfg.heads != 8 && cfg.dim / cfg.heads != 16 && cfg.dim / cfg.heads != 32 && cfg.dim / cfg.heads != 64)
@@ -654,8 +668,8 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
         m.p_tiles = (int)tiles.size();
     }
     AttnPlan ap;
-    ap.segs = static_cast<const Seg*>(m.slot[S_SEGS]);
-    ap.tiles = static_cast<const Tile*>(m.slot[S_TILES]);
+    ap.segs = static_cast<const Seg*>(m.current(S_SEGS));
+    ap.tiles = static_cast<const Tile*>(m.current(S_TILES));
     ap.n_tiles = m.p_tiles;
     trunk(m, s, M, X, TRUNK_X, ap);
 

@@ -234,10 +234,26 @@ Weights read_safetensors(const uint8_t* bytes, size_t n) {
         const Json* off = v.find("data_offsets");
         if (!shp || shp->kind != Json::Arr || !off || off->kind != Json::Arr || off->arr.size() != 2)
             fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` lacks shape/data_offsets");
+        // Offsets and dimensions must be non-negative integers inside the
+        // buffer (weights.rs reads them as u64 and bounds-checks the slice);
+        // every product and sum below is checked before it can wrap.
+        const size_t avail = n - data_start;
+        auto count = [&](const Json& j, const char* what) -> uint64_t {
+            if (j.kind != Json::Num || !(j.num >= 0.0) || j.num != std::floor(j.num) || j.num > 9007199254740992.0)
+                fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` has a " + what + " that is not a non-negative integer");
+            return (uint64_t)j.num;
+        };
         Tensor t;
-        for (auto& d : shp->arr) t.shape.push_back((int64_t)d.num);
-        size_t a = (size_t)off->arr[0].num, b = (size_t)off->arr[1].num;
-        if (b < a || data_start + b > n || (b - a) != (size_t)t.numel() * esize)
+        uint64_t elems = 1;
+        for (auto& d : shp->arr) {
+            const uint64_t dim = count(d, "dimension");
+            if (dim != 0 && elems > (avail / esize) / dim)
+                fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` is larger than the buffer");
+            elems *= dim;
+            t.shape.push_back((int64_t)dim);
+        }
+        const uint64_t a = count(off->arr[0], "data offset"), b = count(off->arr[1], "data offset");
+        if (a > b || b > avail || b - a != elems * esize)
             fail(J2K_ERR_SHAPE, "safetensors: tensor `" + name + "` data range does not match its shape");
         const uint8_t* src = bytes + data_start + a;
         t.data.resize((size_t)t.numel());

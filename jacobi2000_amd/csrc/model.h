@@ -24,7 +24,7 @@ struct LayerW {
     float s_h1, s_o, s_h2, s_g;
 };
 
-// Scratch slots, each grown on demand (the stream is drained before a regrow).
+// Scratch slots, each grown on demand (the device is drained before a regrow).
 enum Slot : int {
     S_X, S_H, S_HB, S_QKV, S_O, S_F, S_XN, S_T0, S_T1, S_Z0, S_Z2, S_C1, S_PATCH, S_MVEC,
     S_WIN_A, S_WIN_B, S_SUMS, S_SEGS, S_TILES, S_SLICES, S_X0, S_XA, S_XB, S_TARG, S_RESID, S_GRID, S_KNOWN, S_META,

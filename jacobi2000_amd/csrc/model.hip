@@ -58,7 +58,9 @@ void* Model::scratch(int s, size_t bytes) {
     if (cap[s] >= bytes) return slot[s];
     graph_epoch++;  // captured graphs may point into the old slot
     if (slot[s]) {
-        HIPCHK(hipStreamSynchronize(stream));
+        // work on any stream (the model's, or a caller's passed to a _dev
+        // entry point) may still read the old buffer
+        HIPCHK(hipDeviceSynchronize());
         HIPCHK(hipFree(slot[s]));
         slot[s] = nullptr;
         cap[s] = 0;

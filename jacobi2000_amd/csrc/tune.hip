@@ -228,7 +228,10 @@ void autotune(Model& m, long max_tokens) {
     if (top < B_MIN) top = B_MIN;
     if (top <= m.tune.tuned_to) return;
     DeviceGuard g(m.device);
-    HIPCHK(hipStreamSynchronize(m.stream));
+    // The tuner rewrites the model's scratch and tables on its own stream:
+    // first let every stream of the device finish, a caller's stream that
+    // may still be reading them included.
+    HIPCHK(hipDeviceSynchronize());
     const bool timing = m.timer.on;
     m.timer.on = false;
     Events ev;

@@ -1,15 +1,18 @@
 // `rollout`: the reference CLI (crates/engine-cli/src/main.rs) over libj2k.
 //
-//   rollout --model <checkpoint-dir> [--steps N] [--backend auto|hip]
+//   rollout --model <checkpoint-dir> [--steps N] [--backend auto|webgpu|hip]
 //           [--case drop|dam_break] [--out field.npy] [--precision f32|bf16]
 //
 // Steps the shallow-water solver beside the model from the same initial
 // condition and prints the model's correlation against it every step
 // (main.rs:99-142). The whole loop runs on the GPU (j2k_rollout_truth):
 // the solver, the model step with its mass projection and the correlation,
-// with one copy in and one out; the lines print when it returns. The CPU
-// engine is the reference's own, so `--backend cpu` is refused with exit
-// code 2.
+// with one copy in and one out; the lines print when it returns. The
+// reference's backends are auto | cpu | webgpu (crates/engine-cli/src/
+// backend.rs:32-66): here `webgpu` (the reference's GPU engine) and `hip`
+// pick the HIP engine, `auto` takes it too, and a missing GPU exits 1 as the
+// reference does when no adapter is granted. The CPU engine is the
+// reference's own, so `--backend cpu` is refused with exit code 2.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -30,7 +33,7 @@ struct Args {
 
 [[noreturn]] void usage() {
     std::fprintf(stderr,
-                 "usage: rollout --model <checkpoint-dir> [--steps N] [--backend auto|hip] [--case drop|dam_break] "
+                 "usage: rollout --model <checkpoint-dir> [--steps N] [--backend auto|webgpu|hip] [--case drop|dam_break] "
                  "[--out field.npy] [--precision f32|bf16]\n");
     std::exit(2);
 }
@@ -97,8 +100,9 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "--backend cpu: the CPU engine is the reference's own; this binary runs the HIP engine\n");
         return 2;
     }
-    if (a.backend != "auto" && a.backend != "hip") {
-        std::fprintf(stderr, "unrecognized --backend `%s` (want auto | hip)\n", a.backend.c_str());
+    if (a.backend != "auto" && a.backend != "webgpu" && a.backend != "hip") {
+        std::fprintf(stderr, "unrecognized --backend `%s` (want auto | webgpu | hip; cpu is the reference's own)\n",
+                     a.backend.c_str());
         return 2;
     }
     const int prec = a.precision == "bf16" ? J2K_PREC_BF16 : J2K_PREC_F32;

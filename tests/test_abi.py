@@ -106,3 +106,30 @@ def test_null_arguments_are_invalid():
     rc = j.lib().j2k_forward(None, None, None, None)
     assert rc == _lib.ERR_INVALID
     assert "NULL" in j.lib().j2k_last_error().decode()
+
+
+def _raw_safetensors(entry: dict, data_bytes: int = 4096) -> bytes:
+    import json
+    import struct
+
+    h = json.dumps({"embed.weight": entry}).encode()
+    h += b" " * ((8 - len(h) % 8) % 8)
+    return struct.pack("<Q", len(h)) + h + bytes(data_bytes)
+
+
+@pytest.mark.parametrize("entry", [
+    # offsets near 2^64: the end offset wraps when added to the data start
+    {"dtype": "F32", "shape": [512], "data_offsets": [2 ** 64 - 4096, 2 ** 64 - 2048]},
+    {"dtype": "F32", "shape": [512], "data_offsets": [-8, 2040]},       # negative offset
+    {"dtype": "F32", "shape": [512], "data_offsets": [0.5, 2048.5]},    # fractional offsets
+    {"dtype": "F32", "shape": [-1, -512], "data_offsets": [0, 2048]},   # negative dimensions
+    {"dtype": "F32", "shape": [2 ** 40, 2 ** 40], "data_offsets": [0, 0]},  # element count overflows
+    {"dtype": "F32", "shape": [1024], "data_offsets": [2048, 6144]},    # past the end of the buffer
+])
+def test_malformed_offsets_and_shapes_are_rejected(tiny_bytes, entry):
+    """The reader rejects offsets or dimensions that are not non-negative
+    integers inside the buffer (weights.rs reads them as u64 and bounds-checks
+    the slice), without reading outside it."""
+    rc, msg = _create(_raw_safetensors(entry), tiny_bytes[1])
+    print(rc, msg)
+    assert rc == _lib.ERR_SHAPE and "embed.weight" in msg

@@ -22,8 +22,24 @@ def test_usage_without_model_exits_2():
 
 def test_unknown_flag_and_case_exit_2(fixture_dir):
     assert run("--model", str(fixture_dir), "--bogus").returncode == 2
-    assert run("--model", str(fixture_dir), "--backend", "webgpu").returncode == 2
     assert run("--model", str(fixture_dir), "--backend", "cpu").returncode == 2
+    r = run("--model", str(fixture_dir), "--backend", "opengl")
+    assert r.returncode == 2 and "unrecognized --backend" in r.stderr
+
+
+def test_webgpu_backend_name_is_the_gpu_engine(fixture_dir):
+    """The reference's GPU backend name selects the HIP engine: it is not
+    refused as unknown (without a GPU it exits 1, as the reference does when
+    no adapter is granted)."""
+    r = run("--model", str(fixture_dir), "--backend", "webgpu", "--steps", "1")
+    assert r.returncode != 2 and "unrecognized" not in r.stderr
+
+
+@pytest.mark.gpu
+def test_webgpu_backend_runs_on_the_gpu(fixture_dir):
+    r = run("--model", str(fixture_dir), "--backend", "webgpu", "--steps", "3")
+    assert r.returncode == 0, r.stderr
+    assert len([l for l in r.stdout.splitlines() if l.startswith("step")]) == 3
 
 
 @pytest.mark.gpu

@@ -405,3 +405,33 @@ def test_server_rollouts_96x96_model20_shape(ckpts, name):
             assert all(vio[w] <= 1e-6 for w in range(3))
     finally:
         _lib.call("j2k_server_destroy", h)
+
+
+def test_first_use_tuning_is_ordered_after_the_callers_stream(ckpts):
+    """A model that tunes on first use: a forward on a caller's stream, then
+    at once an iteration on that stream (which tunes first, rewriting the
+    scratch and tables the forward reads). Both results equal those of a
+    model that never tunes, bit for bit."""
+    st, cj = ckpts["teeth"][:2]
+    tuned = j.Model.from_bytes(st, cj, precision="bf16x2")  # tunes on its first iteration
+    plain = _model(st, cj, "bf16x2")
+    x0, k = S.request("forward", 16, 64, 64)
+    dev = torch.device("cuda:0")
+    side = torch.cuda.Stream()
+    outs = []
+    for model in (tuned, plain):
+        g = torch.from_numpy(np.ascontiguousarray(x0)).to(dev)
+        kd = torch.from_numpy(np.ascontiguousarray(k, dtype=np.uint8)).to(dev)
+        md = torch.from_numpy(META).to(dev)
+        y = torch.empty((x0[0].size, 3), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        model.cell_forward_dev(g.data_ptr(), kd.data_ptr(), md.data_ptr(), *x0.shape[1:], y.data_ptr(),
+                               stream=side.cuda_stream)
+        it, r, v = model.iterate_dev(g.data_ptr(), kd.data_ptr(), md.data_ptr(), *x0.shape[1:], 3, 0.0,
+                                     stream=side.cuda_stream)
+        side.synchronize()
+        outs.append((y.cpu().numpy(), g.cpu().numpy(), it, r))
+    assert tuned.autotune(0)["tuned_to_tokens"] >= 1 << 16
+    (ya, ga, ia, ra), (yb, gb, ib, rb) = outs
+    assert np.array_equal(ya.view(np.uint32), yb.view(np.uint32))
+    assert (ia, ra) == (ib, rb) and np.array_equal(ga.view(np.uint32), gb.view(np.uint32))

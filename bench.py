@@ -116,6 +116,58 @@ def run_precision(prec, st, cj, x0, known, meta, steps, warmup):
     }, out
 
 
+def run_iterating(meta, steps, warmup, headline, no_error):
+    """The single-request workload with an input-sensitive checkpoint of the
+    same shape (N(0, 0.1) from seed 5, energy bound 1.05), on moving drops
+    (the slices differ): the fixed point takes several iterations, so the
+    stall rule, graph replay across iterations and convergence are timed.
+    Per form: ms per request, iterations, converged cells/s; error against
+    the f64 oracle on exact weights and on the form's bf16 weights."""
+    from jacobi2000_amd import checkpoint as ck
+    import jacobi2000_amd as j
+
+    st, cj = ck.random_cell_st(seed=5, std=0.1, r_max=1.05)
+    g = ck.moving_drops_grid(T, NY, NX, 5, 11)
+    known = ck.mask_of("forward", T, NY, NX)
+    x0 = j.fill_blank(g, known)
+    out = {"workload": "64x64 x 16, model20 shape with N(0,0.1) weights (seed 5), energy_r_max 1.05, five moving "
+                       "drops, slice 0 known, tol 1e-4, cap 32"}
+    res = {}
+    for prec in ("f32", "bf16", "bf16x2"):
+        r, grid = run_precision(prec, st, cj, x0, known, meta, steps, warmup)
+        res[prec] = (r, grid)
+        out[prec] = {k: r[k] for k in ("value", "ms_per_step", "p50_ms", "p99_ms", "resid", "energy_violation")}
+        out[prec]["iterations"] = r["iterations"][-1]
+    out["served_vs_plain"] = res[headline][0]["value"] / res["f32"][0]["value"]
+    if not no_error:
+        from oracle import oracle as O
+
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        cpu = O.FormC.load(st, cj)
+        wq = {}
+        for name, a in cpu.w.items():
+            parts = name.split(".")
+            q = len(parts) == 4 and parts[2] in ("q", "k", "v", "proj", "ff1", "ff2") and parts[3] == "weight"
+            wq[name] = ((ck.f32_to_bf16_bits(a).astype(np.uint32) << 16).view(np.float32) if q else a)
+        cpu_bf = O.FormC(cpu.cfg, wq)
+        cache = {}
+        for prec, (r, grid) in res.items():
+            its = r["iterations"][-1]
+            x0d = x0.astype(np.float64)
+            for label, oracle in (("exact_weights", cpu), ("bf16_weights", cpu_bf)):
+                if label == "bf16_weights" and prec == "f32":
+                    continue
+                if (label, its) not in cache:
+                    cache[(label, its)] = oracle.iterate(x0, known, meta, its, 0.0, f64=True)[0]
+                ref = cache[(label, its)]
+                d = np.linalg.norm((grid - ref).reshape(3, -1), axis=1)
+                inc = np.linalg.norm((ref - x0d).reshape(3, -1), axis=1)
+                out[prec][f"error_vs_f64_oracle_{label}"] = {
+                    "rel_l2": float(np.linalg.norm(grid - ref) / np.linalg.norm(ref)),
+                    "rel_to_increment_per_field": [float(a / max(b, 1e-30)) for a, b in zip(d, inc)]}
+    return out
+
+
 def note(msg):
     """Progress line on stderr (the JSON line alone goes to stdout)."""
     print(f"bench: {msg}", file=sys.stderr, flush=True)
@@ -234,6 +286,113 @@ def cpu_baseline(st, cj, x0, known, meta, iters, seconds):
     return {"value": CELLS * done / el, "unit": "converged grid-cells/s", "cores": threads, "kind": "port",
             "sample": f"{done} request(s) of 64x64x16 cells, {iters} iteration(s) each, CPU oracle f32 "
                       f"(restatement of engine::cpu extended to the cell_st form), {el:.1f} s"}
+
+
+# The reference's published figures for its own forward (form P, model20):
+# crates/engine GpuModel::step on an Apple M2 (docs/runs/2026-10-12-engine-
+# webgpu-perf.md:29-39) and the rollout CLI on a Linux desktop with an RTX
+# 3080 (docs/runs/2026-10-07-inference-lean.md:157-160).
+REF_FORM_P_MS = {"webgpu_m2_step": 7.97, "webgpu_vulkan_rollout_step": 17.40, "cpu_rollout_step": 108.87}
+FORM_P_GFLOP = 0.578  # per forward at model20 shape, 64x64 grid, patch 4
+
+
+def _timed(fn, seconds, at_least=1):
+    done, t0 = 0, time.perf_counter()
+    while done < at_least or time.perf_counter() - t0 < seconds:
+        fn()
+        done += 1
+    return (time.perf_counter() - t0) / done, done
+
+
+def run_form_p(steps, warmup, cpu_seconds):
+    """The reference's own forward (form P) at model20 shape: 64x64 grid,
+    patch 4 (256 tokens), d 128, 4 layers, 4 heads x 32, MLP 512, 3x3 token
+    stencil, random init N(0, 0.02) seed 0; the rollout CLI's `drop` case
+    (main.rs:63-78). Times j2k_step with its host copies (state in, next
+    state out: the reference's GpuModel::step also uploads and reads back)
+    and the device rollout loop with the mass projection (main.rs:117-121),
+    per step; the CPU oracle's forward on all host threads and on one."""
+    import ctypes
+
+    import torch
+    import jacobi2000_amd as j
+    from jacobi2000_amd import _lib
+    from jacobi2000_amd import checkpoint as ck
+
+    st, cj = ck.random_patch2d(seed=0)
+    n = 64 * 64
+    h = np.full((64, 64), 1.0, np.float32)
+    ck.add_drop(h, 32.0, 32.0, 0.05, 4.0)
+    x = np.concatenate([(h - np.float32(1.0)).reshape(-1), np.zeros(2 * n, np.float32)])
+    meta = ck.meta_of(1.0, 0.09, 9.81, 1.0, 1e-3)
+    out = {"workload": "reference forward (patch2d) at model20 shape: 64x64 grid, patch 4, d128 L4 H4x32 ff512, "
+                       "3x3 token stencil, drop case",
+           "reference_ms_per_step": REF_FORM_P_MS, "gflop_per_forward": FORM_P_GFLOP}
+    dev = torch.device("cuda:0")
+    for prec in ("f32", "bf16x2"):
+        model = j.Model.from_bytes(st, cj, precision=prec)
+        for _ in range(warmup):
+            model.step(x, meta)
+        step_s, _ = _timed(lambda: model.step(x, meta), 0.0, at_least=max(steps, 10))
+        d_x = torch.from_numpy(x).to(dev)
+        d_meta = torch.from_numpy(meta).to(dev)
+        loop = 50
+        torch.cuda.synchronize()
+
+        def rollout():
+            _lib.call("j2k_rollout_dev", model.handle, ctypes.c_void_p(d_x.data_ptr()),
+                      ctypes.c_void_p(d_meta.data_ptr()), loop, 1, 0, None, None)
+            torch.cuda.synchronize()
+
+        rollout()
+        roll_s, reps = _timed(rollout, 0.0, at_least=3)
+        model.close()
+        out[prec] = {"step_ms": step_s * 1e3, "rollout_ms_per_step": roll_s * 1e3 / loop,
+                     "rollout_steps": loop * reps, "gflops_per_s": FORM_P_GFLOP / (roll_s / loop),
+                     "vs_webgpu_m2": REF_FORM_P_MS["webgpu_m2_step"] / (step_s * 1e3),
+                     "vs_vulkan_rollout": REF_FORM_P_MS["webgpu_vulkan_rollout_step"] / (roll_s * 1e3 / loop)}
+    if cpu_seconds > 0:
+        from oracle import oracle as O
+
+        cpu = O.FormP.load(st, cj)
+        legs = []
+        for threads in (int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), 1):
+            O.set_threads(threads)
+            t, done = _timed(lambda: cpu.forward(x, meta), cpu_seconds, at_least=2)
+            legs.append({"cores": threads, "forward_ms": t * 1e3, "forwards": done})
+        O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        out["cpu_oracle"] = {"kind": "port", "what": "CPU restatement of engine::cpu (cpu.rs:249-352), f32",
+                             "legs": legs}
+    return out
+
+
+def cpu_legs(st, cj, meta, x0, known):
+    """The CPU oracle (f32 restatement of engine::cpu extended to the cell_st
+    form) on one iteration of the 32x32 x 8 request (centred drop) and of the
+    64x64 x 16 request, on all host threads and on one: ms per iteration and
+    cell-iterations/s."""
+    from jacobi2000_amd import checkpoint as ck
+    from oracle import oracle as O
+
+    cpu = O.FormC.load(st, cj)
+    g1 = ck.drops_grid(8, 32, 32, 1, 0, centred=True)
+    k1 = ck.mask_of("forward", 8, 32, 32)
+    from oracle.oracle import fill_blank
+
+    reqs = {"32x32x8": (fill_blank(g1, k1), k1), "64x64x16": (x0, known)}
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"kind": "port", "what": "CPU restatement of engine::cpu extended to the cell_st form, f32, one iteration"}
+    for name, (x, k) in reqs.items():
+        cells = x[0].size
+        legs = []
+        for threads in (all_threads, 1):
+            O.set_threads(threads)
+            t, done = _timed(lambda: cpu.iterate(x, k, meta, 1, 0.0), 1.0 if threads > 1 else 0.0, at_least=1)
+            legs.append({"cores": threads, "ms_per_iteration": t * 1e3, "cell_iterations_per_s": cells / t,
+                         "iterations_timed": done})
+        out[name] = legs
+    O.set_threads(all_threads)
+    return out
 
 
 def run_solver(dataset_states, seconds_cpu):
@@ -687,6 +846,12 @@ def main():
     ap.add_argument("--scores", type=int, default=64, metavar="TRAJ",
                     help="also time the device scores on this many 64x64 trajectories and the excess band on 16x "
                          "as many grids (0: skip)")
+    ap.add_argument("--iterating", type=int, default=10, metavar="STEPS",
+                    help="timed requests per form of the iterating workload (input-sensitive checkpoint; 0: skip)")
+    ap.add_argument("--form-p", type=int, default=20, metavar="STEPS",
+                    help="timed steps per form of the reference forward at model20 shape (0: skip)")
+    ap.add_argument("--no-cpu-legs", dest="cpu_legs", action="store_false",
+                    help="skip the CPU oracle legs on 32x32x8 and 64x64x16 at all threads and one")
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 --pmc runs that measure the dominant kernel's HBM bytes")
     args = ap.parse_args()
@@ -772,10 +937,18 @@ def main():
         s_served = run_stream(st, cj, meta, reqs, args.rate, served=True,
                               precision=args.headline if args.headline != "f32" else "bf16x2")
         s_plain = run_stream(st, cj, meta, reqs, args.rate, served=False)
+        note(f"stream at saturation: {args.stream} requests all at t = 0, served then plain")
+        z_served = run_stream(st, cj, meta, reqs, float("inf"), served=True,
+                              precision=args.headline if args.headline != "f32" else "bf16x2")
+        z_plain = run_stream(st, cj, meta, reqs, float("inf"), served=False)
         line["stream"] = {"workload": f"{args.stream} requests, Poisson arrivals at {args.rate:g}/s, sizes "
                                       "{32,48,64,96}^2 x {8,16}, masks forward/rewind/reconstruction",
                           "served": s_served, "plain_fp32": s_plain,
-                          "served_vs_plain": s_served["value"] / s_plain["value"]}
+                          "served_vs_plain": s_served["value"] / s_plain["value"],
+                          "saturated": {"workload": "the same requests, all submitted at t = 0 (closed loop: the "
+                                                    "server's capacity)",
+                                        "served": z_served, "plain_fp32": z_plain,
+                                        "served_vs_plain": z_served["value"] / z_plain["value"]}}
     if args.wide > 0:
         note(f"wide shape: 128x128x16, f32 / bf16x2 / fp8, {args.wide} timed requests each, errors on 32x32x16")
         line["wide"] = run_wide(meta, args.wide)
@@ -789,6 +962,15 @@ def main():
                             "unit": "grid-cells/s over the whole trajectories",
                             "served": r_served, "plain_fp32": r_plain,
                             "served_vs_plain": r_served["value"] / r_plain["value"]}
+    if args.iterating > 0:
+        note(f"iterating workload: input-sensitive checkpoint, {args.iterating} timed requests per form")
+        line["iterating"] = run_iterating(meta, args.iterating, args.warmup, args.headline, args.no_oracle_error)
+    if args.form_p > 0:
+        note(f"form P: reference forward at model20 shape, {args.form_p} timed steps per form")
+        line["form_p"] = run_form_p(args.form_p, args.warmup, 0.0 if args.no_cpu_baseline else 2.0)
+    if not args.no_cpu_baseline and args.cpu_legs:
+        note("cpu legs: the oracle on 32x32x8 and 64x64x16, all host threads and one")
+        line["cpu_baseline_legs"] = cpu_legs(st, cj, meta, x0, known)
     if args.solver > 0:
         note(f"solver: {args.solver} dataset states, truth partner, 4096x4096 grid")
         line["solver"] = run_solver(args.solver, 0.0 if args.no_cpu_baseline else 3.0)

@@ -207,6 +207,12 @@ struct Slice {
 // writes `copy` when it is not NULL.
 void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4, float* copy = nullptr);
 void cell_unpack(hipStream_t s, int M, const float* x4, float* grid);
+// The request index of every token of a packing: tok_req[t] = r for t in segment r.
+void seg_tokens(hipStream_t s, int n_segs, const Seg* segs, int* tok_req);
+// A chained rollout window of T slices of n cells from the packed anchor
+// slice anchor4 [n] float4: every slice copies its (h, u, v), slice 0 known;
+// written to x0 and x1 (the anchor must not overlap them).
+void cell_window(hipStream_t s, int n, int T, const float* anchor4, float* x0, float* x1);
 // X = embed([h,u,v,known]) + mvec (mvec per request), H = LN1_0(X).
 void cell_embed_ln(hipStream_t s, int M, int dim, const float* x4, const float* we, const float* be,
                    const float* mvec, const int* tok_req, const float* g, const float* b, float* X, float* H,

@@ -167,6 +167,39 @@ __global__ void k_cell_unpack(int M, const float4* __restrict__ x4, float* __res
     g[2 * (long)M + m] = v.z;
 }
 
+// A chained rollout's next window on the device: every slice of M = T * n
+// tokens starts as the anchor slice's (h, u, v), slice 0 known, the others
+// blank (the values a forward request's blank fill gives from that slice).
+__global__ void k_cell_window(int n, int M, const float4* __restrict__ anchor, float4* __restrict__ x0,
+                              float4* __restrict__ x1) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    float4 v = anchor[m % n];
+    v.w = m < n ? 1.0f : 0.0f;
+    x0[m] = v;
+    x1[m] = v;
+}
+
+void cell_window(hipStream_t s, int n, int T, const float* anchor4, float* x0, float* x1) {
+    const int M = n * T;
+    hipLaunchKernelGGL(k_cell_window, dim3(cdiv(M, 256)), dim3(256), 0, s, n, M, (const float4*)anchor4,
+                       (float4*)x0, (float4*)x1);
+    LAUNCHCHK("cell_window");
+}
+
+// tok_req[t] = r for every token t of segment r: one block per segment.
+__global__ void k_seg_tokens(const Seg* __restrict__ segs, int* __restrict__ tok_req) {
+    const Seg g = segs[blockIdx.x];
+    const int n = g.T * g.ny * g.nx;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tok_req[g.base + i] = blockIdx.x;
+}
+
+void seg_tokens(hipStream_t s, int n_segs, const Seg* segs, int* tok_req) {
+    if (n_segs <= 0) return;
+    hipLaunchKernelGGL(k_seg_tokens, dim3(n_segs), dim3(1024), 0, s, segs, tok_req);
+    LAUNCHCHK("seg_tokens");
+}
+
 void cell_unpack(hipStream_t s, int M, const float* x4, float* grid) {
     hipLaunchKernelGGL(k_cell_unpack, dim3(cdiv(M, 256)), dim3(256), 0, s, M, (const float4*)x4, grid);
     LAUNCHCHK("cell_unpack");

@@ -14,19 +14,29 @@
 //
 // A rollout request (j2k_server_submit_rollout) chains windows of T slices
 // along time: window w's last slice becomes window w+1's known slice 0, its
-// other slices start as copies of it, and the request stays active (state
-// re-uploaded from the host copy) until `ticks` slices are covered. Each
-// window retires on its own residual like a plain request, and its energy
-// violation, iterations and completion time are kept per window.
+// other slices start as copies of it, and the request stays active until
+// `ticks` slices are covered. The next window is built on the device from
+// that slice of the packed state (nothing goes back up from the host); each
+// finished window is copied out into the trajectory. Each window retires on
+// its own residual like a plain request, and its energy violation,
+// iterations and completion time are kept per window.
+//
+// Host traffic per turn: the new requests' grids and masks go up into their
+// own regions of one staging buffer and the retiring requests' grids come
+// down from another, with the launches between them back to back and one
+// synchronisation per phase, not per request.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -57,6 +67,7 @@ struct Request {
     StallWatch watch;
     // rollout requests: ticks > 0
     int ticks = 0, window = 0, wi = 0;  // total slices, slices per window, windows done
+    int prev_T = 0;  // > 0: the next window starts from slice prev_T - 1 of the packed state at `base`
     std::vector<float> traj;            // [3, ticks, ny, nx]
     std::vector<double> win_done, win_viol;
     std::vector<int> win_iters;
@@ -136,9 +147,33 @@ struct Server {
     bool graphs_enabled = true;
     unsigned* h_resid = nullptr;  // pinned
     size_t h_resid_cap = 0;
+    // staging of the host copies: uploads of new requests (grid then mask,
+    // each request in its own 256-byte aligned region), downloads of retiring
+    // ones, and the retiring requests' slice table and violation words
+    unsigned char* d_in = nullptr;
+    unsigned char* d_out = nullptr;
+    size_t cap_in = 0, cap_out = 0;
+    Slice* d_rsl = nullptr;
+    size_t cap_rsl = 0;
+    unsigned long long* d_viol = nullptr;
+    size_t cap_viol = 0;
+    // pinned host staging of the same copies, so each turn's traffic is one
+    // DMA per direction; reserved at creation for the largest pack the
+    // budget admits (up to 2^24 tokens), grown if a turn needs more
+    unsigned char* h_in = nullptr;
+    unsigned char* h_out = nullptr;
+    size_t cap_hin = 0, cap_hout = 0;
+    // J2K_SERVER_PHASES=1 at creation: seconds the worker spent per phase,
+    // printed to stderr when the server is destroyed
+    bool phases_on = false;
+    double ph_repack = 0, ph_iter = 0, ph_finish = 0, ph_turns = 0;
+    std::string ph_detail;  // per repack: tables, staging copies, packing launches, scratch, final wait (ms)
 
     Server(Model& model, size_t b, int ma)
-        : m(model), budget(b), max_active(ma), graphs_enabled(!model.graphs_off) {}
+        : m(model), budget(b), max_active(ma), graphs_enabled(!model.graphs_off) {
+        const char* e = std::getenv("J2K_SERVER_PHASES");
+        phases_on = e && std::strcmp(e, "0") != 0;
+    }
 
     ~Server() {
         {
@@ -147,6 +182,9 @@ struct Server {
         }
         cv_work.notify_all();
         if (worker.joinable()) worker.join();
+        if (phases_on)
+            std::fprintf(stderr, "j2k server phases: %.0f turns, repack %.4f s, iterations %.4f s, retire %.4f s%s\n",
+                         ph_turns, ph_repack, ph_iter, ph_finish, ph_detail.c_str());
         int prev = -1;
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(m.device);
@@ -154,6 +192,8 @@ struct Server {
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
         if (h_resid) (void)hipHostFree(h_resid);
+        if (h_in) (void)hipHostFree(h_in);
+        if (h_out) (void)hipHostFree(h_out);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
@@ -170,6 +210,61 @@ struct Server {
         if (it != owned.end()) owned.erase(it);
         (void)hipFree(p);
     }
+    static size_t aligned(size_t b) { return (b + 255) & ~(size_t)255; }
+    // Grows a server-owned device buffer to at least `bytes` (contents not kept).
+    template <class T>
+    void reserve(T*& p, size_t& cap, size_t count) {
+        if (count <= cap) return;
+        HIPCHK(hipStreamSynchronize(m.stream));
+        dev_free(p);
+        cap = count + count / 4;
+        p = dev_alloc<T>(cap);
+    }
+
+    // Grows a pinned host staging buffer to at least `bytes` (contents not kept).
+    void reserve_host(unsigned char*& p, size_t& cap, size_t bytes) {
+        if (bytes <= cap) return;
+        HIPCHK(hipStreamSynchronize(m.stream));
+        if (p) HIPCHK(hipHostFree(p));
+        p = nullptr;
+        cap = 0;
+        const size_t want = bytes + bytes / 4;
+        hip_check(hipHostMalloc((void**)&p, want), "server staging hipHostMalloc", J2K_ERR_OOM);
+        cap = want;
+    }
+
+    // Host copies of one turn, (dst, src, bytes) each, spread over up to
+    // eight threads in pieces of at most 4 MB when there is much to copy.
+    struct Copy {
+        void* dst;
+        const void* src;
+        size_t bytes;
+    };
+    static void copy_all(const std::vector<Copy>& jobs) {
+        size_t total = 0;
+        for (const Copy& c : jobs) total += c.bytes;
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const int nt = total >= (size_t(16) << 20) ? (int)std::min(8u, hw) : 1;
+        if (nt <= 1) {
+            for (const Copy& c : jobs) std::memcpy(c.dst, c.src, c.bytes);
+            return;
+        }
+        constexpr size_t piece = size_t(4) << 20;
+        std::vector<Copy> pieces;
+        for (const Copy& c : jobs)
+            for (size_t o = 0; o < c.bytes; o += piece)
+                pieces.push_back(Copy{static_cast<char*>(c.dst) + o, static_cast<const char*>(c.src) + o,
+                                      std::min(piece, c.bytes - o)});
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t i = next++; i < pieces.size(); i = next++) std::memcpy(pieces[i].dst, pieces[i].src, pieces[i].bytes);
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    }
+
     void drop_graphs() {
         for (auto& g : graph)
             if (g) {
@@ -187,7 +282,9 @@ struct Server {
         std::vector<CellDims> dims;
         for (auto& r : active) dims.push_back(r->dims);
         const bool multi = active.size() > 1;
-        tab.build(dims, attn_tokens_per_block(m.cfg.heads), multi);
+        const double r0 = now_s();
+        tab.build(dims, attn_tokens_per_block(m.cfg.heads), false);  // tok_req is made on the device
+        const double r1 = now_s();
         const size_t M = (size_t)tab.M;
         const bool grow = M > cap_tok;
         const size_t want = grow ? M + M / 4 : cap_tok;
@@ -197,18 +294,40 @@ struct Server {
             alt0 = dev_alloc<float>(4 * want);
             altc = dev_alloc<float>(4 * want);
         }
+        size_t in_bytes = 0;
+        for (auto& r : active)
+            if (r->base < 0) in_bytes += aligned(13 * (size_t)r->dims.tokens());
+        reserve(d_in, cap_in, in_bytes);
+        reserve_host(h_in, cap_hin, in_bytes);
+        {  // every new grid and mask, and the per-token request table, into pinned staging: one upload
+            std::vector<Copy> jobs;
+            size_t o = 0;
+            for (auto& r : active)
+                if (r->base < 0) {
+                    const size_t T = (size_t)r->dims.tokens();
+                    jobs.push_back(Copy{h_in + o, r->grid.data(), 12 * T});
+                    jobs.push_back(Copy{h_in + o + 12 * T, r->known.data(), T});
+                    o += aligned(13 * T);
+                }
+            copy_all(jobs);
+            if (in_bytes) HIPCHK(hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, s));
+        }
+        const double r2 = now_s();
+        size_t in_off = 0;
         int slices = 0;  // the slice table lists each request's slices in order
         for (size_t i = 0; i < active.size(); i++) {
             Request& r = *active[i];
             const size_t nb = (size_t)tab.segs[i].base, T = (size_t)r.dims.tokens();
-            if (r.base < 0) {
-                float* dgrid = m.get<float>(S_GRID, 3 * T);
-                uint8_t* dk = m.get<uint8_t>(S_KNOWN, T);
-                HIPCHK(hipMemcpyAsync(dgrid, r.grid.data(), 12 * T, hipMemcpyHostToDevice, s));
-                HIPCHK(hipMemcpyAsync(dk, r.known.data(), T, hipMemcpyHostToDevice, s));
-                cell_pack(s, (int)T, dgrid, dk, alt0 + 4 * nb);
-                HIPCHK(hipMemcpyAsync(altc + 4 * nb, alt0 + 4 * nb, 16 * T, hipMemcpyDeviceToDevice, s));
-                HIPCHK(hipStreamSynchronize(s));  // S_GRID/S_KNOWN are reused by the next request
+            if (r.base < 0) {  // new: up into its own staging region, packed into x0 and the first iterate
+                const float* dgrid = reinterpret_cast<const float*>(d_in + in_off);
+                const uint8_t* dk = d_in + in_off + 12 * T;
+                cell_pack(s, (int)T, dgrid, dk, alt0 + 4 * nb, altc + 4 * nb);
+                in_off += aligned(13 * T);
+            } else if (r.prev_T > 0) {  // a rollout's next window, from its last slice on the device
+                const size_t n = (size_t)r.dims.ny * r.dims.nx;
+                cell_window(s, (int)n, r.dims.T, xc + 4 * ((size_t)r.base + (size_t)(r.prev_T - 1) * n), alt0 + 4 * nb,
+                            altc + 4 * nb);
+                r.prev_T = 0;
             } else {
                 HIPCHK(hipMemcpyAsync(alt0 + 4 * nb, x0 + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
                 HIPCHK(hipMemcpyAsync(altc + 4 * nb, xc + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
@@ -256,10 +375,12 @@ struct Server {
         HIPCHK(hipMemcpy(d_segs, tab.segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_tiles, tab.tiles.data(), ntile * sizeof(Tile), hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(d_slices, tab.slices.data(), nsl * sizeof(Slice), hipMemcpyHostToDevice));
-        if (multi) HIPCHK(hipMemcpy(d_tokreq, tab.tok_req.data(), tab.tok_req.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (multi) seg_tokens(s, (int)nseg, d_segs, d_tokreq);
         float* dmeta = m.get<float>(S_META, (size_t)m.cfg.meta_dim * active.size());
+        std::vector<float> metas((size_t)m.cfg.meta_dim * active.size());
         for (size_t i = 0; i < active.size(); i++)
-            HIPCHK(hipMemcpy(dmeta + i * m.cfg.meta_dim, active[i]->meta.data(), m.cfg.meta_dim * 4, hipMemcpyHostToDevice));
+            std::memcpy(&metas[i * m.cfg.meta_dim], active[i]->meta.data(), m.cfg.meta_dim * sizeof(float));
+        HIPCHK(hipMemcpyAsync(dmeta, metas.data(), metas.size() * sizeof(float), hipMemcpyHostToDevice, s));
         for (size_t i = 0; i < active.size(); i++)
             meta_linear(s, m.cfg.meta_dim, m.cfg.dim, dmeta + i * m.cfg.meta_dim, m.meta_w, m.meta_b,
                         d_mvec + i * m.cfg.dim);
@@ -280,6 +401,7 @@ struct Server {
         // targets depend on each request's x0 only, so recomputing them is exact
         cell_targets(s, (int)nsl, d_slices, x0, pack.mass_t, pack.eref_t, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
                      m.cfg.norm_depth);
+        const double r3 = now_s();
         // grow the packed activation scratch now, never inside a capture
         const size_t d = m.cfg.dim, ff = m.cfg.ff_inner, e = m.bf16() ? 2 : 4;
         m.get<float>(S_X, M * d);
@@ -296,8 +418,15 @@ struct Server {
             h_resid_cap = active.size() + 64;
             hip_check(hipHostMalloc((void**)&h_resid, h_resid_cap * sizeof(unsigned)), "hipHostMalloc", J2K_ERR_OOM);
         }
+        const double r4 = now_s();
         HIPCHK(hipStreamSynchronize(s));
         packed = true;
+        if (phases_on) {
+            char b[160];
+            std::snprintf(b, sizeof b, "; repack of %zu tokens: tables %.1f, staging %.1f, packing %.1f, scratch %.1f, wait %.1f", M,
+                          1e3 * (r1 - r0), 1e3 * (r2 - r1), 1e3 * (r3 - r2), 1e3 * (r4 - r3), 1e3 * (now_s() - r4));
+            ph_detail += b;
+        }
     }
 
     // One iteration xc -> xn, then the names swap.
@@ -335,33 +464,56 @@ struct Server {
         parity ^= 1;
     }
 
-    // Final energy violation and grid of active[i] (reads xc).
-    void finish(size_t i) {
-        Request& r = *active[i];
+    // Final energy violation and grid of every request active[idx[j]] (reads
+    // xc): one slice table for all of them, their violation and unpack
+    // launches back to back, the copies out, one synchronisation.
+    void retire(const std::vector<size_t>& idx) {
         hipStream_t s = m.stream;
-        const size_t first = (size_t)r.slice0;  // set by repack: later rewindows do not move it
-        const int n = r.dims.ny * r.dims.nx;
+        const size_t k = idx.size();
         std::vector<Slice> sl;
-        for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, 0});
-        Slice* dsl = m.get<Slice>(S_SLICES, sl.size());
-        m.cell_tables.T = -1;  // the single-request tables shared this slot
-        unsigned* vb32 = m.get<unsigned>(S_RESID, 4);
-        HIPCHK(hipMemcpyAsync(dsl, sl.data(), sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
-        fill_u32(s, vb32, 2, 0u);
-        cell_violation(s, (int)sl.size(), dsl, xc, pack.eref_t + first, reinterpret_cast<unsigned long long*>(vb32),
-                       m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth, m.cfg.energy_r_max);
-        const size_t T = (size_t)r.dims.tokens();
-        float* dgrid = m.get<float>(S_GRID, 3 * T);
-        cell_unpack(s, (int)T, xc + 4 * (size_t)r.base, dgrid);
-        double v = 0;
-        HIPCHK(hipMemcpyAsync(&v, vb32, 8, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(r.grid.data(), dgrid, 12 * T, hipMemcpyDeviceToHost, s));
+        std::vector<size_t> first(k), out_off(k);
+        size_t out_bytes = 0;
+        for (size_t j = 0; j < k; j++) {
+            const Request& r = *active[idx[j]];
+            const int n = r.dims.ny * r.dims.nx;
+            first[j] = sl.size();
+            for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, (int)j});
+            out_off[j] = out_bytes;
+            out_bytes += aligned(12 * (size_t)r.dims.tokens());
+        }
+        reserve(d_out, cap_out, out_bytes);
+        reserve(d_rsl, cap_rsl, sl.size());
+        reserve(d_viol, cap_viol, k);
+        HIPCHK(hipMemcpyAsync(d_rsl, sl.data(), sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
+        fill_u32(s, reinterpret_cast<unsigned*>(d_viol), (int)(2 * k), 0u);
+        for (size_t j = 0; j < k; j++) {
+            const Request& r = *active[idx[j]];
+            // slice0 is this window's first slice in the packing's table (set by repack)
+            cell_violation(s, r.dims.T, d_rsl + first[j], xc, pack.eref_t + r.slice0, d_viol, m.cfg.g,
+                           m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth, m.cfg.energy_r_max);
+            cell_unpack(s, r.dims.tokens(), xc + 4 * (size_t)r.base, reinterpret_cast<float*>(d_out + out_off[j]));
+        }
+        reserve_host(h_out, cap_hout, out_bytes + k * sizeof(double));
+        HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        r.res.iters = r.iters;
-        r.res.resid = r.resid;
-        r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
-        r.res.violation = v;
-        r.res.done_s = now_s();
+        std::vector<Copy> jobs;
+        for (size_t j = 0; j < k; j++) {
+            Request& r = *active[idx[j]];
+            jobs.push_back(Copy{r.grid.data(), h_out + out_off[j], 12 * (size_t)r.dims.tokens()});
+        }
+        copy_all(jobs);
+        std::vector<double> v(k);
+        std::memcpy(v.data(), h_out + out_bytes, k * sizeof(double));
+        const double t = now_s();
+        for (size_t j = 0; j < k; j++) {
+            Request& r = *active[idx[j]];
+            r.res.iters = r.iters;
+            r.res.resid = r.resid;
+            r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
+            r.res.violation = v[j];
+            r.res.done_s = t;
+        }
     }
 
     // Records the window just finished (r.grid holds it) into the trajectory
@@ -380,10 +532,13 @@ struct Server {
         r.converged_all = r.converged_all && r.res.converged;
         r.wi++;
         const int total = window_count(r.ticks, r.window);
-        if (r.wi < total) {
-            std::vector<float> anchor(3 * n);
-            for (int c = 0; c < 3; c++) std::memcpy(&anchor[c * n], &r.grid[((size_t)c * T + T - 1) * n], n * sizeof(float));
-            set_window(r, anchor.data(), window_slices(r.ticks, r.window, r.wi));
+        if (r.wi < total) {  // the next window starts on the device from this one's last slice
+            r.prev_T = T;
+            r.dims.T = window_slices(r.ticks, r.window, r.wi);
+            r.grid.resize(3 * (size_t)r.dims.T * n);
+            r.iters = 0;
+            r.resid = 0.f;
+            r.watch = StallWatch{};
             return true;
         }
         // the result of a rollout is its trajectory; counters over all windows
@@ -425,13 +580,20 @@ struct Server {
                 peak_bytes = std::max(peak_bytes, used);
             }
             if (active.empty()) continue;
+            const double t_a = now_s();
             if (changed || !packed) repack();
+            const double t_b = now_s();
             run_iteration();
             HIPCHK(hipMemcpyAsync(h_resid, pack.resid, 4 * active.size(), hipMemcpyDeviceToHost, m.stream));
             HIPCHK(hipStreamSynchronize(m.stream));
+            const double t_c = now_s();
+            ph_turns += 1;
+            ph_repack += t_b - t_a;
+            ph_iter += t_c - t_b;
             std::vector<bool> out(active.size(), false);
             bool any = false;
             bool rewindowed = false;
+            std::vector<size_t> retiring;
             for (size_t i = 0; i < active.size(); i++) {
                 Request& r = *active[i];
                 r.iters++;
@@ -439,15 +601,19 @@ struct Server {
                 const bool stalled = r.watch.update(r.resid) && m.cfg.stall_stop;
                 if (r.resid < m.cfg.tol || r.iters >= m.cfg.max_iter || stalled) {
                     r.res.stalled = r.res.stalled || (r.resid >= m.cfg.tol && stalled) ? 1 : 0;
-                    finish(i);
-                    if (r.ticks > 0 && next_window(r)) {
-                        rewindowed = true;  // stays active with its next window
-                        continue;
-                    }
-                    out[i] = any = true;
+                    retiring.push_back(i);
                 }
             }
+            if (!retiring.empty()) retire(retiring);
+            for (size_t i : retiring) {
+                if (active[i]->ticks > 0 && next_window(*active[i])) {
+                    rewindowed = true;  // stays active with its next window
+                    continue;
+                }
+                out[i] = any = true;
+            }
             if (rewindowed) packed = false;
+            ph_finish += now_s() - t_c;
             if (!any) continue;
             std::lock_guard<std::mutex> lk(mu);
             std::vector<std::unique_ptr<Request>> keep;
@@ -504,6 +670,13 @@ int j2k_server_create(void* model, size_t budget, int max_active, void** out) {
             autotune(m, (long)std::min<size_t>(cap, size_t(1) << 20));
         }
         auto s = std::make_unique<Server>(m, budget, max_active);
+        {  // pinned staging for the largest pack the budget admits, up to 2^24 tokens
+            DeviceGuard dg(m.device);
+            const size_t tokens = std::min<size_t>(budget / bytes_per_token(m), size_t(1) << 24);
+            const size_t slack = 256 * (size_t)std::min(max_active, 4096);
+            s->reserve_host(s->h_in, s->cap_hin, 13 * tokens + 2 * slack);
+            s->reserve_host(s->h_out, s->cap_hout, 12 * tokens + 2 * slack);
+        }
         s->start();
         *out = s.release();
     });

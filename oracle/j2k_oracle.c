@@ -110,11 +110,12 @@ double orc_energy_violation_c(const float* x, const orc_phys* p, int T, int n, c
     return worst;
 }
 
-/* bf16 activations: when on, the trunk rounds the values the served bf16
- * form stores as bf16 (LN outputs, q, k, v, the attention output, the GELU
- * output) to bf16, nearest even, through f32. */
+/* bf16 activations: a mask of the values the trunk rounds to bf16 (nearest
+ * even, through f32), the points where the served bf16 form stores bf16:
+ * ORC_BF16_LN1 (LN1 output), ORC_BF16_QKV (q, k, v), ORC_BF16_ATTN (the
+ * attention output), ORC_BF16_LN2 (LN2 output), ORC_BF16_GELU (GELU output). */
 static int g_act_bf16 = 0;
-void orc_set_act_bf16(int on) { g_act_bf16 = on; }
+void orc_set_act_bf16(int mask) { g_act_bf16 = mask; }
 static float o_bf16f(float f) {
     uint32_t u;
     memcpy(&u, &f, 4);

@@ -107,9 +107,15 @@ float orc_gelu_f32(float x);
 
 /* Threads used by the parallel row loops (OpenMP); 0 = library default. */
 void orc_set_threads(int n);
-/* 1: the trunk rounds what the served bf16 form stores in bf16 (LN outputs,
- * q, k, v, the attention output, the GELU output); 0 (default): no rounding. */
-void orc_set_act_bf16(int on);
+/* Mask of the trunk values rounded to bf16 (0, the default: none;
+ * ORC_BF16_ALL: every point where the served bf16 form stores bf16). */
+#define ORC_BF16_LN1 1
+#define ORC_BF16_QKV 2
+#define ORC_BF16_ATTN 4
+#define ORC_BF16_LN2 8
+#define ORC_BF16_GELU 16
+#define ORC_BF16_ALL 31
+void orc_set_act_bf16(int mask);
 
 /* Shallow-water solver (crates/solvers/src/shallow_water.rs), sw_oracle.c. */
 typedef struct {

@@ -230,14 +230,19 @@ def gelu(x: float) -> float:
 
 # ---------------------------------------------------------------- form C
 
+# bf16 rounding points of the trunk (orc_set_act_bf16)
+ACT_BF16_LN1, ACT_BF16_QKV, ACT_BF16_ATTN, ACT_BF16_LN2, ACT_BF16_GELU = 1, 2, 4, 8, 16
+ACT_BF16_ALL = 31
+
+
 @dataclass
 class FormC:
     """Form C model (per-cell space-time tokens): config dict + tensors.
-    act_bf16 rounds the trunk's stored activations to bf16 (the served bf16
-    form's storage; see orc_set_act_bf16)."""
+    act_bf16 is a mask of trunk values rounded to bf16 (ACT_BF16_* below;
+    ACT_BF16_ALL is the served bf16 form's storage, see orc_set_act_bf16)."""
     cfg: dict
     w: dict
-    act_bf16: bool = False
+    act_bf16: int = 0
 
     @staticmethod
     def load(dir_or_bytes, config_json: str | None = None) -> "FormC":

@@ -68,7 +68,7 @@ def bf16_weights(cpu: O.FormC) -> O.FormC:
 def bf16_form(cpu: O.FormC) -> O.FormC:
     """The oracle with the bf16 form's arithmetic: bf16 block matrices and
     the trunk's stored activations rounded to bf16."""
-    return O.FormC(cpu.cfg, bf16_weights(cpu).w, act_bf16=True)
+    return O.FormC(cpu.cfg, bf16_weights(cpu).w, act_bf16=O.ACT_BF16_ALL)
 
 
 PLANTS = ("readout_zero", "last_attention_dropped", "first_mlp_dropped", "space_radius_0", "time_radius_0")

@@ -144,6 +144,7 @@ struct Server {
     std::vector<void*> owned;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // [0]: xc->xn with the current naming, [1]: swapped
     int parity = 0;
+    int since_repack = 0;  // iterations run on the current packing
     bool graphs_enabled = true;
     unsigned* h_resid = nullptr;  // pinned
     size_t h_resid_cap = 0;
@@ -351,6 +352,7 @@ struct Server {
             cap_tok = want;
         }
         parity = 0;
+        since_repack = 0;
 
         const size_t nseg = tab.segs.size(), ntile = tab.tiles.size(), nsl = tab.slices.size();
         const size_t need = std::max({nseg, ntile, nsl, (size_t)1});
@@ -432,7 +434,12 @@ struct Server {
     // One iteration xc -> xn, then the names swap.
     void run_iteration() {
         hipStream_t s = m.stream;
-        if (graphs_enabled && !graph[parity]) {
+        // The first iteration on a new packing runs eagerly: a packing that
+        // changes every turn (requests converging in one iteration) would
+        // otherwise pay a capture per turn that is never replayed. From the
+        // second iteration on the same packing, each parity is captured once
+        // and replayed.
+        if (graphs_enabled && since_repack >= 1 && !graph[parity]) {
             hipGraph_t g = nullptr;
             bool ok = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess;
             if (ok) {
@@ -453,13 +460,14 @@ struct Server {
                 (void)hipGetLastError();
             }
         }
-        if (graphs_enabled && graph[parity]) {
+        if (graphs_enabled && since_repack >= 1 && graph[parity]) {
             HIPCHK(hipGraphLaunch(graph[parity], s));
             replays++;
         } else {
             cell_iteration(m, s, pack, x0, xc, xn);
         }
         launches++;
+        since_repack++;
         std::swap(xc, xn);
         parity ^= 1;
     }

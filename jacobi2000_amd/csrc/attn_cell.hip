@@ -19,6 +19,8 @@
 //             per CU, twice the barriers.
 //   KV_PLANE  one time plane of K or V at a time, double-buffered: a few KB
 //             of LDS, one barrier per plane.
+//   KV_PAIR2, KV_PAIR3  KV_SPLIT with two x-adjacent queries per lane, which
+//             share their key rows in registers (k_attn_cell_pair).
 //
 // Four lanes share a query, each holding a quarter of the head's dimensions;
 // the dot products are reduced with two DPP quad permutations. The softmax is
@@ -412,13 +414,190 @@ __global__ __launch_bounds__(CNT, ((2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 
 #undef J2K_PLANE_FETCH
 }
 
+// KV_PAIR2 / KV_PAIR3: the KV_SPLIT staging with two x-adjacent queries per
+// lane (four lanes per pair, 256 threads per 8 x 8 x 2 tile). The pair's
+// stencils overlap: for each (time, y) offset the lane reads 2 RS + 2 key
+// rows from LDS instead of 2 (2 RS + 1), so the LDS reads, which bound the
+// kernel, fall by a third (27-key stencil). Each query's dot products,
+// softmax and weighting are those of the other modes, term for term and in
+// the same key order. OCC blocks share a CU (2: the whole staging phase is
+// prefetched in registers; 3: part of it is loaded when it is staged).
+template <int HD, int RS, int RT, class T, int OCC>
+__global__ __launch_bounds__(256, OCC) void k_attn_cell_pair(const T* __restrict__ qkv, T* __restrict__ out, int dim,
+                                                             int heads, const Seg* __restrict__ segs,
+                                                             const CellTile* __restrict__ tiles) {
+    constexpr int NT = 256;
+    constexpr int D4 = HD / 4, VEC = 16 / sizeof(T), VPR = HD / VEC;
+    constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
+    constexpr int KW = 2 * RS + 1, KS = KW * KW, KMAX = KS * (2 * RT + 1);
+    constexpr int NV = ((2 * RT + CT_T) * PLANE * VPR + NT - 1) / NT;  // staged vectors per thread
+    extern __shared__ __align__(16) unsigned char smem[];
+    const CellTile tl = tiles[xcd_remap(blockIdx.x, gridDim.x)];
+    const Seg sg = segs[tl.seg];
+    const int t_lo = max(tl.t - RT, 0), t_hi = min(tl.t + CT_T - 1 + RT, sg.T - 1), nt = t_hi - t_lo + 1;
+    const int ncell = nt * PLANE, nvec = ncell * VPR;
+    T* const Ks = reinterpret_cast<T*>(smem);  // K, then V, of one head
+    const long ld = 3L * dim;
+    const int n = sg.ny * sg.nx;
+    const int tid = threadIdx.x;
+    auto row_of = [&](int v) {
+        if (v >= nvec) return sg.base;
+        const int c = v / VPR, ct = c / PLANE, cr = c - ct * PLANE, cy = cr / HW, cx = cr - cy * HW;
+        const int y = wrap(tl.y0 - RS + cy, sg.ny), x = wrap(tl.x0 - RS + cx, sg.nx);
+        return sg.base + (t_lo + ct) * n + y * sg.nx + x;
+    };
+    constexpr int PF_FIT = (512 / OCC * (OCC > 2 ? 6 : 7) / 10 - 2 * KMAX - 5 * D4) / 5;
+    constexpr int PF = NV < PF_FIT ? NV : (PF_FIT < 4 ? 4 : PF_FIT);
+    int grow[PF];
+#pragma unroll
+    for (int i = 0; i < PF; i++) grow[i] = row_of(tid + NT * i);
+    u32x4 pre[PF];
+#define J2K_PAIR_FETCH(h, w)                                                                                      \
+    _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                              \
+        const int v = tid + NT * i, e = v % VPR;                                                                  \
+        pre[i] = *reinterpret_cast<const u32x4*>(qkv + (long)grow[i] * ld + (1 + (w)) * dim + (h) * HD + e * VEC);   \
+    }
+#define J2K_PAIR_STAGE(h, w)                                                                                      \
+    {                                                                                                             \
+        _Pragma("unroll") for (int i = 0; i < PF; i++) {                                                          \
+            const int v = tid + NT * i;                                                                           \
+            if (v < nvec) *reinterpret_cast<u32x4*>(Ks + (size_t)v * VEC) = pre[i];                              \
+        }                                                                                                         \
+        _Pragma("unroll") for (int i0 = PF; i0 < NV; i0 += 8) {                                                   \
+            u32x4 tmp[8];                                                                                         \
+            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                       \
+                const int i = i0 + j, v = tid + NT * i, e = v % VPR;                                              \
+                if (i < NV)                                                                                       \
+                    tmp[j] = *reinterpret_cast<const u32x4*>(qkv + (long)row_of(v) * ld + (1 + (w)) * dim + (h) * HD + \
+                                                             e * VEC);                                            \
+            }                                                                                                     \
+            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                       \
+                const int i = i0 + j, v = tid + NT * i;                                                           \
+                if (i < NV && v < nvec) *reinterpret_cast<u32x4*>(Ks + (size_t)v * VEC) = tmp[j];                 \
+            }                                                                                                     \
+        }                                                                                                         \
+    }
+    // the lane's queries: cells (qyl, qxl) and (qyl, qxl + 1) of the tile in
+    // slice tq (a slice past the segment's end repeats the tile's first and
+    // stores nothing); keys are planes poff .. poff + ntq - 1 of the staged ones
+    const int pr = (tid >> 2) & 31, sub = tid & 3, ts = tid >> 7;
+    const int qyl = pr >> 2, qxl = (pr & 3) * 2;
+    const int qy = tl.y0 + qyl, qx0 = tl.x0 + qxl;
+    const bool tlive = qy < sg.ny && tl.t + ts < sg.T;
+    const bool live0 = tlive && qx0 < sg.nx, live1 = tlive && qx0 + 1 < sg.nx;
+    const int tq = tl.t + ts < sg.T ? tl.t + ts : tl.t;
+    const int tq_lo = max(tq - RT, 0), ntq = min(tq + RT, sg.T - 1) - tq_lo + 1, poff = tq_lo - t_lo;
+    const long slice0 = (long)sg.base + (long)tq * n;
+    const long qrow0 = slice0 + (live0 ? (long)qy * sg.nx + qx0 : 0);
+    const long qrow1 = slice0 + (live1 ? (long)qy * sg.nx + qx0 + 1 : 0);
+    const float scale = 1.0f / sqrtf((float)HD);
+
+    J2K_PAIR_FETCH(0, 0)
+    for (int h = 0; h < heads; h++) {
+        __syncthreads();  // the previous phase's rows are no longer read
+        J2K_PAIR_STAGE(h, 0)
+        __syncthreads();
+        J2K_PAIR_FETCH(h, 1)  // V of this head, in flight during the scores
+        float q0[D4], q1[D4], kv[D4], a0[D4], a1[D4], s0[KMAX], s1[KMAX];
+        row_f32<D4>(qkv + qrow0 * ld + h * HD + sub * D4, q0);
+        row_f32<D4>(qkv + qrow1 * ld + h * HD + sub * D4, q1);
+        float mx0 = -INFINITY, mx1 = -INFINITY;
+#pragma unroll
+        for (int ct = 0; ct < 2 * RT + 1; ct++) {
+            if (ct >= ntq) break;
+#pragma unroll
+            for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                for (int e = 0; e <= 2 * RS + 1; e++) {
+                    // key cell x offset e - RS from query 0 and e - RS - 1 from query 1
+                    const int c = ((poff + ct) * HH + qyl + RS + dy) * HW + qxl + e;
+                    row_f32<D4>(Ks + (size_t)c * HD + sub * D4, kv);
+                    if (e <= 2 * RS) {
+                        float dot = 0.f;
+#pragma unroll
+                        for (int d = 0; d < D4; d++) dot += q0[d] * kv[d];
+                        dot += quad_xor1(dot);
+                        dot += quad_xor2(dot);
+                        const float sc = dot * scale;
+                        s0[ct * KS + (dy + RS) * KW + e] = sc;
+                        mx0 = fmaxf(mx0, sc);
+                    }
+                    if (e >= 1) {
+                        float dot = 0.f;
+#pragma unroll
+                        for (int d = 0; d < D4; d++) dot += q1[d] * kv[d];
+                        dot += quad_xor1(dot);
+                        dot += quad_xor2(dot);
+                        const float sc = dot * scale;
+                        s1[ct * KS + (dy + RS) * KW + e - 1] = sc;
+                        mx1 = fmaxf(mx1, sc);
+                    }
+                }
+        }
+        __syncthreads();  // every wave's scores are done with K
+        J2K_PAIR_STAGE(h, 1)
+        __syncthreads();
+        if (h + 1 < heads) {
+            J2K_PAIR_FETCH(h + 1, 0)  // next head's K, in flight during the weighting
+        }
+#pragma unroll
+        for (int d = 0; d < D4; d++) {
+            a0[d] = 0.f;
+            a1[d] = 0.f;
+        }
+        float l0 = 0.f, l1 = 0.f;
+#pragma unroll
+        for (int ct = 0; ct < 2 * RT + 1; ct++) {
+            if (ct >= ntq) break;
+#pragma unroll
+            for (int dy = -RS; dy <= RS; dy++)
+#pragma unroll
+                for (int e = 0; e <= 2 * RS + 1; e++) {
+                    const int c = ((poff + ct) * HH + qyl + RS + dy) * HW + qxl + e;
+                    row_f32<D4>(Ks + (size_t)c * HD + sub * D4, kv);
+                    if (e <= 2 * RS) {
+                        const float p = __expf(s0[ct * KS + (dy + RS) * KW + e] - mx0);
+                        l0 += p;
+#pragma unroll
+                        for (int d = 0; d < D4; d++) a0[d] += p * kv[d];
+                    }
+                    if (e >= 1) {
+                        const float p = __expf(s1[ct * KS + (dy + RS) * KW + e - 1] - mx1);
+                        l1 += p;
+#pragma unroll
+                        for (int d = 0; d < D4; d++) a1[d] += p * kv[d];
+                    }
+                }
+        }
+        if (live0) {
+            const float inv = 1.0f / l0;
+#pragma unroll
+            for (int d = 0; d < D4; d++) a0[d] *= inv;
+            row_store<D4>(out + qrow0 * dim + h * HD + sub * D4, a0);
+        }
+        if (live1) {
+            const float inv = 1.0f / l1;
+#pragma unroll
+            for (int d = 0; d < D4; d++) a1[d] *= inv;
+            row_store<D4>(out + qrow1 * dim + h * HD + sub * D4, a1);
+        }
+    }
+#undef J2K_PAIR_STAGE
+#undef J2K_PAIR_FETCH
+}
+
 namespace {
 
 template <int HD, int RS, int RT, class T>
 const void* fn_ptr(int mode) {
-    return mode == KV_JOINT   ? (const void*)k_attn_cell<HD, RS, RT, T, KV_JOINT>
-           : mode == KV_SPLIT ? (const void*)k_attn_cell<HD, RS, RT, T, KV_SPLIT>
-                              : (const void*)k_attn_cell_plane<HD, RS, RT, T>;
+    switch (mode) {
+        case KV_JOINT: return (const void*)k_attn_cell<HD, RS, RT, T, KV_JOINT>;
+        case KV_SPLIT: return (const void*)k_attn_cell<HD, RS, RT, T, KV_SPLIT>;
+        case KV_PLANE: return (const void*)k_attn_cell_plane<HD, RS, RT, T>;
+        case KV_PAIR2: return (const void*)k_attn_cell_pair<HD, RS, RT, T, 2>;
+        case KV_PAIR3: return (const void*)k_attn_cell_pair<HD, RS, RT, T, 3>;
+        default: return nullptr;
+    }
 }
 
 // Kernel for (head size, space radius, time radius, mode); nullptr when not built.
@@ -453,7 +632,7 @@ void attn_cell_prepare() {
     for (int hd : {8, 16, 32, 64})
         for (int rs : {1, 2})
             for (int rt : {0, 1})
-                for (int mode : {KV_JOINT, KV_SPLIT, KV_PLANE}) {
+                for (int mode = 0; mode < KV_MODES; mode++) {
                     HIPCHK(hipFuncSetAttribute(pick<float>(hd, rs, rt, mode),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
                     HIPCHK(hipFuncSetAttribute(pick<uint16_t>(hd, rs, rt, mode),
@@ -465,6 +644,8 @@ size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode) {
     if (mode == KV_PLANE) return 2ull * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;  // two planes of one head
     return (mode == KV_JOINT ? 2ull : 1ull) * (2 * rt + CT_T) * (CT_Y + 2 * rs) * (CT_X + 2 * rs) * hd * elem;
 }
+
+int attn_cell_threads(int mode) { return mode == KV_PAIR2 || mode == KV_PAIR3 ? 256 : CNT; }
 
 // Tiles in the order rows of tiles (y), then slices, then x: the kernels
 // deal consecutive tiles to one XCD, so an XCD holds bands of rows through
@@ -487,7 +668,7 @@ static void launch(hipStream_t s, const T* qkv, T* out, int dim, int heads, int 
     const size_t smem = attn_cell_smem(hd, rs, rt, sizeof(T), mode);
     if (!f || smem > 160 * 1024) fail(J2K_ERR_LIMIT, "attention: no LDS-tiled kernel for this head size / stencil");
     void* args[] = {(void*)&qkv, (void*)&out, (void*)&dim, (void*)&heads, (void*)&segs, (void*)&tiles};
-    HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(CNT), args, smem, s));
+    HIPCHK(hipLaunchKernel(f, dim3(n_tiles), dim3(attn_cell_threads(mode)), args, smem, s));
 }
 
 void attention_cell_f32(hipStream_t s, const float* qkv, float* out, int dim, int heads, int rs, int rt,

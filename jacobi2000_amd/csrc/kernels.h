@@ -155,7 +155,8 @@ int attn_tokens_per_block(int heads);
 struct CellTile {
     int seg, t, y0, x0;
 };
-enum KvMode : int { KV_JOINT = 0, KV_SPLIT = 1, KV_PLANE = 2 };
+// Staging modes of the tiled cell attention (the same result in every one).
+enum KvMode : int { KV_JOINT = 0, KV_SPLIT = 1, KV_PLANE = 2, KV_PAIR2 = 3, KV_PAIR3 = 4, KV_MODES = 5 };
 void build_cell_tiles(const std::vector<Seg>& segs, std::vector<CellTile>& tiles);
 size_t attn_cell_smem(int hd, int rs, int rt, int elem, int mode);
 bool attn_cell_ok(int hd, int rs, int rt, int elem, int mode);  // a tiled kernel exists and fits in LDS

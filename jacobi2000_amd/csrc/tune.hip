@@ -108,7 +108,7 @@ void tune_bucket(Model& m, int b, Events& ev) {
         HIPCHK(hipMemcpyAsync(dtiles, tiles.data(), tiles.size() * sizeof(CellTile), hipMemcpyHostToDevice, s));
         m.p_tables_batch = -1;  // the patch2d and cell_st tables share these slots
         m.cell_tables.T = -1;
-        for (int mode : {KV_JOINT, KV_SPLIT, KV_PLANE}) {
+        for (int mode = 0; mode < KV_MODES; mode++) {
             if (!attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode)) continue;
             m.tune.us[TO_ATTN][b][mode] = time_us(s, ev, [&] {
                 if (split)
@@ -191,7 +191,7 @@ int Tuning::choice(int op, long M) const {
 }
 
 void force_tuning(Model& m, const char* spec) {
-    // "gemm_bm=64|128", "attn_kv_mode=0|1|2", "rows_fused=0..4", comma separated; every bucket
+    // "gemm_bm=64|128", "attn_kv_mode=0..4", "rows_fused=0..4", comma separated; every bucket
     int bm = -1, mode = -1, rows = -1;
     std::string t(spec);
     size_t pos = 0;
@@ -204,7 +204,7 @@ void force_tuning(Model& m, const char* spec) {
             const std::string k = kv.substr(0, eq);
             const int v = std::atoi(kv.c_str() + eq + 1);
             if (k == "gemm_bm" && (v == 64 || v == 128)) bm = v == 64 ? 0 : 1;
-            else if (k == "attn_kv_mode" && (v == KV_JOINT || v == KV_SPLIT || v == KV_PLANE)) mode = v;
+            else if (k == "attn_kv_mode" && v >= 0 && v < KV_MODES) mode = v;
             else if (k == "rows_fused" && v >= 0 && v <= ROWS_2X2) rows = v;
             else fail(J2K_ERR_INVALID, "J2K_TUNE_FORCE: unknown setting '" + kv + "'");
         }
@@ -260,13 +260,10 @@ std::string tuning_json(const Model& m) {
             const int c = m.tune.pick[op][b];
             if (c < 0) continue;
             const int v = op == TO_ATTN || op == TO_ROWS ? c : GEMM_BM[c];
-            if (op == TO_ROWS)
+            if (op == TO_ROWS || op == TO_ATTN)
                 std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f, %.2f, %.2f]", names[op],
                               v, names[op], m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2],
                               m.tune.us[op][b][3], m.tune.us[op][b][4]);
-            else if (op == TO_ATTN)
-                std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f, %.2f]", names[op], v, names[op],
-                              m.tune.us[op][b][0], m.tune.us[op][b][1], m.tune.us[op][b][2]);
             else
                 std::snprintf(buf, sizeof buf, ", \"%s\": %d, \"%s_us\": [%.2f, %.2f]", names[op], v, names[op],
                               m.tune.us[op][b][0], m.tune.us[op][b][1]);

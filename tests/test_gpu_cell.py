@@ -216,7 +216,8 @@ def _model_env(st, cj, precision, **env):
 
 SPECS = ["gemm_bm=64,attn_kv_mode=0,rows_fused=1", "gemm_bm=128,attn_kv_mode=1,rows_fused=0",
          "gemm_bm=64,attn_kv_mode=2,rows_fused=0", "gemm_bm=128,attn_kv_mode=2,rows_fused=2",
-         "gemm_bm=64,attn_kv_mode=1,rows_fused=3", "gemm_bm=128,attn_kv_mode=0,rows_fused=4"]
+         "gemm_bm=64,attn_kv_mode=1,rows_fused=3", "gemm_bm=128,attn_kv_mode=0,rows_fused=4",
+         "gemm_bm=64,attn_kv_mode=3,rows_fused=1", "gemm_bm=128,attn_kv_mode=4,rows_fused=0"]
 
 
 @pytest.mark.parametrize("precision", ["f32", "bf16", "bf16x2"])
@@ -241,19 +242,15 @@ def test_launch_choices_do_not_change_results(model20_cell, precision, spec):
 
 @pytest.mark.parametrize("precision", ["f32", "bf16x2"])
 def test_wide_stencil_staging_modes_agree(precision):
-    """The 75-key stencil (prefetch plus batched remainder) in all three staging modes."""
+    """The 75-key stencil (prefetch plus batched remainder) in every staging mode."""
     st, cj = ck.random_cell_st(dim=256, layers=1, heads=8, ff=1024, seed=1, radius=2)
-    a = _model_env(st, cj, precision, J2K_TUNE_FORCE="attn_kv_mode=0")
-    b = _model_env(st, cj, precision, J2K_TUNE_FORCE="attn_kv_mode=1")
-    c = _model_env(st, cj, precision, J2K_TUNE_FORCE="attn_kv_mode=2")
+    models = [_model_env(st, cj, precision, J2K_TUNE_FORCE=f"attn_kv_mode={m}") for m in range(5)]
     T, ny, nx = 3, 24, 20
     g = ck.drops_grid(T, ny, nx, 2, 23)
     k = ck.mask_of("rewind", T, ny, nx)
     x0 = O.fill_blank(g, k)
-    xa = a.iterate(x0, k, META, max_iter=1, tol=0.0)[0]
-    xb = b.iterate(x0, k, META, max_iter=1, tol=0.0)[0]
-    xc = c.iterate(x0, k, META, max_iter=1, tol=0.0)[0]
-    assert np.array_equal(xa, xb) and np.array_equal(xa, xc)
+    xs = [m.iterate(x0, k, META, max_iter=1, tol=0.0)[0] for m in models]
+    assert all(np.array_equal(xs[0], x) for x in xs[1:])
 
 
 def test_autotune_table_and_results(model20_cell):
@@ -267,7 +264,7 @@ def test_autotune_table_and_results(model20_cell):
     assert [b["tokens"] for b in rep["buckets"]] == [4096, 8192, 16384]
     for b in rep["buckets"]:
         assert b["gemm_qkv_bm"] in (64, 128) and b["gemm_proj_bm"] in (64, 128)
-        assert b["attn_kv_mode"] in (0, 1, 2) and b["rows_fused"] in (0, 1, 2, 3, 4)
+        assert b["attn_kv_mode"] in (0, 1, 2, 3, 4) and b["rows_fused"] in (0, 1, 2, 3, 4)
         assert min(b["gemm_qkv_bm_us"]) > 0 and min(b["attn_kv_mode_us"]) > 0
     plain = _model_env(st, cj, "bf16x2", J2K_AUTOTUNE="0")
     assert plain.autotune(0)["buckets"] == []

@@ -2,39 +2,32 @@
 // allowed keys, as attention.wgsl:47-107 does): per query and head,
 // s = q.k / sqrt(hd) over the stencil's keys, softmax, o = sum p v.
 //
-// One thread per (token, head): a block covers 256/heads consecutive tokens
-// of one segment, so neighbouring threads read neighbouring q/k/v rows and
-// the stencil's overlapping key rows are served from L1/L2. The softmax is
-// the single-pass running-max form (rescale on a new max), accumulated in f32.
+// A group of hd/4 lanes per (token, head), each lane holding 4 of the head's
+// dimensions: the dot products are finished by xor shuffles inside the
+// group, every lane runs the softmax (the single-pass running-max form,
+// rescaled on a new max, in f32) and weights its own 4 dimensions. A block
+// covers 1024/dim consecutive tokens of one segment with all their heads, so
+// a 256-token grid of the reference forward already fills 32 blocks and the
+// stencil's overlapping key rows are served from L1/L2.
 #include "common.h"
 #include "kernels.h"
 
 namespace j2k {
 
-template <int HD>
-__device__ __forceinline__ void load_row(const float* p, float* out) {
-#pragma unroll
-    for (int e = 0; e < HD; e += 4) {
-        float4 v = *reinterpret_cast<const float4*>(p + e);
-        out[e] = v.x;
-        out[e + 1] = v.y;
-        out[e + 2] = v.z;
-        out[e + 3] = v.w;
-    }
+// Four consecutive elements as f32.
+__device__ __forceinline__ void load4(const float* p, float* out) {
+    const float4 v = *reinterpret_cast<const float4*>(p);
+    out[0] = v.x;
+    out[1] = v.y;
+    out[2] = v.z;
+    out[3] = v.w;
 }
-
-template <int HD>
-__device__ __forceinline__ void load_row(const uint16_t* p, float* out) {
-#pragma unroll
-    for (int e = 0; e < HD; e += 8) {
-        uint4 v = *reinterpret_cast<const uint4*>(p + e);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            out[e + 2 * j] = __uint_as_float(w[j] << 16);
-            out[e + 2 * j + 1] = __uint_as_float(w[j] & 0xffff0000u);
-        }
-    }
+__device__ __forceinline__ void load4(const uint16_t* p, float* out) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    out[0] = __uint_as_float(v.x << 16);
+    out[1] = __uint_as_float(v.x & 0xffff0000u);
+    out[2] = __uint_as_float(v.y << 16);
+    out[3] = __uint_as_float(v.y & 0xffff0000u);
 }
 
 __device__ __forceinline__ void store_out(float* p, float v) { *p = v; }
@@ -84,41 +77,46 @@ template <int HD, class T>
 __global__ __launch_bounds__(256) void k_attention(const T* __restrict__ qkv, T* __restrict__ out, int dim, int heads,
                                                    AttnGeo geo, const Seg* __restrict__ segs,
                                                    const Tile* __restrict__ tiles, int tpb) {
+    constexpr int G = HD / 4;  // lanes of one (token, head)
     const Tile tl = tiles[blockIdx.x];
     const Seg sg = segs[tl.seg];
-    const int lt = threadIdx.x / heads, head = threadIdx.x - lt * heads;
-    const int q = tl.start + lt;
-    if (lt >= tpb || q >= sg.T * sg.ny * sg.nx) return;
+    const int grp = threadIdx.x / G, sub = threadIdx.x - grp * G;
+    const int lt = grp / heads, head = grp - lt * heads;
+    // a group past the tile's end follows the tile's first token (so that its
+    // shuffles pair up) and stores nothing
+    const bool live = lt < tpb && tl.start + lt < sg.T * sg.ny * sg.nx;
+    const int q = live ? tl.start + lt : tl.start;
     const long row = (long)sg.base + q, ld = 3L * dim;
-    float qv[HD], acc[HD];
-    load_row<HD>(qkv + row * ld + head * HD, qv);
+    float qv[4], acc[4] = {0.f, 0.f, 0.f, 0.f};
+    load4(qkv + row * ld + head * HD + sub * 4, qv);
     const float scale = 1.0f / sqrtf((float)HD);
-#pragma unroll
-    for (int e = 0; e < HD; e++) acc[e] = 0.f;
     float mx = -INFINITY, l = 0.f;
-    const T* kbase = qkv + (long)sg.base * ld + dim + head * HD;
+    const T* kbase = qkv + (long)sg.base * ld + dim + head * HD + sub * 4;
     for_keys(geo, sg, q, [&](int k) {
-        float kv[HD];
-        load_row<HD>(kbase + (long)k * ld, kv);
+        float kv[4];
+        load4(kbase + (long)k * ld, kv);
         float dot = 0.f;
 #pragma unroll
-        for (int e = 0; e < HD; e++) dot += qv[e] * kv[e];
+        for (int e = 0; e < 4; e++) dot += qv[e] * kv[e];
+#pragma unroll
+        for (int off = 1; off < G; off <<= 1) dot += __shfl_xor(dot, off, G);
         const float s = dot * scale;
         const float mn = fmaxf(mx, s);
         const float corr = expf(mx - mn), p = expf(s - mn);
         l = l * corr + p;
-        load_row<HD>(kbase + dim + (long)k * ld, kv);
+        load4(kbase + dim + (long)k * ld, kv);
 #pragma unroll
-        for (int e = 0; e < HD; e++) acc[e] = acc[e] * corr + p * kv[e];
+        for (int e = 0; e < 4; e++) acc[e] = acc[e] * corr + p * kv[e];
         mx = mn;
     });
+    if (!live) return;
     const float inv = 1.0f / l;
-    T* o = out + row * dim + head * HD;
+    T* o = out + row * dim + head * HD + sub * 4;
 #pragma unroll
-    for (int e = 0; e < HD; e++) store_out(o + e, acc[e] * inv);
+    for (int e = 0; e < 4; e++) store_out(o + e, acc[e] * inv);
 }
 
-int attn_tokens_per_block(int heads) { return 256 / heads; }
+int attn_tokens_per_block(int dim) { return 1024 / dim; }
 
 template <class T>
 static void launch_attention(hipStream_t s, const T* qkv, T* out, int dim, int heads, AttnGeo geo, const Seg* segs,

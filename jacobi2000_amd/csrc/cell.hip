@@ -88,7 +88,7 @@ void cell_setup(Model& m, hipStream_t s, const CellDims& d, const float* d_meta,
     m.p_tables_batch = -1;  // the patch2d tables share these slots
     if (c.T != d.T || c.ny != d.ny || c.nx != d.nx) {
         c.T = -1;
-        run.tab.build({d}, attn_tokens_per_block(m.cfg.heads), false);
+        run.tab.build({d}, attn_tokens_per_block(m.cfg.dim), false);
         HIPCHK(hipStreamSynchronize(s));
         Seg* segs = m.get<Seg>(S_SEGS, run.tab.segs.size());
         Tile* tiles = m.get<Tile>(S_TILES, run.tab.tiles.size());

@@ -8,6 +8,14 @@
 //
 // Loaders and epilogues are functors so one kernel body serves the plain
 // GEMMs (activations x weights), the patch embed and the implicit-GEMM conv.
+//
+// A launch of fewer than 128 tiles (the reference forward's 256-token
+// products and its CellHead convolutions) splits K over blockIdx.z when the
+// caller gives a workspace: each split stores its partial tile, and a second
+// kernel sums the partials in split order and applies the epilogue, so the
+// result does not depend on scheduling. Larger launches are one pass.
+#include <algorithm>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -120,9 +128,12 @@ struct EpiConv {
 };
 
 // A is either a plain row-major matrix (LoadA) or, for the conv, the weight
-// [cout, K]; B is a plain weight (LoadB) or the im2col view.
+// [cout, K]; B is a plain weight (LoadB) or the im2col view. With part,
+// split z covers k in [z kc, (z + 1) kc) and stores its raw tile to
+// part[z][M][N] instead of applying the epilogue.
 template <class AL, class BL, class EP>
-__global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, AL la, BL lb, EP ep) {
+__global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, AL la, BL lb, EP ep, int kc,
+                                                  float* __restrict__ part) {
     __shared__ __align__(16) float As[BK][BM + PAD];
     __shared__ __align__(16) float Bs[BK][BN + PAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -136,7 +147,8 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, AL la, BL
 
     const int ar = tid >> 2, ak = (tid & 3) * 4;    // A tile: row, 4 consecutive k
     const int bk = tid >> 4, bn = (tid & 15) * 4;   // B tile: k row, 4 consecutive n
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);  // kc is a multiple of BK
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         float4 a = la(m0 + ar, k0 + ak);
         float4 b = lb(k0 + bk, n0 + bn);
         As[ak + 0][ar] = a.x;
@@ -168,45 +180,85 @@ __global__ __launch_bounds__(256) void k_gemm_f32(int M, int N, int K, AL la, BL
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-                if (m < M && n < N) ep(m, n, acc[i][j][r]);
+                if (m >= M || n >= N) continue;
+                if (part) part[((long)blockIdx.z * M + m) * N + n] = acc[i][j][r];
+                else ep(m, n, acc[i][j][r]);
             }
         }
+}
+
+// The split-K partials of one launch summed in split order, then the epilogue.
+template <class EP>
+__global__ __launch_bounds__(256) void k_splitk_reduce(int M, int N, int splits, const float* __restrict__ part,
+                                                       EP ep) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x, MN = (long)M * N;
+    if (i >= MN) return;
+    float v = part[i];
+    for (int z = 1; z < splits; z++) v += part[z * MN + i];
+    ep((int)(i / N), (int)(i % N), v);
+}
+
+// Splits of K for a launch of `tiles` tiles (1: one pass), each a multiple
+// of BK and at least 32 deep, so that about 256 blocks run; *kc receives the
+// depth of one split. Needs splits * M * N floats of workspace.
+static int splitk_of(int tiles, int K, int* kc) {
+    int s = 1;
+    if (tiles < 128 && K >= 64) s = std::max(1, std::min(256 / tiles, K / 32));
+    *kc = (cdiv(K, s) + BK - 1) / BK * BK;
+    return cdiv(K, *kc);
+}
+
+template <class AL, class BL, class EP>
+static void launch_gemm(hipStream_t s, int M, int N, int K, const AL& la, const BL& lb, const EP& ep, float* ws,
+                        size_t ws_floats, const char* what) {
+    const int tm = cdiv(M, BM), tn = cdiv(N, BN);
+    int kc = K;
+    int splits = splitk_of(tm * tn, K, &kc);
+    if (!ws || (size_t)splits * M * N > ws_floats) splits = 1, kc = (K + BK - 1) / BK * BK;
+    dim3 grid(tn, tm, splits);
+    hipLaunchKernelGGL((k_gemm_f32<AL, BL, EP>), grid, dim3(256), 0, s, M, N, K, la, lb, ep, kc,
+                       splits > 1 ? ws : nullptr);
+    if (splits > 1) {
+        const long MN = (long)M * N;
+        hipLaunchKernelGGL((k_splitk_reduce<EP>), dim3(cdiv(MN, 256)), dim3(256), 0, s, M, N, splits, ws, ep);
+    }
+    LAUNCHCHK(what);
+}
+
+size_t gemm_f32_workspace(int M, int N, int K) {
+    int kc = K;
+    const int splits = splitk_of(cdiv(M, BM) * cdiv(N, BN), K, &kc);
+    return splits > 1 ? (size_t)splits * M * N : 0;
 }
 
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 void gemm_f32(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, const float* bias, float* C,
-              int ldc, int epi) {
+              int ldc, int epi, float* ws, size_t ws_floats) {
     if (M <= 0 || N <= 0) return;
     LoadA la{A, lda, M, K, aligned16(A) && lda % 4 == 0};
     LoadB lb{B, N, K, N, aligned16(B) && N % 4 == 0};
     EpiPlain ep{bias, C, ldc, epi};
-    dim3 grid(cdiv(N, BN), cdiv(M, BM));
-    hipLaunchKernelGGL((k_gemm_f32<LoadA, LoadB, EpiPlain>), grid, dim3(256), 0, s, M, N, K, la, lb, ep);
-    LAUNCHCHK("gemm_f32");
+    launch_gemm(s, M, N, K, la, lb, ep, ws, ws_floats, "gemm_f32");
 }
 
 void gemm_embed_p(hipStream_t s, int M, int N, int K, const float* A, const float* B, const float* bias,
-                  const float* pos, int tokens, const float* mvec, float* C) {
+                  const float* pos, int tokens, const float* mvec, float* C, float* ws, size_t ws_floats) {
     LoadA la{A, K, M, K, aligned16(A) && K % 4 == 0};
     LoadB lb{B, N, K, N, aligned16(B) && N % 4 == 0};
     EpiEmbed ep{bias, pos, mvec, C, N, tokens};
-    dim3 grid(cdiv(N, BN), cdiv(M, BM));
-    hipLaunchKernelGGL((k_gemm_f32<LoadA, LoadB, EpiEmbed>), grid, dim3(256), 0, s, M, N, K, la, lb, ep);
-    LAUNCHCHK("gemm_embed_p");
+    launch_gemm(s, M, N, K, la, lb, ep, ws, ws_floats, "gemm_embed_p");
 }
 
 void conv_f32(hipStream_t s, int B, int cin, int h, int w, const float* in, const float* in2, int cin1,
               const float* wt, const float* bias, int cout, int k, float* out, int gelu_on, const float* add_src,
-              const float* /*add_out*/) {
+              float* ws, size_t ws_floats) {
     const int K = cin * k * k, N = B * h * w, M = cout;
     if (!in2) cin1 = cin;
     LoadA la{wt, K, M, K, aligned16(wt) && K % 4 == 0};
     LoadIm2col lb{in, in2, cin, cin1, h, w, k, K, N, (long)cin1 * h * w, (long)(cin - cin1) * h * w};
     EpiConv ep{bias, out, add_src, cout, h * w, gelu_on};
-    dim3 grid(cdiv(N, BN), cdiv(M, BM));
-    hipLaunchKernelGGL((k_gemm_f32<LoadA, LoadIm2col, EpiConv>), grid, dim3(256), 0, s, M, N, K, la, lb, ep);
-    LAUNCHCHK("conv_f32");
+    launch_gemm(s, M, N, K, la, lb, ep, ws, ws_floats, "conv_f32");
 }
 
 }  // namespace j2k

@@ -104,11 +104,15 @@ void rows_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, 
 
 // ---- GEMM, f32 operands on v_mfma_f32_16x16x4_f32 ------------------------
 // C[M,N] (ldc) = A[M,K] (lda) @ B[K,N] (row-major, ldb = N) with epilogue.
+// A launch of few tiles splits K when ws holds gemm_f32_workspace(M, N, K)
+// floats (same result on every run; without ws it runs in one pass).
 void gemm_f32(hipStream_t s, int M, int N, int K, const float* A, int lda, const float* B, const float* bias, float* C,
-              int ldc, int epi);
+              int ldc, int epi, float* ws = nullptr, size_t ws_floats = 0);
+size_t gemm_f32_workspace(int M, int N, int K);
 // Form P embed: X[b*tokens + t] = patches @ We + be + pos[t] + mvec.
 void gemm_embed_p(hipStream_t s, int M, int N, int K, const float* A, const float* B, const float* bias,
-                  const float* pos, int tokens, const float* mvec, float* C);
+                  const float* pos, int tokens, const float* mvec, float* C, float* ws = nullptr,
+                  size_t ws_floats = 0);
 // Zero-padded cross-correlation (cpu.rs:171-205) as an implicit GEMM:
 // out[b, co, y, x] = bias[co] + sum w[co, ci, ky, kx] in[b, ci, y+ky-k/2, x+kx-k/2],
 // in [B, cin, h, w] (channel stride `in_bstride` floats per batch item),
@@ -117,7 +121,7 @@ void gemm_embed_p(hipStream_t s, int M, int N, int K, const float* A, const floa
 // concatenated input (cpu.rs:340-342) needs no copy.
 void conv_f32(hipStream_t s, int B, int cin, int h, int w, const float* in, const float* in2, int cin1,
               const float* wt, const float* bias, int cout, int k, float* out, int gelu, const float* add_src,
-              const float* add_out);
+              float* ws = nullptr, size_t ws_floats = 0);
 
 // ---- LayerNorm (cpu.rs:49-65): one wave per row; f32 and/or bf16 out. ----
 void layernorm(hipStream_t s, int rows, int dim, const float* x, const float* g, const float* b, float* out,
@@ -146,7 +150,7 @@ void attention_f32(hipStream_t s, const float* qkv, float* out, int dim, int hea
                    const Tile* tiles, int n_tiles, int tokens_per_block);
 void attention_bf16(hipStream_t s, const uint16_t* qkv, uint16_t* out, int dim, int heads, AttnGeo geo,
                     const Seg* segs, const Tile* tiles, int n_tiles, int tokens_per_block);
-int attn_tokens_per_block(int heads);
+int attn_tokens_per_block(int dim);  // tokens of one block of the generic attention (all heads)
 
 // cell_st attention with K/V staged in LDS: one block per 8 x 8 tile of cells
 // of one time slice ({segment, t, y0, x0}), periodic halo `rs`, time window

@@ -28,7 +28,7 @@ struct LayerW {
 enum Slot : int {
     S_X, S_H, S_HB, S_QKV, S_O, S_F, S_XN, S_T0, S_T1, S_Z0, S_Z2, S_C1, S_PATCH, S_MVEC,
     S_WIN_A, S_WIN_B, S_SUMS, S_SEGS, S_TILES, S_SLICES, S_X0, S_XA, S_XB, S_TARG, S_RESID, S_GRID, S_KNOWN, S_META,
-    S_TOKREQ, S_CTILES, S_COUNT
+    S_TOKREQ, S_CTILES, S_SPLITK, S_COUNT
 };
 
 // Kernel classes timed with HIP events when timing is on (the bench's

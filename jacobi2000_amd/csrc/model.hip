@@ -1,4 +1,5 @@
 // Model construction (weights to HBM) and the two forward passes' shared parts.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -409,7 +410,7 @@ void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPl
                                 ap.ctiles, ap.n_ctiles, mode);
         return;
     }
-    const int tpb = attn_tokens_per_block(H);
+    const int tpb = attn_tokens_per_block(d);
     if (sizeof(Act) == 4)
         attention_f32(s, (const float*)QKV, (float*)O, d, H, geo, ap.segs, ap.tiles, ap.n_tiles, tpb);
     else
@@ -605,6 +606,10 @@ bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
         float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
         float* O = m.get<float>(S_O, (size_t)M * d);
         float* F = m.get<float>(S_F, (size_t)M * ff);
+        // split-K workspace for the products of few tiles (small packs)
+        const size_t wsn = std::max({gemm_f32_workspace(M, 3 * d, d), gemm_f32_workspace(M, d, d),
+                                     gemm_f32_workspace(M, ff, d), gemm_f32_workspace(M, d, ff)});
+        float* ws = wsn ? m.get<float>(S_SPLITK, wsn) : nullptr;
         const double ln_b = (double)M * d * 8, at_b = 16.0 * M * d, g = 2.0 * M * d;
         for (int l = 0; l < m.cfg.layers; l++) {
             const LayerW& L = m.L[l];
@@ -614,7 +619,7 @@ bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
             }
             {
                 TScope t(m, s, KC_GEMM, g * 3 * d);
-                gemm_f32(s, M, 3 * d, d, Hh, d, L.wqkv, L.bqkv, QKV, 3 * d, EPI_STORE);
+                gemm_f32(s, M, 3 * d, d, Hh, d, L.wqkv, L.bqkv, QKV, 3 * d, EPI_STORE, ws, wsn);
             }
             {
                 TScope t(m, s, KC_ATTN, at_b);
@@ -622,7 +627,7 @@ bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
             }
             {
                 TScope t(m, s, KC_GEMM, g * d);
-                gemm_f32(s, M, d, d, O, d, L.wp, L.bp, X, d, EPI_RESID);
+                gemm_f32(s, M, d, d, O, d, L.wp, L.bp, X, d, EPI_RESID, ws, wsn);
             }
             {
                 TScope t(m, s, KC_NORM, ln_b);
@@ -630,11 +635,11 @@ bool trunk(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& 
             }
             {
                 TScope t(m, s, KC_GEMM, g * ff);
-                gemm_f32(s, M, ff, d, Hh, d, L.w1, L.b1, F, ff, EPI_GELU);
+                gemm_f32(s, M, ff, d, Hh, d, L.w1, L.b1, F, ff, EPI_GELU, ws, wsn);
             }
             {
                 TScope t(m, s, KC_GEMM, g * ff);
-                gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID);
+                gemm_f32(s, M, d, ff, F, ff, L.w2, L.b2, X, d, EPI_RESID, ws, wsn);
             }
         }
         return false;
@@ -653,13 +658,19 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
     float* X = m.get<float>(S_X, (size_t)M * d);
     patchify(s, B, c.channels, c.ny, c.nx, c.patch, d_state, patches);
     meta_linear(s, c.meta_dim, d, d_meta, m.meta_w, m.meta_b, mvec);
-    gemm_embed_p(s, M, d, c.patch_dim, patches, m.embed_w, m.embed_b, m.pos, tokens, mvec, X);
+    // split-K workspace of the embedding and the CellHead convolutions (few tiles at 256 tokens)
+    const int B_cells = B * cells, B_tok = B * tokens;
+    const size_t wsn = std::max({gemm_f32_workspace(M, d, c.patch_dim), gemm_f32_workspace(d, B_tok, d * 9),
+                                 gemm_f32_workspace(hd, B_tok, d), gemm_f32_workspace(hd, B_cells, (hd + c.channels) * 9),
+                                 gemm_f32_workspace(c.channels, B_cells, hd * 9)});
+    float* ws = wsn ? m.get<float>(S_SPLITK, wsn) : nullptr;
+    gemm_embed_p(s, M, d, c.patch_dim, patches, m.embed_w, m.embed_b, m.pos, tokens, mvec, X, ws, wsn);
 
     if (m.p_tables_batch != B) {
         std::vector<Seg> segs(B);
         for (int b = 0; b < B; b++) segs[b] = Seg{b * tokens, 1, c.tokens_y, c.tokens_x};
         std::vector<Tile> tiles;
-        build_tiles(segs, attn_tokens_per_block(c.heads), tiles);
+        build_tiles(segs, attn_tokens_per_block(c.dim), tiles);
         HIPCHK(hipStreamSynchronize(s));
         Seg* ds = m.get<Seg>(S_SEGS, segs.size());
         Tile* dt = m.get<Tile>(S_TILES, tiles.size());
@@ -684,12 +695,13 @@ void forward_p(Model& m, hipStream_t s, int B, const float* d_state, const float
     float* c1 = m.get<float>(S_C1, (size_t)B * hd * cells);
     layernorm(s, M, d, X, m.norm_g, m.norm_b, xn, nullptr);
     transpose_td(s, B, tokens, d, xn, t0);
-    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t0, nullptr, d, m.tm_w, m.tm_b, d, 3, t1, 1, t0, nullptr);
-    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t1, nullptr, d, m.tc_w, m.tc_b, hd, 1, z0, 0, nullptr, nullptr);
+    ws = wsn ? m.get<float>(S_SPLITK, wsn) : nullptr;  // the trunk may have grown the slot
+    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t0, nullptr, d, m.tm_w, m.tm_b, d, 3, t1, 1, t0, ws, wsn);
+    conv_f32(s, B, d, c.tokens_y, c.tokens_x, t1, nullptr, d, m.tc_w, m.tc_b, hd, 1, z0, 0, nullptr, ws, wsn);
     upsample2(s, B, hd, c.tokens_y, c.tokens_x, c.patch, z0, z2, (long)hd * cells);
-    conv_f32(s, B, hd + c.channels, c.ny, c.nx, z2, d_state, hd, m.c1_w, m.c1_b, hd, 3, c1, 1, nullptr, nullptr);
+    conv_f32(s, B, hd + c.channels, c.ny, c.nx, z2, d_state, hd, m.c1_w, m.c1_b, hd, 3, c1, 1, nullptr, ws, wsn);
     conv_f32(s, B, hd, c.ny, c.nx, c1, nullptr, hd, m.c2_w, m.c2_b, c.channels, 3, d_out, 0,
-             add_state ? d_state : nullptr, nullptr);
+             add_state ? d_state : nullptr, ws, wsn);
 }
 
 }  // namespace j2k

@@ -10,6 +10,7 @@
 // the region and leaves its contents behind; nothing on this path needs it
 // cleared (every buffer is written before it is read), and calls that
 // overlap on one device race.
+#include <algorithm>
 #include <mutex>
 #include <string>
 
@@ -64,6 +65,11 @@ size_t bytes_needed(const Model& m, const CellTables& t) {
             + aligned(3 * M * d * act)  // q, k, v
             + aligned(M * d * act)      // attention output
             + (rows ? 0 : aligned(M * ff * act));  // MLP intermediate (on chip in the row-panel kernel)
+    if (m.cfg.precision == J2K_PREC_F32) {  // split-K partials of the products of few tiles
+        const size_t ws = std::max({gemm_f32_workspace((int)M, (int)(3 * d), (int)d), gemm_f32_workspace((int)M, (int)d, (int)d),
+                                    gemm_f32_workspace((int)M, (int)ff, (int)d), gemm_f32_workspace((int)M, (int)d, (int)ff)});
+        if (ws) need += aligned(ws * sizeof(float));
+    }
     return need;
 }
 
@@ -96,7 +102,7 @@ extern "C" int j2k_cell_forward_dev_noalloc(void* h, const float* d_grid, const 
         DeviceGuard dg(m.device);
         unsigned char* base = region_of(m.device);
         CellTables t;
-        t.build({dims}, attn_tokens_per_block(m.cfg.heads), false);
+        t.build({dims}, attn_tokens_per_block(m.cfg.dim), false);
         const size_t need = bytes_needed(m, t);
         if (need > J2K_NOALLOC_MAX_BYTES)
             fail(J2K_ERR_OOM, "j2k_cell_forward_dev_noalloc: a " + std::to_string(T) + "x" + std::to_string(ny) + "x" +

@@ -284,7 +284,7 @@ struct Server {
         for (auto& r : active) dims.push_back(r->dims);
         const bool multi = active.size() > 1;
         const double r0 = now_s();
-        tab.build(dims, attn_tokens_per_block(m.cfg.heads), false);  // tok_req is made on the device
+        tab.build(dims, attn_tokens_per_block(m.cfg.dim), false);  // tok_req is made on the device
         const double r1 = now_s();
         const size_t M = (size_t)tab.M;
         const bool grow = M > cap_tok;

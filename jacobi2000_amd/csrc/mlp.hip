@@ -44,9 +44,22 @@ typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef unsigned u4v __attribute__((ext_vector_type(4)));
 typedef unsigned u2v __attribute__((ext_vector_type(2)));
 
-constexpr int RB = 64, RD = 128, RLD = RD + 8;
+constexpr int RB = 64, RD = 128;
 
 __device__ __forceinline__ bf8v frag(const uint16_t* p) { return __builtin_bit_cast(bf8v, *reinterpret_cast<const uint4*>(p)); }
+
+// An LDS panel (64 token rows of bf16, any number of columns) is stored as
+// 16-byte chunks of 8 columns, chunk-major, the 64 rows of chunk ch permuted
+// by row ^ (ch & 7): element (row, col) sits at pidx(row, col). An MFMA
+// operand read (16 lanes, 16 rows of one chunk) is then one 256-byte span
+// (conflict-free on gfx950's 64 banks; row-major rows of 272 bytes put the
+// 8 rows of each 8-lane half on 8 lines and take two passes), and the eight
+// chunks of one row that a staging store writes land on distinct banks. A
+// wave's first row (a multiple of 32) is a plain offset of wm * 8.
+__device__ __forceinline__ int pidx(int row, int col) {
+    const int ch = col >> 3;
+    return ((ch * RB + (row ^ (ch & 7))) << 3) | (col & 7);
+}
 
 // A raw buffer over `bytes` from p: loads past the end read 0, stores past it
 // are dropped.
@@ -77,18 +90,24 @@ __device__ __forceinline__ void load_w(bf8v (&w)[K4][J], const uint16_t* Wf, int
         for (int j = 0; j < J; j++) w[k4][j] = frag(p + ((long)j * (ldk / 32) + k4) * 512);
 }
 
-// acc += (A (16*MI x 32*K4 in LDS with row stride LD, hi and lo) @ the loaded
-// W chunk), transposed: acc[i][j][r] is A's row i*16 + r16, column
+// acc += (A (16*MI x 32*K4, an LDS panel, hi and lo) @ the loaded W chunk),
+// transposed: acc[i][j][r] is A's row i*16 + r16, column
 // wn + j*16 + 4*(lane/16) + r.
-template <bool SPLIT, int MI, int J, int K4 = 4, int LD = RLD>
+template <bool SPLIT, int MI, int J, int K4 = 4>
 __device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah, const uint16_t* Al,
                                           const bf8v (&w)[K4][J], int r16, int kq) {
+    // pidx(i*16 + r16, k4*32 + kq) is a lane base (one for even k4, one for
+    // odd: the row permutation of chunk k4*4 + kq/8) plus a constant
+    // (odd k4 flips row bit 2, element bit 5)
+    const int sq = kq >> 3;
+    const int b0 = (sq * RB + (r16 ^ sq)) << 3;
+    auto at = [&](int k4, int i) { return (k4 & 1 ? b0 ^ 32 : b0) + ((k4 * 4 * RB + i * 16) << 3); };
     // the operands of step k4 + 1 are read while step k4 multiplies
     bf8v a[2][MI], l[2][MI];
 #pragma unroll
     for (int i = 0; i < MI; i++) {
-        a[0][i] = frag(&Ah[(i * 16 + r16) * LD + kq]);
-        if (SPLIT) l[0][i] = frag(&Al[(i * 16 + r16) * LD + kq]);
+        a[0][i] = frag(&Ah[at(0, i)]);
+        if (SPLIT) l[0][i] = frag(&Al[at(0, i)]);
     }
     __builtin_amdgcn_s_setprio(1);  // the other wave on the SIMD fills the MFMA gaps
 #pragma unroll
@@ -97,8 +116,8 @@ __device__ __forceinline__ void mma_panel(f4v (&acc)[MI][J], const uint16_t* Ah,
         if (k4 + 1 < K4)
 #pragma unroll
             for (int i = 0; i < MI; i++) {
-                a[b ^ 1][i] = frag(&Ah[(i * 16 + r16) * LD + (k4 + 1) * 32 + kq]);
-                if (SPLIT) l[b ^ 1][i] = frag(&Al[(i * 16 + r16) * LD + (k4 + 1) * 32 + kq]);
+                a[b ^ 1][i] = frag(&Ah[at(k4 + 1, i)]);
+                if (SPLIT) l[b ^ 1][i] = frag(&Al[at(k4 + 1, i)]);
             }
 #pragma unroll
         for (int i = 0; i < MI; i++)
@@ -119,16 +138,17 @@ __device__ __forceinline__ void zero(f4v (&acc)[MI][J]) {
         for (int j = 0; j < J; j++) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 }
 
-// Four consecutive columns of one row as A operands: bf16, and with SPLIT
-// the bf16 remainder.
-template <bool SPLIT, int LD = RLD>
+// Four consecutive columns (col a multiple of 4) of one panel row as A
+// operands: bf16, and with SPLIT the bf16 remainder.
+template <bool SPLIT>
 __device__ __forceinline__ void put_a4(uint16_t* Ah, uint16_t* Al, int row, int col, f4v v) {
     const u2v h = bf4(v);
-    *reinterpret_cast<u2v*>(&Ah[row * LD + col]) = h;
+    const int o = pidx(row, col);
+    *reinterpret_cast<u2v*>(&Ah[o]) = h;
     if (SPLIT) {
         const f4v r = {v[0] - __uint_as_float(h[0] << 16), v[1] - __uint_as_float(h[0] & 0xffff0000u),
                        v[2] - __uint_as_float(h[1] << 16), v[3] - __uint_as_float(h[1] & 0xffff0000u)};
-        *reinterpret_cast<u2v*>(&Al[row * LD + col]) = bf4(r);
+        *reinterpret_cast<u2v*>(&Al[o]) = bf4(r);
     }
 }
 
@@ -142,7 +162,7 @@ __device__ __forceinline__ void stage_rows(const void* src, int m0, int rows, ui
         for (int q = 0; q < 1024 / NT; q++) {
             const int v = tid + NT * q, row = v >> 4, kv = v & 15;
             const u4v val = __builtin_amdgcn_raw_buffer_load_b128(r, (row * RD + kv * 8) * 2, 0, 0);
-            *reinterpret_cast<u4v*>(&Ah[row * RLD + kv * 8]) = val;
+            *reinterpret_cast<u4v*>(&Ah[pidx(row, kv * 8)]) = val;
         }
     } else {
         const auto r = rows_rsrc(static_cast<const float*>(src) + (long)m0 * RD, rows * RD * 4);
@@ -250,7 +270,7 @@ __device__ __forceinline__ void readout_rows(const f4v (&xm)[MI][J], int wm, int
 // for every product; WM = WN = 2 halves those reads per MFMA and loads twice
 // the weight fragments; WN = 8 (16 columns) holds half the registers, with
 // twice the waves per block. FC: intermediate columns per MLP chunk; at 64
-// the block's LDS (52.5 KB in the split form) lets three blocks share a CU.
+// the block's LDS (48.8 KB in the split form) lets three blocks share a CU.
 // The arithmetic of an output element is the same in every shape. The
 // LayerNorm row sums borrow LDS that is idle at the time: F's before the
 // MLP, H's after it.
@@ -260,11 +280,11 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     static_assert(!RO || (PROJ && !QKV && !LNOUT), "the readout ends the last block");
     static_assert(!EMB || (PROJ && QKV && WM == 1 && WN == 4), "the embedding enters the first block");
     constexpr int WAVES = WM * WN, MI = 4 / WM, J = 8 / WN, NT = 64 * WAVES, P = SPLIT ? 2 : 1;
-    constexpr int FLD = FC + 8, JF = FC / (16 * WN), K4F = FC / 32;  // F row stride, ff1 tiles, ff2 K steps
+    constexpr int JF = FC / (16 * WN), K4F = FC / 32;  // ff1 tiles, ff2 K steps
     static_assert(JF >= 1 && FC % 32 == 0, "each wave owns at least one 16-column tile of an MLP chunk");
-    static_assert(RB * FLD * P * 2 >= 8 * RB * 16, "the LayerNorm row sums fit in F");
-    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
-    __shared__ __align__(16) uint16_t Fs[P * RB * FLD];
+    static_assert(RB * FC * P * 2 >= 8 * RB * 16, "the LayerNorm row sums fit in F");
+    __shared__ __align__(16) uint16_t Hs[P * RB * RD];
+    __shared__ __align__(16) uint16_t Fs[P * RB * FC];
     __shared__ float mrow[RB], irow[RB];
     __shared__ float ys[RO ? 3 : 1][RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -272,8 +292,8 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     const int m0 = blockIdx.x * RB, wn = wnx * 16 * J, wf = wnx * (FC / WN), rows = min(RB, M - m0);
     const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
     // the wave's tokens of H and F (split: hi, lo)
-    uint16_t *const Hh = Hs + wm * RLD, *const Hl = Hs + (SPLIT ? RB * RLD : 0) + wm * RLD;
-    uint16_t *const Fh = Fs + wm * FLD, *const Fl = Fs + (SPLIT ? RB * FLD : 0) + wm * FLD;
+    uint16_t *const Hh = Hs + wm * 8, *const Hl = Hs + (SPLIT ? RB * RD : 0) + wm * 8;
+    uint16_t *const Fh = Fs + wm * 8, *const Fl = Fs + (SPLIT ? RB * FC : 0) + wm * 8;
     const auto rX = rows_rsrc(a.X + (long)m0 * RD, rows * RD * 4);
     // byte offset of the lane's four columns of tile (i, j) in a [64][ld] f32 panel
     auto at = [&](int i, int j, int ld) { return ((wm + i * 16 + r16) * ld + wn + j * 16 + c4) * 4; };
@@ -286,7 +306,7 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
     f4v xk[KEEP ? MI : 1][J];
 
     // ---- H, the MLP's input: LN2 of the projected residual, or Hin
-    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hs, Hs + (SPLIT ? RB * RLD : 0), tid);
+    stage_rows<SPLIT, NT>(PROJ ? a.O : a.Hin, m0, rows, Hs, Hs + (SPLIT ? RB * RD : 0), tid);
     if (PROJ) load_w(w1, a.Wp, RD, 0, 0, wn, lane);
     __syncthreads();
     if (PROJ) {
@@ -354,10 +374,10 @@ __global__ __launch_bounds__(64 * WM * WN, (WM * WN == 4 && FC == 64) ? 3 : WM *
                 f4v v;
 #pragma unroll
                 for (int r = 0; r < 4; r++) v[r] = gelu_fast(acc1[i][j][r] + bb[j][r]);
-                put_a4<SPLIT, FLD>(Fh, Fl, i * 16 + r16, wf + j * 16 + c4, v);
+                put_a4<SPLIT>(Fh, Fl, i * 16 + r16, wf + j * 16 + c4, v);
             }
         __syncthreads();
-        mma_panel<SPLIT, MI, J, K4F, FLD>(acc2, Fh, Fl, w2, r16, kq);
+        mma_panel<SPLIT, MI, J, K4F>(acc2, Fh, Fl, w2, r16, kq);
     }
     f4v xm[MI][J];
 #pragma unroll
@@ -442,12 +462,12 @@ __global__ __launch_bounds__(256, 2) void k_rows_embed_qkv(int M, const float4* 
                                                            const float* __restrict__ mvec,
                                                            const int* __restrict__ tok_req, RowsArgs a) {
     constexpr int J = 2, P = SPLIT ? 2 : 1;
-    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
+    __shared__ __align__(16) uint16_t Hs[P * RB * RD];
     __shared__ float mrow[RB], irow[RB];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.x * RB, wn = wave * 32, rows = min(RB, M - m0);
     const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
-    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RD : 0);
     bf8v w[4][J];
     load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
     if (blockIdx.x == 0)
@@ -530,11 +550,11 @@ __global__ __launch_bounds__(256, 2) void k_rows_readout(int M, const float* __r
 template <bool SPLIT>
 __global__ __launch_bounds__(256, 2) void k_rows_qkv(int M, RowsArgs a) {
     constexpr int J = 2, P = SPLIT ? 2 : 1;
-    __shared__ __align__(16) uint16_t Hs[P * RB * RLD];
+    __shared__ __align__(16) uint16_t Hs[P * RB * RD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.x * RB, wn = wave * 32, rows = min(RB, M - m0);
     const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
-    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RLD : 0);
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * RD : 0);
     bf8v w[4][J];
     load_w(w, a.Wqkv, RD, 0, 0, wn, lane);
     stage_rows<SPLIT, 256>(a.Hin, m0, rows, Hh, Hl, tid);

@@ -35,10 +35,14 @@ constexpr int TBN = 128, TBK = 64, TLD = TBK + 8;
 // 4*(lane/16) + r, column wn + j*16 + lane%16) are spread over the block's
 // four waves: v = acc * alpha + bias, then store / GELU + store / residual
 // add into X / residual add and the LayerNorm of the updated rows. alpha is
-// a[n] * colscale[n] with colscale per output column (nullptr: 1). C and H
+// a * colscale[n] with colscale per output column (nullptr: 1), and with RS
+// also times the row's scale rowmax[m] / 448 (1 for a row of zeros). C and H
 // are f32 (OUTF32) or bf16.
-template <int EPI, bool OUTF32, int BM, int J>
-__device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const float* __restrict__ colscale, int M,
+__device__ __forceinline__ float row_scale(float amax) { return amax > 0.f ? amax / 448.0f : 1.0f; }
+
+template <int EPI, bool OUTF32, int BM, int J, bool RS = false>
+__device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const float* __restrict__ colscale,
+                                              const float* __restrict__ rowmax, int M,
                                               int N, int m0, int n0, int wm, int wn, int lane,
                                               const float* __restrict__ bias, void* __restrict__ Cv,
                                               float* __restrict__ X, const float* __restrict__ lng,
@@ -61,10 +65,11 @@ __device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const f
             for (int r = 0; r < 4; r++) {
                 const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
                 if (m >= M) continue;
+                const float rs = RS ? row_scale(rowmax[m]) : 1.0f;
 #pragma unroll
                 for (int j = 0; j < J; j++) {
                     if (!colok[j]) continue;
-                    float v = acc[i][j][r] * acol[j] + bcol[j];
+                    float v = acc[i][j][r] * (RS ? rs * acol[j] : acol[j]) + bcol[j];
                     if (EPI == EPI_GELU) v = gelu_fast(v);
                     const long o = (long)m * N + n0 + wn + j * 16 + r16;
                     if (OUTF32) static_cast<float*>(Cv)[o] = v;
@@ -79,11 +84,12 @@ __device__ __forceinline__ void tile_epilogue(f4v (&acc)[4][J], float a, const f
         for (int r = 0; r < 4; r++) {
             const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
             if (m >= M) continue;
+            const float rs = RS ? row_scale(rowmax[m]) : 1.0f;
 #pragma unroll
             for (int j = 0; j < J; j++) {
                 if (!colok[j]) continue;
                 float* p = X + (long)m * N + n0 + wn + j * 16 + r16;
-                const float v = *p + acc[i][j][r] * acol[j] + bcol[j];
+                const float v = *p + acc[i][j][r] * (RS ? rs * acol[j] : acol[j]) + bcol[j];
                 *p = v;
                 acc[i][j][r] = v;
             }
@@ -223,20 +229,25 @@ __global__ __launch_bounds__(256, 2) void k_tile_bf16(int M, int N, int K, const
         }
     }
 
-    tile_epilogue<EPI, SPLIT, BM, J>(acc, 1.0f, nullptr, M, N, m0, n0, wm, wn, lane, bias, Cv, X, lng, lnb, Hv, red, mrow, irow);
+    tile_epilogue<EPI, SPLIT, BM, J>(acc, 1.0f, nullptr, nullptr, M, N, m0, n0, wm, wn, lane, bias, Cv, X, lng, lnb, Hv, red,
+                                     mrow, irow);
 }
 
-// FP8 form: A is f32 and is quantized while staged, q = clamp(a * inv_sa,
-// +-448) rounded to OCP E4M3 (v_cvt_pk_fp8_f32); Bt holds E4M3 weight codes
-// [N, K]; v_mfma_f32_16x16x32_fp8_fp8 accumulates the exact code products in
-// f32, and the epilogue rescales by sa * colscale[n] (the weights'
-// per-tensor scale, per output column because the fused qkv matrix joins
-// three tensors). Same tiling as the bf16 kernel with 8-byte fragments.
+// FP8 form: A is f32 and is quantized while staged, q = clamp(a * inv_m,
+// +-448) rounded to OCP E4M3 (v_cvt_pk_fp8_f32), with each row's (token's)
+// scale sa_m = rowmax[m] / 448 from its measured maximum |A[m, :]| (written by
+// k_row_amax before this launch; 1 for a row of zeros, inv_m = 1 / sa_m
+// computed as 448 / rowmax[m]): a token's quantization never depends on the
+// other tokens of a pack. Bt holds E4M3 weight codes [N, K];
+// v_mfma_f32_16x16x32_fp8_fp8 accumulates the exact code products in f32,
+// and the epilogue rescales by sa_m * colscale[n] (the weights' per-tensor
+// scale, per output column because the fused qkv matrix joins three
+// tensors). Same tiling as the bf16 kernel with 8-byte fragments.
 constexpr int T8LD = TBK + 16;  // bytes per LDS row
 
 template <int EPI, int BM>
-__global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const float* __restrict__ A, float inv_sa,
-                                                    float sa, const uint8_t* __restrict__ Bt,
+__global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const float* __restrict__ A,
+                                                    const float* __restrict__ rowmax, const uint8_t* __restrict__ Bt,
                                                     const float* __restrict__ colscale, const float* __restrict__ bias,
                                                     float* __restrict__ C, float* __restrict__ X,
                                                     const float* __restrict__ lng, const float* __restrict__ lnb,
@@ -262,6 +273,13 @@ __global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const f
     constexpr int NA = 16 * BM / 256;  // float4 of A per thread per chunk
     float4 pa[NA];
     uint4 pb[2];
+    float inv_sa[NA];  // 1 / scale of the rows this thread stages (the same rows in every chunk)
+#pragma unroll
+    for (int q = 0; q < NA; q++) {
+        const int m = m0 + ((tid + 256 * q) >> 4);
+        const float am = m < M ? rowmax[m] : 0.f;
+        inv_sa[q] = am > 0.f ? 448.0f / am : 1.0f;
+    }
 #define J2K_F8_LOAD(kc)                                                                                        \
     {                                                                                                          \
         _Pragma("unroll") for (int q = 0; q < NA; q++) {                                                       \
@@ -280,10 +298,10 @@ __global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const f
 #pragma unroll
         for (int q = 0; q < NA; q++) {
             const int v = tid + 256 * q, row = v >> 4, kv = v & 15;
-            const float e0 = fminf(fmaxf(pa[q].x * inv_sa, -448.f), 448.f);
-            const float e1 = fminf(fmaxf(pa[q].y * inv_sa, -448.f), 448.f);
-            const float e2 = fminf(fmaxf(pa[q].z * inv_sa, -448.f), 448.f);
-            const float e3 = fminf(fmaxf(pa[q].w * inv_sa, -448.f), 448.f);
+            const float e0 = fminf(fmaxf(pa[q].x * inv_sa[q], -448.f), 448.f);
+            const float e1 = fminf(fmaxf(pa[q].y * inv_sa[q], -448.f), 448.f);
+            const float e2 = fminf(fmaxf(pa[q].z * inv_sa[q], -448.f), 448.f);
+            const float e3 = fminf(fmaxf(pa[q].w * inv_sa[q], -448.f), 448.f);
             int w = __builtin_amdgcn_cvt_pk_fp8_f32(e0, e1, 0, false);
             w = __builtin_amdgcn_cvt_pk_fp8_f32(e2, e3, w, true);
             *reinterpret_cast<int*>(&As[row * T8LD + kv * 4]) = w;
@@ -309,7 +327,8 @@ __global__ __launch_bounds__(256, 2) void k_tile_f8(int M, int N, int K, const f
         }
     }
 #undef J2K_F8_LOAD
-    tile_epilogue<EPI, true, BM, J>(acc, sa, colscale, M, N, m0, n0, wm, wn, lane, bias, C, X, lng, lnb, H, red, mrow, irow);
+    tile_epilogue<EPI, true, BM, J, true>(acc, 1.0f, colscale, rowmax, M, N, m0, n0, wm, wn, lane, bias, C, X, lng, lnb,
+                                          H, red, mrow, irow);
 }
 
 bool gemm_tile_ok(int N, int K, bool split) { return N % 16 == 0 && K % (split ? 4 : 8) == 0; }
@@ -347,31 +366,52 @@ static void tile(hipStream_t s, int M, int N, int K, const void* A, const uint16
 }
 
 template <int BM>
-static void tile_f8_bm(hipStream_t s, int M, int N, int K, const float* A, float sa, const uint8_t* Bt,
+static void tile_f8_bm(hipStream_t s, int M, int N, int K, const float* A, const float* amax_a, const uint8_t* Bt,
                        const float* cs, const float* bias, float* C, float* X, int epi, const float* g,
                        const float* b, float* H) {
     const int mt = cdiv(M, BM), nt = cdiv(N, TBN);
     dim3 grid(mt * nt), block(256);
-    const float inv = 1.0f / sa;
     switch (epi) {
-        case EPI_STORE: hipLaunchKernelGGL((k_tile_f8<EPI_STORE, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
-        case EPI_GELU: hipLaunchKernelGGL((k_tile_f8<EPI_GELU, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
-        case EPI_RESID: hipLaunchKernelGGL((k_tile_f8<EPI_RESID, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
-        default: hipLaunchKernelGGL((k_tile_f8<EPI_RESID_LN, BM>), grid, block, 0, s, M, N, K, A, inv, sa, Bt, cs, bias, C, X, g, b, H, nt); break;
+        case EPI_STORE: hipLaunchKernelGGL((k_tile_f8<EPI_STORE, BM>), grid, block, 0, s, M, N, K, A, amax_a, Bt, cs, bias, C, X, g, b, H, nt); break;
+        case EPI_GELU: hipLaunchKernelGGL((k_tile_f8<EPI_GELU, BM>), grid, block, 0, s, M, N, K, A, amax_a, Bt, cs, bias, C, X, g, b, H, nt); break;
+        case EPI_RESID: hipLaunchKernelGGL((k_tile_f8<EPI_RESID, BM>), grid, block, 0, s, M, N, K, A, amax_a, Bt, cs, bias, C, X, g, b, H, nt); break;
+        default: hipLaunchKernelGGL((k_tile_f8<EPI_RESID_LN, BM>), grid, block, 0, s, M, N, K, A, amax_a, Bt, cs, bias, C, X, g, b, H, nt); break;
     }
 }
 
-void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, float sa, const uint8_t* Bt, const float* cs,
-                  const float* bias, float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H,
-                  int bm) {
+// out[m] = max |A[m, :]| for the M rows of A [M, K] (K a multiple of 4, A
+// 16-byte aligned): one wave per row, four rows a block.
+__global__ __launch_bounds__(256) void k_row_amax(int M, int K, const float* __restrict__ A, float* __restrict__ out) {
+    const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (m >= M) return;
+    const float4* row = reinterpret_cast<const float4*>(A + (long)m * K);
+    float a = 0.f;
+    for (int i = lane; i < K / 4; i += 64) {
+        const float4 v = row[i];
+        a = fmaxf(a, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+    }
+    a = wave_max(a);
+    if (lane == 0) out[m] = a;
+}
+
+void row_amax_f32(hipStream_t s, int M, int K, const float* A, float* out) {
+    if (M <= 0) return;
+    if ((K & 3) || ((uintptr_t)A & 15)) fail(J2K_ERR_INVALID, "row amax: rows must be 16-byte aligned");
+    hipLaunchKernelGGL(k_row_amax, dim3(cdiv(M, 4)), dim3(256), 0, s, M, K, A, out);
+    LAUNCHCHK("row amax");
+}
+
+void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, const float* amax_a, const uint8_t* Bt,
+                  const float* cs, const float* bias, float* C, float* X, int epi, const float* ln_g,
+                  const float* ln_b, float* H, int bm) {
     if (M <= 0) return;
     if (N % 16 || K % 16) fail(J2K_ERR_LIMIT, "gemm fp8 tile: N and K must be multiples of 16");
-    if (!(sa > 0.f)) fail(J2K_ERR_INVALID, "gemm fp8 tile: activation scale must be positive");
+    if (!amax_a) fail(J2K_ERR_INVALID, "gemm fp8 tile: missing the rows' maxima");
     if (((uintptr_t)A & 15) || ((uintptr_t)Bt & 15)) fail(J2K_ERR_INVALID, "gemm fp8 tile: operands must be 16-byte aligned");
     if (epi == EPI_RESID_LN && N != TBN) fail(J2K_ERR_LIMIT, "gemm fp8 tile: the LayerNorm epilogue needs N == 128");
     if (bm <= 0) bm = gemm_tile_default_bm(M, N);
-    if (bm == 64) tile_f8_bm<64>(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
-    else tile_f8_bm<128>(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
+    if (bm == 64) tile_f8_bm<64>(s, M, N, K, A, amax_a, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
+    else tile_f8_bm<128>(s, M, N, K, A, amax_a, Bt, cs, bias, C, X, epi, ln_g, ln_b, H);
     LAUNCHCHK("gemm fp8 tile");
 }
 

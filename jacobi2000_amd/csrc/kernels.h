@@ -30,12 +30,15 @@ void gemm_bf16_tile(hipStream_t s, int M, int N, int K, const uint16_t* A, const
 void gemm_bf16x2_tile(hipStream_t s, int M, int N, int K, const float* A, const uint16_t* Bt, const float* bias,
                       float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H, int bm = 0);
 
-// FP8 form: A f32 quantized to E4M3 with per-tensor scale sa inside the
-// kernel, Bt E4M3 codes [N, K], C = (A_q @ Bt_q) * sa * cs[n] + bias with the
-// same epilogues (C, H f32). N % 16 == 0, K % 16 == 0.
-void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, float sa, const uint8_t* Bt, const float* cs,
-                  const float* bias, float* C, float* X, int epi, const float* ln_g, const float* ln_b, float* H,
-                  int bm = 0);
+// FP8 form: A f32 quantized to E4M3 inside the kernel with each row's scale
+// sa_m = rowmax[m] / 448 (rowmax from row_amax_f32 on the same stream), Bt
+// E4M3 codes [N, K], C = (A_q @ Bt_q) * sa_m * cs[n] + bias with the same
+// epilogues (C, H f32). N % 16 == 0, K % 16 == 0.
+void gemm_f8_tile(hipStream_t s, int M, int N, int K, const float* A, const float* rowmax, const uint8_t* Bt,
+                  const float* cs, const float* bias, float* C, float* X, int epi, const float* ln_g,
+                  const float* ln_b, float* H, int bm = 0);
+// out[m] = max |A[m, :]| over the K columns of each of the M rows of A.
+void row_amax_f32(hipStream_t s, int M, int K, const float* A, float* out);
 
 // ---- row-panel kernel (dim == 128, ff % 128 == 0) --------------------------
 // Per 64-token panel: [X += O @ Wp + bp; H = LN2(X)] if Wp, else H = Hin;

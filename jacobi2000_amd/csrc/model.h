@@ -17,18 +17,16 @@ struct LayerW {
     // the same in MFMA fragment order (the row-panel kernel's operands; d = 128)
     uint16_t *wqkv_f = nullptr, *wp_f = nullptr, *w1_f = nullptr, *w2_f = nullptr;
     // FP8 form: E4M3 codes [N, K], each weight tensor's scale per output
-    // column, and the static per-tensor scales of the four products' inputs
-    // (LN1 output, attention output, LN2 output, GELU output).
+    // column (the inputs' scales are measured per token and launch).
     uint8_t *wqkv_8, *wp_8, *w1_8, *w2_8;
     float *cs_qkv, *cs_p, *cs_1, *cs_2;
-    float s_h1, s_o, s_h2, s_g;
 };
 
 // Scratch slots, each grown on demand (the device is drained before a regrow).
 enum Slot : int {
     S_X, S_H, S_HB, S_QKV, S_O, S_F, S_XN, S_T0, S_T1, S_Z0, S_Z2, S_C1, S_PATCH, S_MVEC,
     S_WIN_A, S_WIN_B, S_SUMS, S_SEGS, S_TILES, S_SLICES, S_X0, S_XA, S_XB, S_TARG, S_RESID, S_GRID, S_KNOWN, S_META,
-    S_TOKREQ, S_CTILES, S_SPLITK, S_COUNT
+    S_TOKREQ, S_CTILES, S_SPLITK, S_AMAX, S_COUNT
 };
 
 // Kernel classes timed with HIP events when timing is on (the bench's

@@ -225,41 +225,6 @@ struct Uploader {
     }
 };
 
-// Static bounds of the FP8 products' inputs (so their per-tensor scales
-// never saturate): an LN output obeys |h| <= max|gamma| sqrt(d-1) + max|beta|
-// (one element of d values of unit biased variance is at most sqrt(d-1)); a
-// product input z = h W + b obeys |z_n| <= bound(h) sum_k |W[k][n]| + |b_n|
-// (W as quantized, the weights the product uses);
-// the attention output is a convex combination of v rows; |GELU(z)| <=
-// max(bound(z), 0.17).
-float max_abs(const Tensor& t) {
-    float a = 0.f;
-    for (float v : t.data) a = std::max(a, std::fabs(v));
-    return a;
-}
-// A weight tensor as the FP8 products see it: its E4M3 codes times its
-// per-tensor scale (the tensor itself when the checkpoint stored codes).
-Tensor as_f8(const Tensor& t) {
-    if (!t.f8.empty()) return t;
-    Tensor q = t;
-    float amax = 0.f;
-    for (float v : t.data) amax = std::max(amax, std::fabs(v));
-    const float sc = amax > 0.f ? amax / 448.0f : 1.0f;
-    for (float& v : q.data) v = e4m3_decode(e4m3_encode(v / sc)) * sc;
-    return q;
-}
-float ln_bound(const Tensor& g, const Tensor& b, int d) { return max_abs(g) * std::sqrt((float)(d - 1)) + max_abs(b); }
-float lin_bound(float hb, const Tensor& w, const Tensor& b) {
-    const int K = (int)w.shape[0], N = (int)w.shape[1];
-    float worst = 0.f;
-    for (int n = 0; n < N; n++) {
-        float s = 0.f;
-        for (int k = 0; k < K; k++) s += std::fabs(w.data[(size_t)k * N + n]);
-        worst = std::max(worst, hb * s + std::fabs(b.data[n]));
-    }
-    return worst;
-}
-
 }  // namespace
 
 Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int device, int precision) {
@@ -340,14 +305,6 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
             L.wp_8 = up.f8_t({&pw}, d, &L.cs_p);
             L.w1_8 = up.f8_t({&w1}, d, &L.cs_1);
             L.w2_8 = up.f8_t({&w2}, ff, &L.cs_2);
-            const float bh1 = ln_bound(get(w, p + "norm1.gamma"), get(w, p + "norm1.beta"), d);
-            const float bh2 = ln_bound(get(w, p + "norm2.gamma"), get(w, p + "norm2.beta"), d);
-            const float bo = lin_bound(bh1, as_f8(v), get(w, p + "v.bias"));
-            const float bz = lin_bound(bh2, as_f8(w1), get(w, p + "ff1.bias"));
-            L.s_h1 = bh1 / 448.0f;
-            L.s_o = bo / 448.0f;
-            L.s_h2 = bh2 / 448.0f;
-            L.s_g = std::max(bz, 0.17f) / 448.0f;
         } else if (precision != J2K_PREC_F32) {
             L.wqkv_t = up.bf16_t(wqkv, d, 3 * d);
             L.wp_t = up.bf16_t(get(w, p + "proj.weight").data, d, d);
@@ -541,21 +498,24 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
 }
 
 // The trunk in the FP8 form: activations f32 in HBM, the four block
-// products on E4M3 codes (tile GEMM quantizing its A operand), attention,
-// LayerNorm and residual in f32. d = 128 takes the LayerNorms in the GEMM
-// epilogues.
+// products on E4M3 codes (tile GEMM quantizing each row of its A operand
+// with the scale max|row| / 448, measured by a reduction launch just
+// before), attention, LayerNorm and residual in f32. d = 128 takes the
+// LayerNorms in the GEMM epilogues.
 void trunk_f8(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPlan& ap) {
     const int d = m.cfg.dim, ff = m.cfg.ff_inner, nl = m.cfg.layers;
     float* Hh = m.get<float>(S_H, (size_t)M * d);
     float* QKV = m.get<float>(S_QKV, (size_t)M * 3 * d);
     float* O = m.get<float>(S_O, (size_t)M * d);
     float* F = m.get<float>(S_F, (size_t)M * ff);
+    float* rmax = m.get<float>(S_AMAX, (size_t)M);  // the rows' maxima of the next product's input
     const double g = 2.0 * M * d, ln_b = (double)M * d * 8, at_b = 16.0 * M * d;
     const bool fuse_ln = d == 128;
-    auto gemm = [&](int op, int N, int K, const float* A, float sa, const uint8_t* Bt, const float* cs,
-                    const float* bias, float* C, int epi, const float* lg, const float* lb) {
+    auto gemm = [&](int op, int N, int K, const float* A, const uint8_t* Bt, const float* cs, const float* bias,
+                    float* C, int epi, const float* lg, const float* lb) {
         TScope t(m, s, KC_GEMM, g / d * N * K);
-        gemm_f8_tile(s, M, N, K, A, sa, Bt, cs, bias, C, X, epi, lg, lb, Hh, m.tune.choice(op, M));
+        row_amax_f32(s, M, K, A, rmax);
+        gemm_f8_tile(s, M, N, K, A, rmax, Bt, cs, bias, C, X, epi, lg, lb, Hh, m.tune.choice(op, M));
     };
     auto norm = [&](const float* lg, const float* lb) {
         TScope t(m, s, KC_NORM, ln_b);
@@ -565,24 +525,24 @@ void trunk_f8(Model& m, hipStream_t s, int M, float* X, int ready, const AttnPla
     for (int l = 0; l < nl; l++) {
         const LayerW& L = m.L[l];
         const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;
-        gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.s_h1, L.wqkv_8, L.cs_qkv, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_8, L.cs_qkv, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
         {
             TScope t(m, s, KC_ATTN, at_b);
             attend<float>(m, s, M, QKV, O, ap);
         }
         if (fuse_ln) {
-            gemm(TO_GEMM_PROJ, d, d, O, L.s_o, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
+            gemm(TO_GEMM_PROJ, d, d, O, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID_LN, L.n2g, L.n2b);
         } else {
-            gemm(TO_GEMM_PROJ, d, d, O, L.s_o, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_PROJ, d, d, O, L.wp_8, L.cs_p, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
             norm(L.n2g, L.n2b);
         }
-        gemm(TO_GEMM_FF, ff, d, Hh, L.s_h2, L.w1_8, L.cs_1, L.b1, F, EPI_GELU, nullptr, nullptr);
+        gemm(TO_GEMM_FF, ff, d, Hh, L.w1_8, L.cs_1, L.b1, F, EPI_GELU, nullptr, nullptr);
         if (!next) {
-            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
         } else if (fuse_ln) {
-            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID_LN, next->n1g, next->n1b);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID_LN, next->n1g, next->n1b);
         } else {
-            gemm(TO_GEMM_FF, d, ff, F, L.s_g, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
+            gemm(TO_GEMM_FF, d, ff, F, L.w2_8, L.cs_2, L.b2, nullptr, EPI_RESID, nullptr, nullptr);
             norm(next->n1g, next->n1b);
         }
     }

@@ -65,6 +65,7 @@ size_t bytes_needed(const Model& m, const CellTables& t) {
             + aligned(3 * M * d * act)  // q, k, v
             + aligned(M * d * act)      // attention output
             + (rows ? 0 : aligned(M * ff * act));  // MLP intermediate (on chip in the row-panel kernel)
+    if (m.fp8()) need += aligned(M * sizeof(float));  // the rows' maxima of each product's input
     if (m.cfg.precision == J2K_PREC_F32) {  // split-K partials of the products of few tiles
         const size_t ws = std::max({gemm_f32_workspace((int)M, (int)(3 * d), (int)d), gemm_f32_workspace((int)M, (int)d, (int)d),
                                     gemm_f32_workspace((int)M, (int)ff, (int)d), gemm_f32_workspace((int)M, (int)d, (int)ff)});

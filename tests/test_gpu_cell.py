@@ -373,22 +373,26 @@ def test_fp8_form_error_vs_f64_oracle(model20_cell, shape):
     rel = np.linalg.norm(xg - xd) / np.linalg.norm(xd)
     inc = np.linalg.norm(xg - x0) / max(np.linalg.norm(xd - x0), 1e-30)
     print(f"fp8 {shape}: rel L2 {rel:.3e}, increment ratio {inc:.3f}")
-    # The FP8 forward is off by 3% (model20) to 7% (wide) against the exact
-    # forward (E4M3 keeps 3 mantissa bits); through the fixed point and the
-    # energy projection that is 5e-2 / 3.5e-1 of the blank-cell anomalies.
-    # No target is stated for FP8: the bound only catches a blow-up; the
-    # arithmetic is pinned against its restatement in the next test.
-    assert rel < (0.15 if shape == "model20" else 0.6)
+    # The FP8 forward is off by about 3.5% (model20) and 2-4% (wide) against
+    # the exact forward (E4M3 keeps 3 mantissa bits, with each token's own
+    # scale). model20's fixed point moves the blank h cells by only ~3e-3
+    # about the slice mean, below that per-token noise, so its relative error
+    # here is about 0.33; the wide model's is about 0.43. No target is stated
+    # for FP8: the bound only catches a blow-up; the arithmetic is pinned
+    # against its restatement in the next test.
+    assert rel < 0.6
     assert vg <= 1e-6
 
 
 @pytest.mark.parametrize("shape", ["model20", "wide"])
 def test_fp8_forward_matches_quantized_restatement(model20_cell, shape):
     """GPU FP8 forward of one block against the numpy restatement of the
-    same quantized arithmetic (E4M3 codes, per-tensor scales): it must sit
-    far closer to it than to the unquantized forward. Differences come from
-    f32 sums and the rare code flips they cause; with more blocks each flip
-    feeds the next quantization, so one block is compared."""
+    same quantized arithmetic (E4M3 codes, per-tensor weight scales,
+    per-token input scales): it must sit far closer to it than to the
+    unquantized forward. Differences come from f32 sums and the code flips
+    they cause (a token's scale is its own f32 maximum, so a rounding
+    difference there moves its whole row); with more blocks each flip feeds
+    the next quantization, so one block is compared."""
     from tests import fp8_emulation as E
 
     st, cj = (ck.random_cell_st(dim=128, layers=1, heads=4, ff=512, seed=0) if shape == "model20" else
@@ -405,7 +409,7 @@ def test_fp8_forward_matches_quantized_restatement(model20_cell, shape):
     d_emu = np.linalg.norm(y - emu) / np.linalg.norm(ref)
     d_ref = np.linalg.norm(emu - ref) / np.linalg.norm(ref)
     print(f"fp8 {shape}: |gpu - quantized| {d_emu:.2e}, |quantized - exact| {d_ref:.2e} (relative L2)")
-    assert d_emu < 0.1 * d_ref  # measured: 0.03x (d=128)
+    assert d_emu < 0.2 * d_ref  # measured: 0.04x (d=128), 0.15x (wide)
 
 
 def test_fp8_checkpoint_codes_match_load_time_quantization(model20_cell):

@@ -91,6 +91,11 @@ bool rows_fused_ok(int dim, int ff);
 // over 32 tokens and 64 columns each.
 enum { ROWS_4W = 1, ROWS_8W = 2, ROWS_4W_F64 = 3, ROWS_2X2 = 4 };
 void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a, int shape = ROWS_4W);
+// X += ff2(GELU(ff1(Hin))) + biases for dim == 256 (Hin, W1, b1, W2, b2, X of
+// a; weights in fragment order), the tile GEMMs' GELU and residual epilogue
+// arithmetic with the intermediate kept on chip.
+bool mlp_fused_ok(int dim, int ff);
+void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
 // QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
 // arithmetic: the first layer's product after the embedding.
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);

@@ -583,6 +583,85 @@ __global__ __launch_bounds__(256, 2) void k_rows_qkv(int M, RowsArgs a) {
     }
 }
 
+// The MLP of a wider trunk (D = 256: the wide shape) over 64-token panels,
+// X += ff2(GELU(ff1(H))) + biases: the ff-wide intermediate never reaches
+// HBM (at 262,144 tokens and ff 1024 it is 1 GB of f32 per block otherwise,
+// written by one GEMM and read back by the next). Eight waves, each owning
+// 32 of the D output columns of ff2 and 16 of the 128 columns of an ff1
+// chunk, over all 64 tokens; H (hi + lo in the split form) and the current
+// intermediate chunk sit in LDS (96 KB split: one block per CU). Products
+// as in k_rows (transposed, weights in fragment order from L2 one chunk
+// ahead), so every output is the same dot product, in the same order, with
+// the same bias, GELU and residual arithmetic as the tile GEMMs' GELU and
+// residual epilogues: the two arrangements give bit-identical results.
+template <bool SPLIT, int D>
+__global__ __launch_bounds__(512, 1) void k_mlp_rows(int M, int ff, RowsArgs a) {
+    constexpr int WN = 8, NT = 64 * WN, MI = 4, J = D / (16 * WN), FC = 128, JF = FC / (16 * WN);
+    constexpr int K4D = D / 32, K4F = FC / 32, P = SPLIT ? 2 : 1;
+    static_assert(J >= 1 && JF == 1, "eight waves over D output columns and a 128-column chunk");
+    __shared__ __align__(16) uint16_t Hs[P * RB * D];
+    __shared__ __align__(16) uint16_t Fs[P * RB * FC];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, rows = min(RB, M - m0);
+    const int wn = wave * 16 * J, wf = wave * 16;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    uint16_t *const Hh = Hs, *const Hl = Hs + (SPLIT ? RB * D : 0);
+    uint16_t *const Fh = Fs, *const Fl = Fs + (SPLIT ? RB * FC : 0);
+    // H: the panel's rows x D (bf16, or f32 split into hi + lo); rows past the end read 0
+    if (SPLIT) {
+        const auto r = rows_rsrc(static_cast<const float*>(a.Hin) + (long)m0 * D, rows * D * 4);
+#pragma unroll
+        for (int q = 0; q < RB * D / 4 / NT; q++) {
+            const int v = tid + NT * q, row = v / (D / 4), kv = v % (D / 4);
+            put_a4<true>(Hh, Hl, row, kv * 4, bload4(r, (row * D + kv * 4) * 4));
+        }
+    } else {
+        const auto r = rows_rsrc(static_cast<const uint16_t*>(a.Hin) + (long)m0 * D, rows * D * 2);
+#pragma unroll
+        for (int q = 0; q < RB * D / 8 / NT; q++) {
+            const int v = tid + NT * q, row = v / (D / 8), kv = v % (D / 8);
+            const u4v val = __builtin_amdgcn_raw_buffer_load_b128(r, (row * D + kv * 8) * 2, 0, 0);
+            *reinterpret_cast<u4v*>(&Hh[pidx(row, kv * 8)]) = val;
+        }
+    }
+    f4v acc2[MI][J];
+    zero(acc2);
+    bf8v v1[K4D][JF], w2[K4F][J];
+    load_w<JF, K4D>(v1, a.W1, D, 0, 0, wf, lane);
+    __syncthreads();
+    for (int c0 = 0; c0 < ff; c0 += FC) {
+        load_w<J, K4F>(w2, a.W2, ff, 0, c0, wn, lane);  // in flight during ff1
+        f4v acc1[MI][JF];
+        zero(acc1);
+        mma_panel<SPLIT, MI, JF, K4D>(acc1, Hh, Hl, v1, r16, kq);
+        const f4v bb = *reinterpret_cast<const f4v*>(a.b1 + c0 + wf + c4);
+        if (c0 + FC < ff) load_w<JF, K4D>(v1, a.W1, D, c0 + FC, 0, wf, lane);  // in flight during GELU and ff2
+        __syncthreads();  // every wave has finished reading the previous intermediate chunk
+#pragma unroll
+        for (int i = 0; i < MI; i++) {
+            f4v v;
+#pragma unroll
+            for (int r = 0; r < 4; r++) v[r] = gelu_fast(acc1[i][0][r] + bb[r]);
+            put_a4<SPLIT>(Fh, Fl, i * 16 + r16, wf + c4, v);
+        }
+        __syncthreads();
+        mma_panel<SPLIT, MI, J, K4F>(acc2, Fh, Fl, w2, r16, kq);
+    }
+    const auto rX = rows_rsrc(a.X + (long)m0 * D, rows * D * 4);
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const f4v b = *reinterpret_cast<const f4v*>(a.b2 + wn + j * 16 + c4);
+#pragma unroll
+        for (int i = 0; i < MI; i++) {
+            const int o = ((i * 16 + r16) * D + wn + j * 16 + c4) * 4;
+            f4v x = bload4(rX, o);
+#pragma unroll
+            for (int r = 0; r < 4; r++) x[r] = x[r] + acc2[i][j][r] + b[r];
+            bstore4(rX, o, x);
+        }
+    }
+}
+
 }  // namespace
 
 bool rows_fused_ok(int dim, int ff) { return dim == RD && ff % RD == 0; }
@@ -666,6 +745,17 @@ void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {
     if (split) hipLaunchKernelGGL(k_rows_qkv<true>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, a);
     else hipLaunchKernelGGL(k_rows_qkv<false>, dim3(cdiv(M, RB)), dim3(256), 0, s, M, a);
     LAUNCHCHK("rows_qkv");
+}
+
+bool mlp_fused_ok(int dim, int ff) { return dim == 256 && ff % 128 == 0; }
+
+void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a) {
+    if (M <= 0) return;
+    if (!mlp_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "fused MLP kernel needs dim == 256 and ff a multiple of 128");
+    if (!a.Hin || !a.W1 || !a.b1 || !a.W2 || !a.b2 || !a.X) fail(J2K_ERR_INVALID, "fused MLP kernel: missing operand");
+    if (split) hipLaunchKernelGGL((k_mlp_rows<true, 256>), dim3(cdiv(M, RB)), dim3(512), 0, s, M, ff, a);
+    else hipLaunchKernelGGL((k_mlp_rows<false, 256>), dim3(cdiv(M, RB)), dim3(512), 0, s, M, ff, a);
+    LAUNCHCHK("mlp_fused");
 }
 
 }  // namespace j2k

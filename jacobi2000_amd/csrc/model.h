@@ -69,7 +69,8 @@ enum TuneOp : int {
     TO_GEMM_PROJ = 1,  // block height of the attention-output GEMM
     TO_GEMM_FF = 2,    // block height of ff1 / ff2 when the MLP is not fused
     TO_ATTN = 3,       // staging mode of the tiled cell attention (KvMode: joint, split, plane)
-    TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), fused on 1: 4 or 2: 8 waves
+    TO_ROWS = 4,       // d = 128 token-local work: 0 apart (proj / qkv GEMMs), fused in one of the
+                       // row-panel launch shapes 1 .. 4; d = 256 MLP: 0 two tile GEMMs, 1 one fused launch
     TO_COUNT = 5
 };
 struct Tuning {

@@ -315,6 +315,9 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
                 L.wp_f = up.bf16_frag(get(w, p + "proj.weight").data, d, d);
                 L.w1_f = up.bf16_frag(get(w, p + "ff1.weight").data, d, ff);
                 L.w2_f = up.bf16_frag(get(w, p + "ff2.weight").data, ff, d);
+            } else if (mlp_fused_ok(d, ff)) {
+                L.w1_f = up.bf16_frag(get(w, p + "ff1.weight").data, d, ff);
+                L.w2_f = up.bf16_frag(get(w, p + "ff2.weight").data, ff, d);
             }
         }
         m->L.push_back(L);
@@ -394,6 +397,8 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
     Act* F = rows ? nullptr : m.get<Act>(S_F, (size_t)M * ff);  // the row kernel keeps it on chip
     const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const int rows_pick = rows ? m.tune.choice(TO_ROWS, M) : 0;
+    // d = 256: the MLP in one launch (the tuner's pick 1) or as two tile GEMMs
+    const bool mlp = mlp_fused_ok(d, ff) && !m.L.empty() && m.L[0].w1_f && m.tune.choice(TO_ROWS, M) == 1;
     const bool fused = rows_pick >= ROWS_4W;
     const int rows_shape = fused ? rows_pick : ROWS_4W;
     if (emb && !(ready == TRUNK_QKV && fused && rows_shape == ROWS_4W && nl > 1))
@@ -480,6 +485,21 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
         } else {
             gemm(TO_GEMM_PROJ, d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr);
             norm(L.n2g, L.n2b);
+        }
+        if (mlp) {
+            RowsArgs a;
+            a.Hin = Hh;
+            a.W1 = L.w1_f;
+            a.b1 = L.b1;
+            a.W2 = L.w2_f;
+            a.b2 = L.b2;
+            a.X = X;
+            {
+                TScope t(m, s, KC_ROWS_TAIL, (double)M * d * (sizeof(Act) + 8.0));
+                mlp_fused(s, M, d, ff, split, a);
+            }
+            if (next) norm(next->n1g, next->n1b);
+            continue;
         }
         gemm(TO_GEMM_FF, ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr);
         if (!next) {

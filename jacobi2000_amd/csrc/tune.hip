@@ -165,6 +165,25 @@ void tune_bucket(Model& m, int b, Events& ev) {
             });
         settle(TO_ROWS);
     }
+    if (mlp_fused_ok(d, ff) && L.w1_f) {
+        // the MLP (H -> X) as two tile GEMMs with the block height just chosen, or in one launch
+        const int bf = GEMM_BM[m.tune.pick[TO_GEMM_FF][b]];
+        m.tune.us[TO_ROWS][b][0] = time_us(s, ev, [&] {
+            gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr, bf);
+            gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr, bf);
+        });
+        m.tune.us[TO_ROWS][b][1] = time_us(s, ev, [&] {
+            RowsArgs a;
+            a.Hin = Hh;
+            a.W1 = L.w1_f;
+            a.b1 = L.b1;
+            a.W2 = L.w2_f;
+            a.b2 = L.b2;
+            a.X = X;
+            mlp_fused(s, (int)M, d, ff, split, a);
+        });
+        settle(TO_ROWS);
+    }
 }
 
 }  // namespace

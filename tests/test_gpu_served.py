@@ -278,6 +278,34 @@ def test_wide_six_layers(precision):
             assert np.abs(xg - ref[it - 1]).max() < S.F32_MAX_ABS
 
 
+@pytest.mark.parametrize("precision", ["bf16x2", "bf16"])
+def test_wide_fused_mlp_bit_identical(precision):
+    """d = 256: the MLP in one launch (its ff-wide intermediate on chip)
+    gives the same raw output and iterates, bit for bit, as the two tile
+    GEMMs, on a grid whose token count is no multiple of the 64-token panel
+    (3 x 24 x 20); the fused launch runs once per block; bf16x2's raw output
+    sits within its bound of the f64 oracle on its bf16 weights."""
+    st, cj = ck.random_cell_st(dim=256, layers=2, heads=8, ff=1024, seed=1, radius=2)
+    fused = _model(st, cj, precision, "rows_fused=1")
+    apart = _model(st, cj, precision, "rows_fused=0")
+    x0, k = S.request("reconstruction", 3, 24, 20)
+    print(f"case wide fused MLP {precision}", flush=True)
+    ya, ca = gpu_forward(apart, x0, k, timing=True)
+    yf, cf = gpu_forward(fused, x0, k, timing=True)
+    assert cf[KC_ROWS_TAIL] == 2 and ca[KC_ROWS_TAIL] == 0, (cf, ca)
+    assert np.isfinite(yf).all()
+    assert np.array_equal(ya.view(np.uint32), yf.view(np.uint32))
+    xa, ia, ra, _ = apart.iterate(x0, k, META, max_iter=2, tol=0.0)
+    xf, i_f, rf, _ = fused.iterate(x0, k, META, max_iter=2, tol=0.0)
+    assert (ia, ra) == (i_f, rf)
+    assert np.array_equal(xa.view(np.uint32), xf.view(np.uint32))
+    if precision == "bf16x2":
+        cpu = O.FormC.load(st, cj)
+        d = S.rel_var(yf, S.bf16_weights(cpu).forward(x0, k, META, f64=True))
+        print(f"wide fused MLP bf16x2 raw output rel_var vs bf16-weight oracle {d}")
+        assert d.max() < S.Y_TOL["bf16x2"]
+
+
 # ---------------------------------------------------------------- serving at model20 shape
 
 def _server(model, budget=1 << 36, max_active=64):

@@ -192,6 +192,9 @@ struct AttnPlan {
 void patchify(hipStream_t s, int B, int C, int ny, int nx, int p, const float* state, float* patches);
 // cpu.rs:276-277: mvec = meta @ Wm + bm (one row).
 void meta_linear(hipStream_t s, int meta_dim, int dim, const float* meta, const float* w, const float* b, float* out);
+// meta_linear of n vectors meta [n][meta_dim] into out [n][dim], one launch.
+void meta_linear_n(hipStream_t s, int n, int meta_dim, int dim, const float* meta, const float* w, const float* b,
+                   float* out);
 // cpu.rs:319-324: xn [B, tokens, dim] -> t0 [B, dim, tokens].
 void transpose_td(hipStream_t s, int B, int tokens, int dim, const float* in, float* out);
 // cpu.rs:211-245, both axes: in [B, c, h, w] -> out [B, c, h*p, w*p] (out batch stride given).
@@ -220,6 +223,22 @@ struct Slice {
 // writes `copy` when it is not NULL.
 void cell_pack(hipStream_t s, int M, const float* grid, const uint8_t* known, float* x4, float* copy = nullptr);
 void cell_unpack(hipStream_t s, int M, const float* x4, float* grid);
+// Many requests in one launch each (the server's turn): job j moves `tokens`
+// tokens between byte offset `buf` of a staging buffer (the request's grid
+// planes [3][tokens] f32, for the pack followed by its mask [tokens] u8) and
+// packed token `tok` (x4 / copy as in cell_pack and cell_unpack).
+struct CellJob {
+    long buf;
+    int tok, tokens;
+};
+void cell_pack_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const unsigned char* staging,
+                    float* x4, float* copy);
+void cell_unpack_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const float* x4,
+                      unsigned char* staging);
+// Repacking: job j moves tokens [buf, buf + tokens) of x0_old and x_old to
+// [tok, tok + tokens) of x0 and x (buf is a token index here).
+void cell_move_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const float* x0_old,
+                    const float* x_old, float* x0, float* x);
 // The request index of every token of a packing: tok_req[t] = r for t in segment r.
 void seg_tokens(hipStream_t s, int n_segs, const Seg* segs, int* tok_req);
 // A chained rollout window of T slices of n cells from the packed anchor
@@ -252,10 +271,12 @@ void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const
                   const double* eref_t, unsigned* resid_bits, float g, const float* mean, const float* stdv,
                   float depth, float r_max, unsigned long long* viol_bits = nullptr, float* grid = nullptr,
                   int M = 0);
-// Per-slice relative energy violation, max into viol_bits[req] (double bits, atomic max).
+// Per-slice relative energy violation, max into viol_bits[req] (double bits,
+// atomic max). Slice i's reference energy is eref_t[i], or eref_t[eref_idx[i]]
+// when eref_idx is given.
 void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x4, const double* eref_t,
                     unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,
-                    float r_max);
+                    float r_max, const int* eref_idx = nullptr);
 
 }  // namespace j2k
 

@@ -82,6 +82,22 @@ void meta_linear(hipStream_t s, int meta_dim, int dim, const float* meta, const 
     LAUNCHCHK("meta_linear");
 }
 
+__global__ void k_meta_linear_n(int md, int dim, const float* meta, const float* w, const float* b, float* out) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= dim) return;
+    out[(long)blockIdx.y * dim + o] = meta_out(md, dim, meta + (long)blockIdx.y * md, w, b, o);
+}
+
+void meta_linear_n(hipStream_t s, int n, int meta_dim, int dim, const float* meta, const float* w, const float* b,
+                   float* out) {
+    for (int i0 = 0; i0 < n; i0 += 65535) {
+        const int k = std::min(65535, n - i0);
+        hipLaunchKernelGGL(k_meta_linear_n, dim3(cdiv(dim, 256), k), dim3(256), 0, s, meta_dim, dim,
+                           meta + (long)i0 * meta_dim, w, b, out + (long)i0 * dim);
+        LAUNCHCHK("meta_linear_n");
+    }
+}
+
 __global__ void k_transpose_td(int B, int tokens, int dim, const float* __restrict__ in, float* __restrict__ out) {
     __shared__ float tile[32][33];
     const int b = blockIdx.z, t0 = blockIdx.x * 32, d0 = blockIdx.y * 32;
@@ -165,6 +181,78 @@ __global__ void k_cell_unpack(int M, const float4* __restrict__ x4, float* __res
     g[m] = v.x;
     g[(long)M + m] = v.y;
     g[2 * (long)M + m] = v.z;
+}
+
+__global__ void k_cell_pack_jobs(const CellJob* __restrict__ jobs, const unsigned char* __restrict__ staging,
+                                 float4* __restrict__ x4, float4* __restrict__ copy) {
+    const CellJob j = jobs[blockIdx.y];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= j.tokens) return;
+    const float* g = reinterpret_cast<const float*>(staging + j.buf);
+    const uint8_t* kn = staging + j.buf + 12L * j.tokens;
+    const long M = j.tokens;
+    const float4 v = make_float4(g[m], g[M + m], g[2 * M + m], kn[m] ? 1.0f : 0.0f);
+    x4[(long)j.tok + m] = v;
+    if (copy) copy[(long)j.tok + m] = v;
+}
+
+// job tables of more than 65535 entries go in several launches (grid y limit)
+constexpr int JOBS_PER_LAUNCH = 65535;
+
+void cell_pack_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const unsigned char* staging,
+                    float* x4, float* copy) {
+    if (max_tokens <= 0) return;
+    for (int j0 = 0; j0 < n_jobs; j0 += JOBS_PER_LAUNCH) {
+        const int n = std::min(JOBS_PER_LAUNCH, n_jobs - j0);
+        hipLaunchKernelGGL(k_cell_pack_jobs, dim3(cdiv(max_tokens, 256), n), dim3(256), 0, s, jobs + j0, staging,
+                           (float4*)x4, (float4*)copy);
+        LAUNCHCHK("cell_pack_jobs");
+    }
+}
+
+// job j: tokens [buf, buf + tokens) of x0_old and x_old to [tok, tok + tokens) of x0 and x
+__global__ void k_cell_move_jobs(const CellJob* __restrict__ jobs, const float4* __restrict__ x0_old,
+                                 const float4* __restrict__ x_old, float4* __restrict__ x0, float4* __restrict__ x) {
+    const CellJob j = jobs[blockIdx.y];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= j.tokens) return;
+    x0[(long)j.tok + m] = x0_old[j.buf + m];
+    x[(long)j.tok + m] = x_old[j.buf + m];
+}
+
+void cell_move_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const float* x0_old,
+                    const float* x_old, float* x0, float* x) {
+    if (max_tokens <= 0) return;
+    for (int j0 = 0; j0 < n_jobs; j0 += JOBS_PER_LAUNCH) {
+        const int n = std::min(JOBS_PER_LAUNCH, n_jobs - j0);
+        hipLaunchKernelGGL(k_cell_move_jobs, dim3(cdiv(max_tokens, 256), n), dim3(256), 0, s, jobs + j0,
+                           (const float4*)x0_old, (const float4*)x_old, (float4*)x0, (float4*)x);
+        LAUNCHCHK("cell_move_jobs");
+    }
+}
+
+__global__ void k_cell_unpack_jobs(const CellJob* __restrict__ jobs, const float4* __restrict__ x4,
+                                   unsigned char* __restrict__ staging) {
+    const CellJob j = jobs[blockIdx.y];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= j.tokens) return;
+    float* g = reinterpret_cast<float*>(staging + j.buf);
+    const long M = j.tokens;
+    const float4 v = x4[(long)j.tok + m];
+    g[m] = v.x;
+    g[M + m] = v.y;
+    g[2 * M + m] = v.z;
+}
+
+void cell_unpack_jobs(hipStream_t s, int n_jobs, const CellJob* jobs, int max_tokens, const float* x4,
+                      unsigned char* staging) {
+    if (max_tokens <= 0) return;
+    for (int j0 = 0; j0 < n_jobs; j0 += JOBS_PER_LAUNCH) {
+        const int n = std::min(JOBS_PER_LAUNCH, n_jobs - j0);
+        hipLaunchKernelGGL(k_cell_unpack_jobs, dim3(cdiv(max_tokens, 256), n), dim3(256), 0, s, jobs + j0,
+                           (const float4*)x4, staging);
+        LAUNCHCHK("cell_unpack_jobs");
+    }
 }
 
 // A chained rollout's next window on the device: every slice of M = T * n

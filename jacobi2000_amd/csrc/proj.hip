@@ -488,7 +488,8 @@ void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const
 
 __global__ __launch_bounds__(CPT) void k_cell_violation(const Slice* __restrict__ sl, const float4* __restrict__ x4,
                                                        const double* __restrict__ eref_t,
-                                                       unsigned long long* __restrict__ viol, Phys ph) {
+                                                       unsigned long long* __restrict__ viol, Phys ph,
+                                                       const int* __restrict__ eref_idx) {
     __shared__ double sh[1][CPT / 64];
     const Slice S = sl[blockIdx.x];
     double v[1] = {0};
@@ -499,7 +500,7 @@ __global__ __launch_bounds__(CPT) void k_cell_violation(const Slice* __restrict_
     }
     block_sum<1, CPT>(v, sh);
     if (threadIdx.x == 0) {
-        const double e = eref_t[blockIdx.x];
+        const double e = eref_t[eref_idx ? eref_idx[blockIdx.x] : (int)blockIdx.x];
         const double r = fmax((v[0] - (double)ph.r_max * e) / e, 0.0);
         atomicMax(viol + S.req, (unsigned long long)__double_as_longlong(r));
     }
@@ -507,10 +508,10 @@ __global__ __launch_bounds__(CPT) void k_cell_violation(const Slice* __restrict_
 
 void cell_violation(hipStream_t s, int n_slices, const Slice* sl, const float* x4, const double* eref_t,
                     unsigned long long* viol_bits, float g, const float* mean, const float* stdv, float depth,
-                    float r_max) {
+                    float r_max, const int* eref_idx) {
     if (n_slices <= 0) return;
     hipLaunchKernelGGL(k_cell_violation, dim3(n_slices), dim3(CPT), 0, s, sl, (const float4*)x4, eref_t, viol_bits,
-                       phys_of(g, mean, stdv, depth, r_max));
+                       phys_of(g, mean, stdv, depth, r_max), eref_idx);
     LAUNCHCHK("cell_violation");
 }
 

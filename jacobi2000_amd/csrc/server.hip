@@ -164,6 +164,16 @@ struct Server {
     unsigned char* h_in = nullptr;
     unsigned char* h_out = nullptr;
     size_t cap_hin = 0, cap_hout = 0;
+    // pinned host copies of a turn's small tables (tiles, segments, slices,
+    // meta vectors, pack and move jobs; at retirement the slice table, its
+    // reference-energy indices and the unpack jobs), uploaded without a
+    // synchronisation; and the device job tables
+    unsigned char* h_tab = nullptr;
+    size_t cap_htab = 0;
+    CellJob* d_jobs = nullptr;
+    size_t cap_jobs = 0;
+    int* d_ridx = nullptr;
+    size_t cap_ridx = 0;
     // J2K_SERVER_PHASES=1 at creation: seconds the worker spent per phase,
     // printed to stderr when the server is destroyed
     bool phases_on = false;
@@ -195,6 +205,7 @@ struct Server {
         if (h_resid) (void)hipHostFree(h_resid);
         if (h_in) (void)hipHostFree(h_in);
         if (h_out) (void)hipHostFree(h_out);
+        if (h_tab) (void)hipHostFree(h_tab);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
@@ -314,24 +325,62 @@ struct Server {
             if (in_bytes) HIPCHK(hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, s));
         }
         const double r2 = now_s();
-        size_t in_off = 0;
+        // new requests are packed, survivors moved, by one launch each over
+        // a job table; a rollout's next window is made from its last slice
+        std::vector<CellJob> jobs;  // packs, then moves
+        size_t n_pack = 0;
+        int max_pack = 0, max_move = 0;
+        {
+            size_t in_off = 0;
+            for (size_t i = 0; i < active.size(); i++) {
+                const Request& r = *active[i];
+                const int T = r.dims.tokens();
+                if (r.base < 0) {
+                    jobs.push_back(CellJob{(long)in_off, tab.segs[i].base, T});
+                    in_off += aligned(13 * (size_t)T);
+                    max_pack = std::max(max_pack, T);
+                }
+            }
+            n_pack = jobs.size();
+            for (size_t i = 0; i < active.size(); i++) {
+                const Request& r = *active[i];
+                if (r.base >= 0 && r.prev_T == 0) {
+                    jobs.push_back(CellJob{(long)r.base, tab.segs[i].base, r.dims.tokens()});
+                    max_move = std::max(max_move, r.dims.tokens());
+                }
+            }
+        }
+        const size_t nseg = tab.segs.size(), ntile = tab.tiles.size(), nsl = tab.slices.size();
+        const size_t md = (size_t)m.cfg.meta_dim, nreq = active.size();
+        // the turn's small tables in one pinned buffer: ctiles, segs, tiles, slices, metas, jobs
+        size_t off[7] = {0};
+        {
+            const size_t bytes[6] = {tab.ctiles.size() * sizeof(CellTile), nseg * sizeof(Seg), ntile * sizeof(Tile),
+                                     nsl * sizeof(Slice), md * nreq * sizeof(float), jobs.size() * sizeof(CellJob)};
+            for (int i = 0; i < 6; i++) off[i + 1] = off[i] + aligned(bytes[i]);
+            reserve_host(h_tab, cap_htab, off[6]);
+            std::memcpy(h_tab + off[0], tab.ctiles.data(), bytes[0]);
+            std::memcpy(h_tab + off[1], tab.segs.data(), bytes[1]);
+            std::memcpy(h_tab + off[2], tab.tiles.data(), bytes[2]);
+            std::memcpy(h_tab + off[3], tab.slices.data(), bytes[3]);
+            for (size_t i = 0; i < nreq; i++)
+                std::memcpy(h_tab + off[4] + i * md * sizeof(float), active[i]->meta.data(), md * sizeof(float));
+            std::memcpy(h_tab + off[5], jobs.data(), bytes[5]);
+        }
+        reserve(d_jobs, cap_jobs, jobs.size());
+        if (!jobs.empty())
+            HIPCHK(hipMemcpyAsync(d_jobs, h_tab + off[5], jobs.size() * sizeof(CellJob), hipMemcpyHostToDevice, s));
+        cell_pack_jobs(s, (int)n_pack, d_jobs, max_pack, d_in, alt0, altc);
+        cell_move_jobs(s, (int)(jobs.size() - n_pack), d_jobs + n_pack, max_move, x0, xc, alt0, altc);
         int slices = 0;  // the slice table lists each request's slices in order
         for (size_t i = 0; i < active.size(); i++) {
             Request& r = *active[i];
-            const size_t nb = (size_t)tab.segs[i].base, T = (size_t)r.dims.tokens();
-            if (r.base < 0) {  // new: up into its own staging region, packed into x0 and the first iterate
-                const float* dgrid = reinterpret_cast<const float*>(d_in + in_off);
-                const uint8_t* dk = d_in + in_off + 12 * T;
-                cell_pack(s, (int)T, dgrid, dk, alt0 + 4 * nb, altc + 4 * nb);
-                in_off += aligned(13 * T);
-            } else if (r.prev_T > 0) {  // a rollout's next window, from its last slice on the device
+            const size_t nb = (size_t)tab.segs[i].base;
+            if (r.base >= 0 && r.prev_T > 0) {  // a rollout's next window, from its last slice on the device
                 const size_t n = (size_t)r.dims.ny * r.dims.nx;
                 cell_window(s, (int)n, r.dims.T, xc + 4 * ((size_t)r.base + (size_t)(r.prev_T - 1) * n), alt0 + 4 * nb,
                             altc + 4 * nb);
                 r.prev_T = 0;
-            } else {
-                HIPCHK(hipMemcpyAsync(alt0 + 4 * nb, x0 + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
-                HIPCHK(hipMemcpyAsync(altc + 4 * nb, xc + 4 * (size_t)r.base, 16 * T, hipMemcpyDeviceToDevice, s));
             }
             r.base = (int)nb;
             r.slice0 = slices;
@@ -354,7 +403,6 @@ struct Server {
         parity = 0;
         since_repack = 0;
 
-        const size_t nseg = tab.segs.size(), ntile = tab.tiles.size(), nsl = tab.slices.size();
         const size_t need = std::max({nseg, ntile, nsl, (size_t)1});
         if (need > cap_tab) {
             for (void* p : {(void*)d_segs, (void*)d_tiles, (void*)d_slices, (void*)d_mvec, (void*)d_resid, (void*)d_targ})
@@ -373,19 +421,14 @@ struct Server {
             cap_ctiles = tab.ctiles.size() + tab.ctiles.size() / 2;
             d_ctiles = dev_alloc<CellTile>(cap_ctiles);
         }
-        HIPCHK(hipMemcpy(d_ctiles, tab.ctiles.data(), tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_segs, tab.segs.data(), nseg * sizeof(Seg), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_tiles, tab.tiles.data(), ntile * sizeof(Tile), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_slices, tab.slices.data(), nsl * sizeof(Slice), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpyAsync(d_ctiles, h_tab + off[0], tab.ctiles.size() * sizeof(CellTile), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_segs, h_tab + off[1], nseg * sizeof(Seg), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_tiles, h_tab + off[2], ntile * sizeof(Tile), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_slices, h_tab + off[3], nsl * sizeof(Slice), hipMemcpyHostToDevice, s));
         if (multi) seg_tokens(s, (int)nseg, d_segs, d_tokreq);
-        float* dmeta = m.get<float>(S_META, (size_t)m.cfg.meta_dim * active.size());
-        std::vector<float> metas((size_t)m.cfg.meta_dim * active.size());
-        for (size_t i = 0; i < active.size(); i++)
-            std::memcpy(&metas[i * m.cfg.meta_dim], active[i]->meta.data(), m.cfg.meta_dim * sizeof(float));
-        HIPCHK(hipMemcpyAsync(dmeta, metas.data(), metas.size() * sizeof(float), hipMemcpyHostToDevice, s));
-        for (size_t i = 0; i < active.size(); i++)
-            meta_linear(s, m.cfg.meta_dim, m.cfg.dim, dmeta + i * m.cfg.meta_dim, m.meta_w, m.meta_b,
-                        d_mvec + i * m.cfg.dim);
+        float* dmeta = m.get<float>(S_META, md * nreq);
+        HIPCHK(hipMemcpyAsync(dmeta, h_tab + off[4], md * nreq * sizeof(float), hipMemcpyHostToDevice, s));
+        meta_linear_n(s, (int)nreq, (int)md, m.cfg.dim, dmeta, m.meta_w, m.meta_b, d_mvec);
         pack.M = (int)M;
         pack.n_req = (int)active.size();
         pack.n_slices = (int)nsl;
@@ -473,44 +516,58 @@ struct Server {
     }
 
     // Final energy violation and grid of every request active[idx[j]] (reads
-    // xc): one slice table for all of them, their violation and unpack
-    // launches back to back, the copies out, one synchronisation.
+    // xc): one slice table for all of them (with each slice's place in the
+    // packing's table, for its reference energy), one violation launch over
+    // it, one unpack launch over a job table, the copies out, one
+    // synchronisation.
     void retire(const std::vector<size_t>& idx) {
         hipStream_t s = m.stream;
         const size_t k = idx.size();
         std::vector<Slice> sl;
-        std::vector<size_t> first(k), out_off(k);
+        std::vector<int> sidx;
+        std::vector<CellJob> jobs(k);
+        std::vector<size_t> out_off(k);
         size_t out_bytes = 0;
+        int max_tok = 0;
         for (size_t j = 0; j < k; j++) {
             const Request& r = *active[idx[j]];
             const int n = r.dims.ny * r.dims.nx;
-            first[j] = sl.size();
-            for (int t = 0; t < r.dims.T; t++) sl.push_back(Slice{r.base + t * n, n, (int)j});
+            for (int t = 0; t < r.dims.T; t++) {
+                sl.push_back(Slice{r.base + t * n, n, (int)j});
+                sidx.push_back(r.slice0 + t);  // slice0: this window's first slice in the packing's table
+            }
             out_off[j] = out_bytes;
+            jobs[j] = CellJob{(long)out_bytes, r.base, r.dims.tokens()};
+            max_tok = std::max(max_tok, r.dims.tokens());
             out_bytes += aligned(12 * (size_t)r.dims.tokens());
         }
         reserve(d_out, cap_out, out_bytes);
         reserve(d_rsl, cap_rsl, sl.size());
+        reserve(d_ridx, cap_ridx, sidx.size());
+        reserve(d_jobs, cap_jobs, k);
         reserve(d_viol, cap_viol, k);
-        HIPCHK(hipMemcpyAsync(d_rsl, sl.data(), sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
+        const size_t o1 = aligned(sl.size() * sizeof(Slice)), o2 = o1 + aligned(sidx.size() * sizeof(int));
+        reserve_host(h_tab, cap_htab, o2 + k * sizeof(CellJob));
+        std::memcpy(h_tab, sl.data(), sl.size() * sizeof(Slice));
+        std::memcpy(h_tab + o1, sidx.data(), sidx.size() * sizeof(int));
+        std::memcpy(h_tab + o2, jobs.data(), k * sizeof(CellJob));
+        HIPCHK(hipMemcpyAsync(d_rsl, h_tab, sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_ridx, h_tab + o1, sidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_jobs, h_tab + o2, k * sizeof(CellJob), hipMemcpyHostToDevice, s));
         fill_u32(s, reinterpret_cast<unsigned*>(d_viol), (int)(2 * k), 0u);
-        for (size_t j = 0; j < k; j++) {
-            const Request& r = *active[idx[j]];
-            // slice0 is this window's first slice in the packing's table (set by repack)
-            cell_violation(s, r.dims.T, d_rsl + first[j], xc, pack.eref_t + r.slice0, d_viol, m.cfg.g,
-                           m.cfg.norm_mean, m.cfg.norm_std, m.cfg.norm_depth, m.cfg.energy_r_max);
-            cell_unpack(s, r.dims.tokens(), xc + 4 * (size_t)r.base, reinterpret_cast<float*>(d_out + out_off[j]));
-        }
+        cell_violation(s, (int)sl.size(), d_rsl, xc, pack.eref_t, d_viol, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
+                       m.cfg.norm_depth, m.cfg.energy_r_max, d_ridx);
+        cell_unpack_jobs(s, (int)k, d_jobs, max_tok, xc, d_out);
         reserve_host(h_out, cap_hout, out_bytes + k * sizeof(double));
         HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s));
         HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        std::vector<Copy> jobs;
+        std::vector<Copy> copies;
         for (size_t j = 0; j < k; j++) {
             Request& r = *active[idx[j]];
-            jobs.push_back(Copy{r.grid.data(), h_out + out_off[j], 12 * (size_t)r.dims.tokens()});
+            copies.push_back(Copy{r.grid.data(), h_out + out_off[j], 12 * (size_t)r.dims.tokens()});
         }
-        copy_all(jobs);
+        copy_all(copies);
         std::vector<double> v(k);
         std::memcpy(v.data(), h_out + out_bytes, k * sizeof(double));
         const double t = now_s();
@@ -684,6 +741,9 @@ int j2k_server_create(void* model, size_t budget, int max_active, void** out) {
             const size_t slack = 256 * (size_t)std::min(max_active, 4096);
             s->reserve_host(s->h_in, s->cap_hin, 13 * tokens + 2 * slack);
             s->reserve_host(s->h_out, s->cap_hout, 12 * tokens + 2 * slack);
+            // the turn's tables: about 1.2 bytes a token (attention tiles, cell tiles) and a few per request
+            s->reserve_host(s->h_tab, s->cap_htab, 2 * tokens + 64 * slack);
+            s->reserve(s->d_jobs, s->cap_jobs, (size_t)std::min(max_active, 1 << 16));
         }
         s->start();
         *out = s.release();

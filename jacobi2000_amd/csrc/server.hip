@@ -24,7 +24,9 @@
 // Host traffic per turn: the new requests' grids and masks go up into their
 // own regions of one staging buffer and the retiring requests' grids come
 // down from another, with the launches between them back to back and one
-// synchronisation per phase, not per request.
+// synchronisation per phase, not per request. The next turn's pending
+// requests are copied into the other of two upload buffers while an
+// iteration runs.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -63,6 +65,8 @@ struct Request {
     float resid = 0.f;
     size_t bytes = 0;
     int base = -1;  // first token in the current packing (-1: not yet on the device)
+    int stage_buf = -1;    // >= 0: grid and mask already copied into that pinned staging buffer
+    size_t stage_off = 0;  // at this offset
     int slice0 = 0;  // its first slice in the packing's slice table
     StallWatch watch;
     // rollout requests: ticks > 0
@@ -161,9 +165,17 @@ struct Server {
     // pinned host staging of the same copies, so each turn's traffic is one
     // DMA per direction; reserved at creation for the largest pack the
     // budget admits (up to 2^24 tokens), grown if a turn needs more
-    unsigned char* h_in = nullptr;
+    // Uploads go through two pinned buffers: a repack uploads from
+    // h_stage[up] and flips `up`; while the iteration after it runs, the
+    // worker copies the next pending requests into the other buffer
+    // (prestage), so their host copy is off the critical path.
+    unsigned char* h_stage[2] = {nullptr, nullptr};
+    size_t cap_stage[2] = {0, 0};
+    int up = 0;
+    std::vector<Request*> staged;  // pre-staged into h_stage[up]
+    size_t staged_end = 0;
     unsigned char* h_out = nullptr;
-    size_t cap_hin = 0, cap_hout = 0;
+    size_t cap_hout = 0;
     // pinned host copies of a turn's small tables (tiles, segments, slices,
     // meta vectors, pack and move jobs; at retirement the slice table, its
     // reference-energy indices and the unpack jobs), uploaded without a
@@ -203,7 +215,8 @@ struct Server {
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
         if (h_resid) (void)hipHostFree(h_resid);
-        if (h_in) (void)hipHostFree(h_in);
+        for (unsigned char* p : h_stage)
+            if (p) (void)hipHostFree(p);
         if (h_out) (void)hipHostFree(h_out);
         if (h_tab) (void)hipHostFree(h_tab);
         if (prev >= 0) (void)hipSetDevice(prev);
@@ -306,23 +319,39 @@ struct Server {
             alt0 = dev_alloc<float>(4 * want);
             altc = dev_alloc<float>(4 * want);
         }
-        size_t in_bytes = 0;
-        for (auto& r : active)
-            if (r->base < 0) in_bytes += aligned(13 * (size_t)r->dims.tokens());
-        reserve(d_in, cap_in, in_bytes);
-        reserve_host(h_in, cap_hin, in_bytes);
-        {  // every new grid and mask, and the per-token request table, into pinned staging: one upload
-            std::vector<Copy> jobs;
-            size_t o = 0;
+        {  // every new grid and mask into pinned staging (the pre-staged ones are there already): one upload
+            size_t end = staged_end;
             for (auto& r : active)
-                if (r->base < 0) {
+                if (r->base < 0 && r->stage_buf != up) end += aligned(13 * (size_t)r->dims.tokens());
+            if (end > cap_stage[up]) {  // grows (contents lost): stage every new request again
+                for (Request* r : staged) r->stage_buf = -1;
+                staged.clear();
+                staged_end = 0;
+                end = 0;
+                for (auto& r : active)
+                    if (r->base < 0) end += aligned(13 * (size_t)r->dims.tokens());
+                reserve_host(h_stage[up], cap_stage[up], end);
+            }
+            unsigned char* hb = h_stage[up];
+            std::vector<Copy> jobs;
+            size_t o = staged_end;
+            for (auto& r : active)
+                if (r->base < 0 && r->stage_buf != up) {
                     const size_t T = (size_t)r->dims.tokens();
-                    jobs.push_back(Copy{h_in + o, r->grid.data(), 12 * T});
-                    jobs.push_back(Copy{h_in + o + 12 * T, r->known.data(), T});
+                    jobs.push_back(Copy{hb + o, r->grid.data(), 12 * T});
+                    jobs.push_back(Copy{hb + o + 12 * T, r->known.data(), T});
+                    r->stage_buf = up;
+                    r->stage_off = o;
                     o += aligned(13 * T);
                 }
             copy_all(jobs);
-            if (in_bytes) HIPCHK(hipMemcpyAsync(d_in, h_in, in_bytes, hipMemcpyHostToDevice, s));
+            reserve(d_in, cap_in, end);
+            if (end) HIPCHK(hipMemcpyAsync(d_in, hb, end, hipMemcpyHostToDevice, s));
+            // pre-staged requests left pending are staged again for a later turn
+            for (Request* r : staged) r->stage_buf = -1;
+            staged.clear();
+            staged_end = 0;
+            up ^= 1;
         }
         const double r2 = now_s();
         // new requests are packed, survivors moved, by one launch each over
@@ -331,13 +360,12 @@ struct Server {
         size_t n_pack = 0;
         int max_pack = 0, max_move = 0;
         {
-            size_t in_off = 0;
             for (size_t i = 0; i < active.size(); i++) {
-                const Request& r = *active[i];
+                Request& r = *active[i];
                 const int T = r.dims.tokens();
                 if (r.base < 0) {
-                    jobs.push_back(CellJob{(long)in_off, tab.segs[i].base, T});
-                    in_off += aligned(13 * (size_t)T);
+                    jobs.push_back(CellJob{(long)r.stage_off, tab.segs[i].base, T});
+                    r.stage_buf = -1;
                     max_pack = std::max(max_pack, T);
                 }
             }
@@ -472,6 +500,40 @@ struct Server {
                           1e3 * (r1 - r0), 1e3 * (r2 - r1), 1e3 * (r3 - r2), 1e3 * (r4 - r3), 1e3 * (now_s() - r4));
             ph_detail += b;
         }
+    }
+
+    // While the GPU runs an iteration: copy the next pending requests (in
+    // admission order, up to max_active and the buffer's size) into the
+    // staging buffer the next repack uploads from. Only the worker admits
+    // requests, so the ones it takes here stay valid after the lock.
+    void prestage() {
+        std::vector<Request*> take;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            size_t end = staged_end;
+            size_t n = staged.size();
+            for (auto& p : pending) {
+                if ((int)n >= max_active) break;
+                Request* r = p.get();
+                if (r->stage_buf == up) continue;
+                const size_t b = aligned(13 * (size_t)r->dims.tokens());
+                if (end + b > cap_stage[up]) break;
+                r->stage_buf = up;
+                r->stage_off = end;
+                end += b;
+                n++;
+                take.push_back(r);
+            }
+            staged_end = end;
+        }
+        std::vector<Copy> jobs;
+        for (Request* r : take) {
+            const size_t T = (size_t)r->dims.tokens();
+            jobs.push_back(Copy{h_stage[up] + r->stage_off, r->grid.data(), 12 * T});
+            jobs.push_back(Copy{h_stage[up] + r->stage_off + 12 * T, r->known.data(), T});
+        }
+        copy_all(jobs);
+        staged.insert(staged.end(), take.begin(), take.end());
     }
 
     // One iteration xc -> xn, then the names swap.
@@ -650,6 +712,7 @@ struct Server {
             const double t_b = now_s();
             run_iteration();
             HIPCHK(hipMemcpyAsync(h_resid, pack.resid, 4 * active.size(), hipMemcpyDeviceToHost, m.stream));
+            prestage();  // beside the iteration
             HIPCHK(hipStreamSynchronize(m.stream));
             const double t_c = now_s();
             ph_turns += 1;
@@ -739,7 +802,8 @@ int j2k_server_create(void* model, size_t budget, int max_active, void** out) {
             DeviceGuard dg(m.device);
             const size_t tokens = std::min<size_t>(budget / bytes_per_token(m), size_t(1) << 24);
             const size_t slack = 256 * (size_t)std::min(max_active, 4096);
-            s->reserve_host(s->h_in, s->cap_hin, 13 * tokens + 2 * slack);
+            s->reserve_host(s->h_stage[0], s->cap_stage[0], 13 * tokens + 2 * slack);
+            s->reserve_host(s->h_stage[1], s->cap_stage[1], 13 * tokens + 2 * slack);
             s->reserve_host(s->h_out, s->cap_hout, 12 * tokens + 2 * slack);
             // the turn's tables: about 1.2 bytes a token (attention tiles, cell tiles) and a few per request
             s->reserve_host(s->h_tab, s->cap_htab, 2 * tokens + 64 * slack);

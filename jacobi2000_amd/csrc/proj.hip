@@ -225,256 +225,220 @@ void cell_targets(hipStream_t s, int n_slices, const Slice* sl, const float* x0,
     LAUNCHCHK("cell_targets");
 }
 
-// k_cell_project for a slice of at most CPT*K cells: the same passes in the
-// same order, with each thread's cells (tid, tid + CPT, ...) held in
-// registers from the first load to the one store, so a pass costs its
-// arithmetic and one block reduction instead of a round trip to memory.
-template <int K>
-__device__ __forceinline__ void cell_project_regs(const Slice& S, float4* __restrict__ x,
-                                                  const float4* __restrict__ p, double mass, double eref,
-                                                  unsigned* __restrict__ resid, unsigned long long* __restrict__ viol,
-                                                  const Phys& ph, double (*sh)[CPT / 64], float* shm,
-                                                  float* __restrict__ grid, long M) {
+// The blank-cell projection of one slice (mass, then energy) is planned
+// from eight f64 moments of the slice taken in one pass over the cells it
+// holds before the projection: over all cells the sum of normalized h;
+// over known cells the count and the potential and kinetic energy sums;
+// over blank cells the count and the sums of physical h, h^2, q = u^2 + v^2
+// and h q. The mass shift delta (normalized) moves blank h by ds = delta s0,
+// so after it sum h = Sh + nb ds, sum h^2 = Sh2 + 2 ds Sh + nb ds^2 and
+// sum h q = Shq + ds Sq give the blank energies without a second pass; the
+// anomaly sum of the shrink case is sum h^2 - nb hbar^2. One block
+// reduction plans the projection instead of one per pass.
+enum { MO_X, MO_NB, MO_PK, MO_KK, MO_H, MO_H2, MO_Q, MO_HQ, MO_N };
+
+struct ProjPlan {
+    double delta = 0, su = 1, hbar = 0, lam = 1;
+    bool shift = false, scale = false, shrink = false;
+};
+
+__device__ __forceinline__ void add_moments(double (&mo)[MO_N], const float4& c, const Phys& ph) {
+    double h, u, w;
+    phys_huv(c, ph, h, u, w);
+    const double q = u * u + w * w;
+    // weights instead of branches: a branch here becomes a select of the
+    // array index, which puts the moments in scratch memory
+    const double kn = c.w != 0.f ? 1.0 : 0.0, bl = 1.0 - kn;
+    mo[MO_X] += (double)c.x;
+    mo[MO_PK] += kn * (0.5 * (double)ph.g * h * h);
+    mo[MO_KK] += kn * (0.5 * h * q);
+    mo[MO_NB] += bl;
+    mo[MO_H] += bl * h;
+    mo[MO_H2] += bl * (h * h);
+    mo[MO_Q] += bl * q;
+    mo[MO_HQ] += bl * (h * q);
+}
+
+__device__ __forceinline__ ProjPlan plan_projection(const double (&mo)[MO_N], int n, double mass, double eref,
+                                                    const Phys& ph) {
+    ProjPlan pl;
+    const double nb = mo[MO_NB];
+    if (!(nb > 0)) return pl;
+    const double g = ph.g;
+    pl.shift = true;
+    pl.delta = (mass * (double)n - mo[MO_X]) / nb;
+    const double ds = pl.delta * (double)ph.stdv[0];
+    const double sh = mo[MO_H] + nb * ds, sh2 = mo[MO_H2] + 2.0 * ds * mo[MO_H] + nb * ds * ds;
+    const double pb = 0.5 * g * sh2, kb = 0.5 * (mo[MO_HQ] + ds * mo[MO_Q]);
+    const double bound = (double)ph.r_max * eref;
+    if (mo[MO_PK] + mo[MO_KK] + pb + kb > bound) {
+        const double target = bound * (1.0 - 1e-7);
+        const double budget = target - mo[MO_PK] - mo[MO_KK] - pb;
+        pl.scale = true;
+        pl.su = budget >= 0 ? fmin(sqrt(budget / fmax(kb, 1e-12)), 1.0) : 0.0;
+        if (budget < 0) {
+            pl.shrink = true;
+            pl.hbar = sh / nb;
+            const double a = fmax(sh2 - nb * pl.hbar * pl.hbar, 0.0);
+            const double avail = target - mo[MO_PK] - mo[MO_KK] - 0.5 * g * nb * pl.hbar * pl.hbar;
+            pl.lam = (avail > 0 && a > 0) ? fmin(sqrt(avail / (0.5 * g * a)), 1.0) : 0.0;
+        }
+    }
+    return pl;
+}
+
+// The plan applied to one cell (blank cells only): shift h, then scale
+// (u, v), then shrink the h anomaly about hbar; float rounding after each.
+__device__ __forceinline__ void apply_projection(float4& c, const ProjPlan& pl, const Phys& ph) {
+    if (c.w != 0.f) return;
     const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
-    const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
-    float4 c[K];
-    bool in[K];
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        const int i = threadIdx.x + CPT * k;
-        in[k] = i < S.n;
-        c[k] = in[k] ? x[i] : make_float4(0.f, 0.f, 0.f, 1.f);
+    const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2];
+    if (pl.shift) c.x = (float)((double)c.x + pl.delta);
+    if (pl.scale) {
+        const double u = (double)c.y * s1 + m1, w = (double)c.z * s2 + m2;
+        c.y = (float)((u * pl.su - m1) / s1);
+        c.z = (float)((w * pl.su - m2) / s2);
     }
-    double v[2] = {0, 0};
-#pragma unroll
-    for (int k = 0; k < K; k++)
-        if (in[k]) {
-            v[0] += (double)c[k].x;
-            v[1] += c[k].w != 0.f ? 0.0 : 1.0;
-        }
-    block_sum<2, CPT>(v, sh);
-    const double nb = v[1];
-    if (nb > 0) {
-        const double delta = (mass * (double)S.n - v[0]) / nb;
-#pragma unroll
-        for (int k = 0; k < K; k++)
-            if (c[k].w == 0.f) c[k].x = (float)((double)c[k].x + delta);
-        double e[5] = {0, 0, 0, 0, 0};  // pk, kk, pb, kb, sum of blank h
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            if (!in[k]) continue;
-            double h, u, w;
-            phys_huv(c[k], ph, h, u, w);
-            const double pp = 0.5 * g * h * h, kq = 0.5 * h * (u * u + w * w);
-            if (c[k].w != 0.f) {
-                e[0] += pp;
-                e[1] += kq;
-            } else {
-                e[2] += pp;
-                e[3] += kq;
-                e[4] += h;
-            }
-        }
-        block_sum<5, CPT>(e, sh);
-        const double bound = (double)ph.r_max * eref;
-        if (e[0] + e[1] + e[2] + e[3] > bound) {
-            const double target = bound * (1.0 - 1e-7);
-            const double budget = target - e[0] - e[1] - e[2];
-            double su = 0.0;
-            if (budget >= 0) su = fmin(sqrt(budget / fmax(e[3], 1e-12)), 1.0);
-#pragma unroll
-            for (int k = 0; k < K; k++) {
-                if (c[k].w != 0.f) continue;
-                const double u = (double)c[k].y * s1 + m1, w = (double)c[k].z * s2 + m2;
-                c[k].y = (float)((u * su - m1) / s1);
-                c[k].z = (float)((w * su - m2) / s2);
-            }
-            if (budget < 0) {
-                const double hbar = e[4] / nb;
-                double a[1] = {0};
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if (c[k].w != 0.f) continue;
-                    const double d = (double)c[k].x * s0 + m0 - hbar;
-                    a[0] += d * d;
-                }
-                block_sum<1, CPT>(a, sh);
-                const double avail = target - e[0] - e[1] - 0.5 * g * nb * hbar * hbar;
-                double lam = 0.0;
-                if (avail > 0 && a[0] > 0) lam = fmin(sqrt(avail / (0.5 * g * a[0])), 1.0);
-#pragma unroll
-                for (int k = 0; k < K; k++) {
-                    if (c[k].w != 0.f) continue;
-                    const double h = (double)c[k].x * s0 + m0;
-                    c[k].x = (float)(((hbar + lam * (h - hbar)) - m0) / s0);
-                }
-            }
-        }
-    }
-    if (viol) {  // k_cell_violation's sum, in its order, on the final values
-        double v[1] = {0};
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            if (!in[k]) continue;
-            double h, u, w;
-            phys_huv(c[k], ph, h, u, w);
-            v[0] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
-        }
-        block_sum<1, CPT>(v, sh);
-        if (threadIdx.x == 0)
-            atomicMax(viol + S.req, (unsigned long long)__double_as_longlong(fmax((v[0] - (double)ph.r_max * eref) / eref, 0.0)));
-    }
-    float r = 0.f;
-#pragma unroll
-    for (int k = 0; k < K; k++) {
-        if (!in[k]) continue;
-        x[threadIdx.x + CPT * k] = c[k];
-        if (grid) {
-            const long m = (long)S.start + threadIdx.x + CPT * k;
-            grid[m] = c[k].x;
-            grid[M + m] = c[k].y;
-            grid[2 * M + m] = c[k].z;
-        }
-        const float4 b = p[threadIdx.x + CPT * k];
-        if (c[k].w != 0.f) continue;
-        r = fmaxf(r, fmaxf(fabsf(c[k].x - b.x), fmaxf(fabsf(c[k].y - b.y), fabsf(c[k].z - b.z))));
-    }
-    r = wave_max(r);
-    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
-        for (int w = 0; w < CPT / 64; w++) t = fmaxf(t, shm[w]);
-        atomicMax(resid + S.req, __float_as_uint(t));
+    if (pl.shrink) {
+        const double h = (double)c.x * s0 + m0;
+        c.x = (float)(((pl.hbar + pl.lam * (h - pl.hbar)) - m0) / s0);
     }
 }
 
-// Blank-cell projection of one slice: mass shift, then scale blank (u, v)
-// down to r_max*Eref*(1-1e-7); if potential alone is over it, zero blank
-// (u, v) and shrink blank h anomalies about their mean (mass unchanged).
-// Then the slice's max |xn - xc| over blank cells. Slices of up to 4096
-// cells keep their cells in registers (cell_project_regs). With grid, the
-// projected cells are also written to grid [3][M] (the unpack).
+// Block totals of the moments, every thread gets them: wave sums, then one
+// thread per moment adds the waves' partials in wave order.
+__device__ __forceinline__ void moments_reduce(double (&mo)[MO_N], double (*sh)[CPT / 64], double* tot) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < MO_N; j++) {
+        const double t = wave_sum_d(mo[j]);
+        if (lane == 0) sh[j][wave] = t;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < MO_N) {
+        double t = 0;
+        for (int w = 0; w < CPT / 64; w++) t += sh[threadIdx.x][w];
+        tot[threadIdx.x] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < MO_N; j++) mo[j] = tot[j];
+}
+
+// Block totals of the energy of the projected cells (the violation sum) and
+// of the residual max, in one round: thread 0 gets both.
+__device__ __forceinline__ void viol_resid_reduce(double& v, float& r, double* shv, float* shr) {
+    v = wave_sum_d(v);
+    r = wave_max(r);
+    if ((threadIdx.x & 63) == 0) {
+        shv[threadIdx.x >> 6] = v;
+        shr[threadIdx.x >> 6] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tv = 0;
+        float tr = 0.f;
+        for (int w = 0; w < CPT / 64; w++) {
+            tv += shv[w];
+            tr = fmaxf(tr, shr[w]);
+        }
+        v = tv;
+        r = tr;
+    }
+}
+
+__device__ __forceinline__ void finish_slice(double v, float r, const Slice& S, double eref, const Phys& ph,
+                                             unsigned* __restrict__ resid, unsigned long long* __restrict__ viol) {
+    if (threadIdx.x != 0) return;
+    if (viol)
+        atomicMax(viol + S.req,
+                  (unsigned long long)__double_as_longlong(fmax((v - (double)ph.r_max * eref) / eref, 0.0)));
+    atomicMax(resid + S.req, __float_as_uint(r));
+}
+
+// Blank-cell projection of one slice, planned from its moments (above):
+// mass shift, then scale blank (u, v) down to r_max*Eref*(1-1e-7); if
+// potential alone is over it, zero blank (u, v) and shrink blank h
+// anomalies about their mean (mass unchanged). Then the slice's energy on
+// the projected cells (the violation) and max |xn - xc| over blank cells.
+// Slices of up to 4096 cells keep their cells in registers from the one
+// load to the one store; larger ones take two passes over memory. With
+// grid, the projected cells are also written to grid [3][M] (the unpack).
 __global__ __launch_bounds__(CPT) void k_cell_project(const Slice* __restrict__ sl, float4* __restrict__ xn,
                                                      const float4* __restrict__ xc, const double* __restrict__ mass_t,
                                                      const double* __restrict__ eref_t, unsigned* __restrict__ resid,
                                                      unsigned long long* __restrict__ viol, Phys ph,
                                                      float* __restrict__ grid, long M) {
-    __shared__ double sh[5][CPT / 64];
+    __shared__ double sh[MO_N][CPT / 64];
+    __shared__ double tot[MO_N];
     __shared__ float shm[CPT / 64];
     const Slice S = sl[blockIdx.x];
     float4* x = xn + S.start;
-    if (S.n <= 4 * CPT) {
-        cell_project_regs<4>(S, x, xc + S.start, mass_t[blockIdx.x], eref_t[blockIdx.x], resid, viol, ph, sh, shm,
-                             grid, M);
-        return;
-    }
-    const double s0 = ph.stdv[0], s1 = ph.stdv[1], s2 = ph.stdv[2];
-    const double m0 = (double)ph.mean[0] + (double)ph.depth, m1 = ph.mean[1], m2 = ph.mean[2], g = ph.g;
-    {
-        double v[2] = {0, 0};
-        for (int i = threadIdx.x; i < S.n; i += CPT) {
-            const float4 c = x[i];
-            v[0] += (double)c.x;
-            v[1] += c.w != 0.f ? 0.0 : 1.0;
-        }
-        block_sum<2, CPT>(v, sh);
-        const double nb = v[1];
-        if (nb > 0) {
-            const double delta = (mass_t[blockIdx.x] * (double)S.n - v[0]) / nb;
-            for (int i = threadIdx.x; i < S.n; i += CPT)
-                if (x[i].w == 0.f) x[i].x = (float)((double)x[i].x + delta);
-            __syncthreads();
-            double e[5] = {0, 0, 0, 0, 0};  // pk, kk, pb, kb, sum of blank h
-            for (int i = threadIdx.x; i < S.n; i += CPT) {
-                const float4 c = x[i];
-                double h, u, w;
-                phys_huv(c, ph, h, u, w);
-                const double pp = 0.5 * g * h * h, kq = 0.5 * h * (u * u + w * w);
-                if (c.w != 0.f) {
-                    e[0] += pp;
-                    e[1] += kq;
-                } else {
-                    e[2] += pp;
-                    e[3] += kq;
-                    e[4] += h;
-                }
-            }
-            block_sum<5, CPT>(e, sh);
-            const double bound = (double)ph.r_max * eref_t[blockIdx.x];
-            if (e[0] + e[1] + e[2] + e[3] > bound) {
-                const double target = bound * (1.0 - 1e-7);
-                const double budget = target - e[0] - e[1] - e[2];
-                double su = 0.0;
-                if (budget >= 0) su = fmin(sqrt(budget / fmax(e[3], 1e-12)), 1.0);
-                for (int i = threadIdx.x; i < S.n; i += CPT) {
-                    float4 c = x[i];
-                    if (c.w != 0.f) continue;
-                    const double u = (double)c.y * s1 + m1, w = (double)c.z * s2 + m2;
-                    c.y = (float)((u * su - m1) / s1);
-                    c.z = (float)((w * su - m2) / s2);
-                    x[i] = c;
-                }
-                if (budget < 0) {
-                    const double hbar = e[4] / nb;
-                    double a[1] = {0};
-                    for (int i = threadIdx.x; i < S.n; i += CPT) {
-                        const float4 c = x[i];
-                        if (c.w != 0.f) continue;
-                        const double d = (double)c.x * s0 + m0 - hbar;
-                        a[0] += d * d;
-                    }
-                    block_sum<1, CPT>(a, sh);
-                    const double avail = target - e[0] - e[1] - 0.5 * g * nb * hbar * hbar;
-                    double lam = 0.0;
-                    if (avail > 0 && a[0] > 0) lam = fmin(sqrt(avail / (0.5 * g * a[0])), 1.0);
-                    for (int i = threadIdx.x; i < S.n; i += CPT) {
-                        float4 c = x[i];
-                        if (c.w != 0.f) continue;
-                        const double h = (double)c.x * s0 + m0;
-                        c.x = (float)(((hbar + lam * (h - hbar)) - m0) / s0);
-                        x[i] = c;
-                    }
-                }
-                __syncthreads();
-            }
-        }
-    }
-    if (viol) {  // k_cell_violation's sum on the final values (each thread reads what it wrote)
-        double v[1] = {0};
-        for (int i = threadIdx.x; i < S.n; i += CPT) {
-            double h, u, w;
-            phys_huv(x[i], ph, h, u, w);
-            v[0] += 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
-        }
-        block_sum<1, CPT>(v, sh);
-        if (threadIdx.x == 0) {
-            const double e = eref_t[blockIdx.x];
-            atomicMax(viol + S.req, (unsigned long long)__double_as_longlong(fmax((v[0] - (double)ph.r_max * e) / e, 0.0)));
-        }
-    }
-    // Residual over blank cells (all three channels), max is order-free.
-    float r = 0.f;
     const float4* p = xc + S.start;
-    for (int i = threadIdx.x; i < S.n; i += CPT) {
-        const float4 a = x[i], b = p[i];
+    const double mass = mass_t[blockIdx.x], eref = eref_t[blockIdx.x];
+    auto store = [&](int i, const float4& c) {
+        x[i] = c;
         if (grid) {
             const long m = (long)S.start + i;
-            grid[m] = a.x;
-            grid[M + m] = a.y;
-            grid[2 * M + m] = a.z;
+            grid[m] = c.x;
+            grid[M + m] = c.y;
+            grid[2 * M + m] = c.z;
         }
-        if (a.w != 0.f) continue;
-        r = fmaxf(r, fmaxf(fabsf(a.x - b.x), fmaxf(fabsf(a.y - b.y), fabsf(a.z - b.z))));
+    };
+    auto energy = [&](const float4& c) {
+        double h, u, w;
+        phys_huv(c, ph, h, u, w);
+        return 0.5 * (double)ph.g * h * h + 0.5 * h * (u * u + w * w);
+    };
+    double mo[MO_N] = {0, 0, 0, 0, 0, 0, 0, 0};
+    constexpr int K = 4;
+    if (S.n <= K * CPT) {
+        float4 c[K], b[K];  // the cells, and the previous iterate's (read now, used for the residual)
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = threadIdx.x + CPT * k;
+            if (i < S.n) {
+                c[k] = x[i];
+                b[k] = p[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; k++)
+            if (threadIdx.x + CPT * k < S.n) add_moments(mo, c[k], ph);
+        moments_reduce(mo, sh, tot);
+        const ProjPlan pl = plan_projection(mo, S.n, mass, eref, ph);
+        double v = 0;
+        float r = 0.f;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const int i = threadIdx.x + CPT * k;
+            if (i >= S.n) continue;
+            apply_projection(c[k], pl, ph);
+            store(i, c[k]);
+            if (viol) v += energy(c[k]);
+            if (c[k].w != 0.f) continue;
+            r = fmaxf(r, fmaxf(fabsf(c[k].x - b[k].x), fmaxf(fabsf(c[k].y - b[k].y), fabsf(c[k].z - b[k].z))));
+        }
+        viol_resid_reduce(v, r, sh[0], shm);
+        finish_slice(v, r, S, eref, ph, resid, viol);
+        return;
     }
-    r = wave_max(r);
-    if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = r;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float t = 0.f;
-        for (int w = 0; w < CPT / 64; w++) t = fmaxf(t, shm[w]);
-        atomicMax(resid + S.req, __float_as_uint(t));
+    for (int i = threadIdx.x; i < S.n; i += CPT) add_moments(mo, x[i], ph);
+    moments_reduce(mo, sh, tot);
+    const ProjPlan pl = plan_projection(mo, S.n, mass, eref, ph);
+    double v = 0;
+    float r = 0.f;
+    for (int i = threadIdx.x; i < S.n; i += CPT) {
+        float4 c = x[i];
+        apply_projection(c, pl, ph);
+        store(i, c);
+        if (viol) v += energy(c);
+        if (c.w != 0.f) continue;
+        const float4 b = p[i];
+        r = fmaxf(r, fmaxf(fabsf(c.x - b.x), fmaxf(fabsf(c.y - b.y), fabsf(c.z - b.z))));
     }
+    viol_resid_reduce(v, r, sh[0], shm);
+    finish_slice(v, r, S, eref, ph, resid, viol);
 }
 
 void cell_project(hipStream_t s, int n_slices, const Slice* sl, float* xn, const float* xc, const double* mass_t,

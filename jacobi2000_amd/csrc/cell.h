@@ -40,8 +40,15 @@ struct CellTables {
     std::vector<Slice> slices;
     std::vector<int> tok_req;
     int M = 0;
-    void build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req);
+    // generic_tiles false: leave `tiles` empty (the tiled cell attention,
+    // which reads ctiles, takes every launch)
+    void build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req, bool generic_tiles = true);
 };
+
+// Whether the model's attention can fall back to the generic gather (and so
+// needs its token tiles): false when the tiled cell attention exists for the
+// model's head size and stencil in its default staging mode.
+bool cell_needs_generic_tiles(const Model& m);
 
 void check_cell_dims(const Model& m, const CellDims& d);
 

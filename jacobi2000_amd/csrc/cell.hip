@@ -5,7 +5,7 @@
 
 namespace j2k {
 
-void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req) {
+void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_token_req, bool generic_tiles) {
     segs.clear();
     slices.clear();
     tok_req.clear();
@@ -18,8 +18,15 @@ void CellTables::build(const std::vector<CellDims>& reqs, int tpb, bool per_toke
         if (per_token_req) tok_req.insert(tok_req.end(), (size_t)d.tokens(), (int)r);
         M += d.tokens();
     }
-    build_tiles(segs, tpb, tiles);
+    tiles.clear();
+    if (generic_tiles) build_tiles(segs, tpb, tiles);
     build_cell_tiles(segs, ctiles);
+}
+
+bool cell_needs_generic_tiles(const Model& m) {
+    const AttnGeo g = m.geo();
+    const int elem = m.bf16() ? 2 : 4;  // the activations the attention reads (f32 in every other form)
+    return !(g.kind == ATTN_CELL_ST && attn_cell_ok(m.cfg.dim / m.cfg.heads, g.radius, g.time_radius, elem, KV_JOINT));
 }
 
 void check_cell_dims(const Model& m, const CellDims& d) {

@@ -308,7 +308,8 @@ struct Server {
         for (auto& r : active) dims.push_back(r->dims);
         const bool multi = active.size() > 1;
         const double r0 = now_s();
-        tab.build(dims, attn_tokens_per_block(m.cfg.dim), false);  // tok_req is made on the device
+        // tok_req is made on the device; the generic gather's tiles only when it may run
+        tab.build(dims, attn_tokens_per_block(m.cfg.dim), false, cell_needs_generic_tiles(m));
         const double r1 = now_s();
         const size_t M = (size_t)tab.M;
         const bool grow = M > cap_tok;

@@ -96,6 +96,11 @@ void rows_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArg
 // arithmetic with the intermediate kept on chip.
 bool mlp_fused_ok(int dim, int ff);
 void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
+// dim == 256, everything of a block between two attentions in one launch: X
+// += O @ Wp + bp, H = LN2(X), X += ff2(GELU(ff1(H))) + b2 (stored), and with
+// a.Wqkv the next block's LN1 and QKV; the same bits as the tile GEMMs, the
+// fused MLP and k_layernorm (weights in fragment order).
+void rows256(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
 // QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
 // arithmetic: the first layer's product after the embedding.
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);

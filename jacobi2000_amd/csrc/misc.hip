@@ -6,10 +6,15 @@
 namespace j2k {
 
 // ---- LayerNorm: one wave per row, cpu.rs:49-65 (two-pass mean/variance,
-// biased, eps 1e-5 inside the sqrt). dim <= 1024.
+// biased, eps 1e-5 inside the sqrt). dim <= 1024. Each lane adds its values
+// d = lane + 64 i in order, the wave butterfly (wave_sum) adds the lanes;
+// squares are rounded before they are added; the output is
+// fma((v - mean) * inv, gamma, beta). The d = 256 row-panel kernel
+// computes the same statistics in the same order.
 __global__ __launch_bounds__(256) void k_layernorm(int rows, int dim, const float* __restrict__ x,
                                                    const float* __restrict__ g, const float* __restrict__ b,
                                                    float* __restrict__ out, uint16_t* __restrict__ out_bf) {
+#pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= rows) return;
@@ -35,7 +40,7 @@ __global__ __launch_bounds__(256) void k_layernorm(int rows, int dim, const floa
     for (int i = 0; i < 16; i++) {
         int d = lane + i * 64;
         if (d < dim) {
-            float o = (v[i] - mean) * inv * g[d] + b[d];
+            const float o = __builtin_fmaf((v[i] - mean) * inv, g[d], b[d]);
             if (out) out[(long)row * dim + d] = o;
             if (out_bf) out_bf[(long)row * dim + d] = f2bf(o);
         }

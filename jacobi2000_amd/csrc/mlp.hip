@@ -662,6 +662,206 @@ __global__ __launch_bounds__(512, 1) void k_mlp_rows(int M, int ff, RowsArgs a) 
     }
 }
 
+// One block of the wider trunk (D = 256) between two attentions, over
+// 64-token panels, eight waves (32 columns of every D-column product and 16
+// of every 128-column ff1 chunk each): X += O @ Wp + bp; H = LN2(X);
+// X += ff2(GELU(ff1(H))) + b2 (X stored); with QKV, H = LN1'(X) and
+// QKV = H @ Wqkv' + bqkv'. Every product is the tile GEMM's dot product in
+// its order, with its bias and residual arithmetic, and each LayerNorm's
+// statistics are taken as k_layernorm takes them (the panel's residual rows
+// go through LDS so one wave reads a row as k_layernorm's lanes do), so the
+// launch gives the same bits as the tile GEMMs, the MLP launch and the
+// LayerNorm launches it replaces. LDS: one 64 KB panel (O, then X as f32
+// for the statistics, then H) and one 128-column intermediate chunk (96 KB
+// split: one block per CU).
+template <bool SPLIT, int D>
+__device__ __forceinline__ void ln_panel_k(const f4v (&xm)[4][D / 128], float* Xs, float* mrow, float* irow, int wn,
+                                           int r16, int c4, int wave, int lane) {
+#pragma clang fp contract(off)
+    constexpr int J = D / 128;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < J; j++)
+            *reinterpret_cast<f4v*>(&Xs[(i * 16 + r16) * D + wn + j * 16 + c4]) = xm[i][j];
+    __syncthreads();
+    // k_layernorm's order: lane l adds d = l + 64 k in k order, then the
+    // butterfly; the rows of the panel go to the eight waves in turn
+#pragma unroll 1
+    for (int r = wave; r < RB; r += 8) {
+        const float* xr = Xs + r * D;
+        float v[D / 64];
+        float sum = 0.f;
+#pragma unroll
+        for (int k = 0; k < D / 64; k++) {
+            v[k] = xr[lane + 64 * k];
+            sum += v[k];
+        }
+        const float mean = wave_sum(sum) / (float)D;
+        float q = 0.f;
+#pragma unroll
+        for (int k = 0; k < D / 64; k++) {
+            const float c = v[k] - mean;
+            q += c * c;
+        }
+        const float inv = 1.0f / sqrtf(wave_sum(q) / (float)D + 1e-5f);
+        if (lane == 0) {
+            mrow[r] = mean;
+            irow[r] = inv;
+        }
+    }
+    __syncthreads();  // every read of Xs is done; the statistics are in
+}
+
+template <bool SPLIT, bool QKV>
+__global__ __launch_bounds__(512, 1) void k_rows256(int M, int ff, RowsArgs a) {
+    constexpr int D = 256, WN = 8, NT = 64 * WN, MI = 4, J = D / (16 * WN), P = SPLIT ? 2 : 1;
+    constexpr int FC = 128, K4D = D / 32, K4F = FC / 32;
+    static_assert(J == 2, "eight waves of 32 columns");
+    constexpr int PANEL = RB * D * 4;  // bytes: the f32 residual rows, or the bf16 (pair) panel
+    __shared__ __align__(16) unsigned char As[PANEL];
+    __shared__ __align__(16) uint16_t Fs[P * RB * FC];
+    __shared__ float mrow[RB], irow[RB];
+    uint16_t* const Ah = reinterpret_cast<uint16_t*>(As);
+    uint16_t* const Al = Ah + (SPLIT ? RB * D : 0);
+    float* const Xs = reinterpret_cast<float*>(As);
+    uint16_t *const Fh = Fs, *const Fl = Fs + (SPLIT ? RB * FC : 0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, rows = min(RB, M - m0);
+    const int wn = wave * 16 * J, wf = wave * 16;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    const auto rX = rows_rsrc(a.X + (long)m0 * D, rows * D * 4);
+    auto at = [&](int i, int j) { return ((i * 16 + r16) * D + wn + j * 16 + c4) * 4; };
+    // a panel of D columns from global (f32 split into hi + lo, or bf16) into Ah / Al
+    auto stage = [&](const void* src) {
+        if (SPLIT) {
+            const auto r = rows_rsrc(static_cast<const float*>(src) + (long)m0 * D, rows * D * 4);
+#pragma unroll
+            for (int q = 0; q < RB * D / 4 / NT; q++) {
+                const int v = tid + NT * q, row = v / (D / 4), kv = v % (D / 4);
+                put_a4<true>(Ah, Al, row, kv * 4, bload4(r, (row * D + kv * 4) * 4));
+            }
+        } else {
+            const auto r = rows_rsrc(static_cast<const uint16_t*>(src) + (long)m0 * D, rows * D * 2);
+#pragma unroll
+            for (int q = 0; q < RB * D / 8 / NT; q++) {
+                const int v = tid + NT * q, row = v / (D / 8), kv = v % (D / 8);
+                const u4v val = __builtin_amdgcn_raw_buffer_load_b128(r, (row * D + kv * 8) * 2, 0, 0);
+                *reinterpret_cast<u4v*>(&Ah[pidx(row, kv * 8)]) = val;
+            }
+        }
+    };
+    // H = LN(xm) with gamma g, beta b into Ah / Al (statistics in mrow / irow)
+    auto put_ln = [&](const f4v(&xm)[MI][J], const float* g, const float* b) {
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v gv = *reinterpret_cast<const f4v*>(g + wn + j * 16 + c4);
+            const f4v bv = *reinterpret_cast<const f4v*>(b + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < MI; i++) {
+                const int row = i * 16 + r16;
+                const float mu = mrow[row], iv = irow[row];
+                f4v h;
+#pragma unroll
+                for (int r = 0; r < 4; r++) h[r] = __builtin_fmaf((xm[i][j][r] - mu) * iv, gv[r], bv[r]);
+                put_a4<SPLIT>(Ah, Al, row, wn + j * 16 + c4, h);
+            }
+        }
+    };
+
+    // ---- X += O @ Wp + bp
+    bf8v w[K4D][J];
+    stage(a.O);
+    load_w<J, K4D>(w, a.Wp, D, 0, 0, wn, lane);
+    __syncthreads();
+    f4v xm[MI][J];
+    {
+        f4v acc[MI][J];
+        zero(acc);
+        mma_panel<SPLIT, MI, J, K4D>(acc, Ah, Al, w, r16, kq);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v b = *reinterpret_cast<const f4v*>(a.bp + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < MI; i++) {
+                const f4v x = bload4(rX, at(i, j));
+#pragma unroll
+                for (int r = 0; r < 4; r++) xm[i][j][r] = x[r] + acc[i][j][r] + b[r];
+            }
+        }
+    }
+    __syncthreads();  // every read of O is done
+    // ---- H = LN2(X)
+    ln_panel_k<SPLIT, D>(xm, Xs, mrow, irow, wn, r16, c4, wave, lane);
+    put_ln(xm, a.n2g, a.n2b);
+    __syncthreads();
+
+    // ---- X += ff2(GELU(ff1(H))) + b2, FC intermediate columns at a time
+    f4v acc2[MI][J];
+    zero(acc2);
+    bf8v v1[K4D][1], w2[K4F][J];
+    load_w<1, K4D>(v1, a.W1, D, 0, 0, wf, lane);
+    for (int c0 = 0; c0 < ff; c0 += FC) {
+        load_w<J, K4F>(w2, a.W2, ff, 0, c0, wn, lane);
+        f4v acc1[MI][1];
+        zero(acc1);
+        mma_panel<SPLIT, MI, 1, K4D>(acc1, Ah, Al, v1, r16, kq);
+        const f4v bb = *reinterpret_cast<const f4v*>(a.b1 + c0 + wf + c4);
+        if (c0 + FC < ff) load_w<1, K4D>(v1, a.W1, D, c0 + FC, 0, wf, lane);
+        __syncthreads();  // every wave has finished reading the previous intermediate chunk
+#pragma unroll
+        for (int i = 0; i < MI; i++) {
+            f4v v;
+#pragma unroll
+            for (int r = 0; r < 4; r++) v[r] = gelu_fast(acc1[i][0][r] + bb[r]);
+            put_a4<SPLIT>(Fh, Fl, i * 16 + r16, wf + c4, v);
+        }
+        __syncthreads();
+        mma_panel<SPLIT, MI, J, K4F>(acc2, Fh, Fl, w2, r16, kq);
+    }
+#pragma unroll
+    for (int j = 0; j < J; j++) {
+        const f4v b = *reinterpret_cast<const f4v*>(a.b2 + wn + j * 16 + c4);
+#pragma unroll
+        for (int i = 0; i < MI; i++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) xm[i][j][r] = xm[i][j][r] + acc2[i][j][r] + b[r];
+            bstore4(rX, at(i, j), xm[i][j]);
+        }
+    }
+    if (!QKV) return;
+
+    // ---- the next block's LN1 and its qkv product (every ff1 read of H is
+    // done: the last chunk's barriers came after it)
+    ln_panel_k<SPLIT, D>(xm, Xs, mrow, irow, wn, r16, c4, wave, lane);
+    load_w<J, K4D>(w, a.Wqkv, D, 0, 0, wn, lane);
+    put_ln(xm, a.n1g, a.n1b);
+    __syncthreads();
+    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * D * (SPLIT ? 4 : 2),
+                              rows * 3 * D * (SPLIT ? 4 : 2));
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        asm volatile("" ::: "memory");  // H's fragments read per chunk, not held across all three
+        f4v acc[MI][J];
+        zero(acc);
+        mma_panel<SPLIT, MI, J, K4D>(acc, Ah, Al, w, r16, kq);
+        if (c < 2) load_w<J, K4D>(w, a.Wqkv, D, (c + 1) * D, 0, wn, lane);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * D + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < MI; i++) {
+                f4v v;
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
+                const int o = (((i * 16 + r16) * 3 + c) * D + wn + j * 16 + c4) * 4;
+                if (SPLIT) bstore4(rQ, o, v);
+                else bstore4(rQ, o / 2, bf4(v));
+            }
+        }
+    }
+}
+
 }  // namespace
 
 bool rows_fused_ok(int dim, int ff) { return dim == RD && ff % RD == 0; }
@@ -748,6 +948,24 @@ void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {
 }
 
 bool mlp_fused_ok(int dim, int ff) { return dim == 256 && ff % 128 == 0; }
+
+void rows256(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a) {
+    if (M <= 0) return;
+    if (!mlp_fused_ok(dim, ff)) fail(J2K_ERR_LIMIT, "d = 256 row-panel kernel needs dim == 256 and ff a multiple of 128");
+    if (!a.O || !a.Wp || !a.bp || !a.n2g || !a.n2b || !a.W1 || !a.b1 || !a.W2 || !a.b2 || !a.X)
+        fail(J2K_ERR_INVALID, "d = 256 row-panel kernel: missing operand");
+    const bool qkv = a.Wqkv != nullptr;
+    if (qkv && (!a.bqkv || !a.QKV || !a.n1g || !a.n1b)) fail(J2K_ERR_INVALID, "d = 256 row-panel kernel: missing qkv operand");
+    const dim3 grid(cdiv(M, RB)), block(512);
+    if (split) {
+        if (qkv) hipLaunchKernelGGL((k_rows256<true, true>), grid, block, 0, s, M, ff, a);
+        else hipLaunchKernelGGL((k_rows256<true, false>), grid, block, 0, s, M, ff, a);
+    } else {
+        if (qkv) hipLaunchKernelGGL((k_rows256<false, true>), grid, block, 0, s, M, ff, a);
+        else hipLaunchKernelGGL((k_rows256<false, false>), grid, block, 0, s, M, ff, a);
+    }
+    LAUNCHCHK("rows256");
+}
 
 void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a) {
     if (M <= 0) return;

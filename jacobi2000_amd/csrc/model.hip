@@ -316,6 +316,8 @@ Model* create_model(const uint8_t* st, size_t n, const char* cfg_json, int devic
                 L.w1_f = up.bf16_frag(get(w, p + "ff1.weight").data, d, ff);
                 L.w2_f = up.bf16_frag(get(w, p + "ff2.weight").data, ff, d);
             } else if (mlp_fused_ok(d, ff)) {
+                L.wqkv_f = up.bf16_frag(wqkv, d, 3 * d);
+                L.wp_f = up.bf16_frag(get(w, p + "proj.weight").data, d, d);
                 L.w1_f = up.bf16_frag(get(w, p + "ff1.weight").data, d, ff);
                 L.w2_f = up.bf16_frag(get(w, p + "ff2.weight").data, ff, d);
             }
@@ -397,8 +399,10 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
     Act* F = rows ? nullptr : m.get<Act>(S_F, (size_t)M * ff);  // the row kernel keeps it on chip
     const double g = 2.0 * M * d, ln_b = (double)M * d * (4 + sizeof(Act)), at_b = 4.0 * sizeof(Act) * M * d;
     const int rows_pick = rows ? m.tune.choice(TO_ROWS, M) : 0;
-    // d = 256: the MLP in one launch (the tuner's pick 1) or as two tile GEMMs
-    const bool mlp = mlp_fused_ok(d, ff) && !m.L.empty() && m.L[0].w1_f && m.tune.choice(TO_ROWS, M) == 1;
+    // d = 256: everything between two attentions in one launch (the tuner's
+    // pick 2), the MLP in one launch (1), or tile GEMMs and LayerNorm launches
+    const int wpick = mlp_fused_ok(d, ff) && !m.L.empty() && m.L[0].w1_f ? m.tune.choice(TO_ROWS, M) : 0;
+    const bool mlp = wpick == 1, full = wpick == 2;
     const bool fused = rows_pick >= ROWS_4W;
     const int rows_shape = fused ? rows_pick : ROWS_4W;
     if (emb && !(ready == TRUNK_QKV && fused && rows_shape == ROWS_4W && nl > 1))
@@ -433,7 +437,7 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
     for (int l = 0; l < nl; l++) {
         const LayerW& L = m.L[l];
         const LayerW* next = l + 1 < nl ? &m.L[l + 1] : nullptr;
-        if (l > 0 && !fused) gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
+        if (l > 0 && !fused && !full) gemm(TO_GEMM_QKV, 3 * d, d, Hh, L.wqkv_t, L.bqkv, QKV, EPI_STORE, nullptr, nullptr);
         {
             TScope t(m, s, KC_ATTN, at_b);
             attend<Act>(m, s, M, QKV, O, ap);
@@ -478,6 +482,31 @@ bool trunk_lowp(Model& m, hipStream_t s, int M, float* X, int ready, const AttnP
             }
             TScope t(m, s, cls, bytes);
             rows_fused(s, M, d, ff, split, a, rows_shape);
+            continue;
+        }
+        if (full) {
+            RowsArgs a;
+            a.O = O;
+            a.Wp = L.wp_f;
+            a.bp = L.bp;
+            a.n2g = L.n2g;
+            a.n2b = L.n2b;
+            a.W1 = L.w1_f;
+            a.b1 = L.b1;
+            a.W2 = L.w2_f;
+            a.b2 = L.b2;
+            a.X = X;
+            double bytes = (double)M * d * (sizeof(Act) + 8.0);  // O in, X read and written
+            if (next) {
+                a.n1g = next->n1g;
+                a.n1b = next->n1b;
+                a.Wqkv = next->wqkv_f;
+                a.bqkv = next->bqkv;
+                a.QKV = QKV;
+                bytes += 3.0 * M * d * sizeof(Act);
+            }
+            TScope t(m, s, next ? KC_ROWS : KC_ROWS_TAIL, bytes);
+            rows256(s, M, d, ff, split, a);
             continue;
         }
         if (d == 128) {

@@ -165,22 +165,54 @@ void tune_bucket(Model& m, int b, Events& ev) {
             });
         settle(TO_ROWS);
     }
-    if (mlp_fused_ok(d, ff) && L.w1_f) {
-        // the MLP (H -> X) as two tile GEMMs with the block height just chosen, or in one launch
+    if (mlp_fused_ok(d, ff) && L.w1_f && m.cfg.layers > 1) {
+        // the token-local work between two attentions (proj + residual, LN2,
+        // MLP, LN1 of the next block, its qkv): tile GEMMs and LayerNorm
+        // launches with the block heights just chosen (0), the MLP in one
+        // launch (1), or all of it in one launch (2)
+        const LayerW& N1 = m.L[1];
+        const int bq = GEMM_BM[m.tune.pick[TO_GEMM_QKV][b]], bp = GEMM_BM[m.tune.pick[TO_GEMM_PROJ][b]];
         const int bf = GEMM_BM[m.tune.pick[TO_GEMM_FF][b]];
+        auto norm = [&](const float* g, const float* be) {
+            if (split) layernorm(s, (int)M, d, X, g, be, (float*)Hh, nullptr);
+            else layernorm(s, (int)M, d, X, g, be, nullptr, (uint16_t*)Hh);
+        };
+        RowsArgs r;
+        r.W1 = L.w1_f;
+        r.b1 = L.b1;
+        r.W2 = L.w2_f;
+        r.b2 = L.b2;
+        r.X = X;
         m.tune.us[TO_ROWS][b][0] = time_us(s, ev, [&] {
+            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr, bp);
+            norm(L.n2g, L.n2b);
             gemm(ff, d, Hh, L.w1_t, L.b1, F, EPI_GELU, nullptr, nullptr, bf);
             gemm(d, ff, F, L.w2_t, L.b2, nullptr, EPI_RESID, nullptr, nullptr, bf);
+            norm(N1.n1g, N1.n1b);
+            gemm(3 * d, d, Hh, N1.wqkv_t, N1.bqkv, QKV, EPI_STORE, nullptr, nullptr, bq);
         });
         m.tune.us[TO_ROWS][b][1] = time_us(s, ev, [&] {
-            RowsArgs a;
+            gemm(d, d, O, L.wp_t, L.bp, nullptr, EPI_RESID, nullptr, nullptr, bp);
+            norm(L.n2g, L.n2b);
+            RowsArgs a = r;
             a.Hin = Hh;
-            a.W1 = L.w1_f;
-            a.b1 = L.b1;
-            a.W2 = L.w2_f;
-            a.b2 = L.b2;
-            a.X = X;
             mlp_fused(s, (int)M, d, ff, split, a);
+            norm(N1.n1g, N1.n1b);
+            gemm(3 * d, d, Hh, N1.wqkv_t, N1.bqkv, QKV, EPI_STORE, nullptr, nullptr, bq);
+        });
+        m.tune.us[TO_ROWS][b][2] = time_us(s, ev, [&] {
+            RowsArgs a = r;
+            a.O = O;
+            a.Wp = L.wp_f;
+            a.bp = L.bp;
+            a.n2g = L.n2g;
+            a.n2b = L.n2b;
+            a.n1g = N1.n1g;
+            a.n1b = N1.n1b;
+            a.Wqkv = N1.wqkv_f;
+            a.bqkv = N1.bqkv;
+            a.QKV = QKV;
+            rows256(s, (int)M, d, ff, split, a);
         });
         settle(TO_ROWS);
     }

@@ -280,29 +280,35 @@ def test_wide_six_layers(precision):
 
 @pytest.mark.parametrize("precision", ["bf16x2", "bf16"])
 def test_wide_fused_mlp_bit_identical(precision):
-    """d = 256: the MLP in one launch (its ff-wide intermediate on chip)
-    gives the same raw output and iterates, bit for bit, as the two tile
-    GEMMs, on a grid whose token count is no multiple of the 64-token panel
-    (3 x 24 x 20); the fused launch runs once per block; bf16x2's raw output
+    """d = 256: the MLP in one launch (its ff-wide intermediate on chip), and
+    everything between two attentions in one launch (proj + residual, LN2,
+    MLP, the next block's LN1 and qkv), give the same raw output and
+    iterates, bit for bit, as the tile GEMMs and LayerNorm launches, on a
+    grid whose token count is no multiple of the 64-token panel (3 x 24 x
+    20); the fused MLP runs once per block, the whole-block launch once per
+    block (with the next qkv but for the last block); bf16x2's raw output
     sits within its bound of the f64 oracle on its bf16 weights."""
-    st, cj = ck.random_cell_st(dim=256, layers=2, heads=8, ff=1024, seed=1, radius=2)
-    fused = _model(st, cj, precision, "rows_fused=1")
+    st, cj = ck.random_cell_st(dim=256, layers=3, heads=8, ff=1024, seed=1, radius=2)
     apart = _model(st, cj, precision, "rows_fused=0")
     x0, k = S.request("reconstruction", 3, 24, 20)
     print(f"case wide fused MLP {precision}", flush=True)
     ya, ca = gpu_forward(apart, x0, k, timing=True)
-    yf, cf = gpu_forward(fused, x0, k, timing=True)
-    assert cf[KC_ROWS_TAIL] == 2 and ca[KC_ROWS_TAIL] == 0, (cf, ca)
-    assert np.isfinite(yf).all()
-    assert np.array_equal(ya.view(np.uint32), yf.view(np.uint32))
+    assert ca[KC_ROWS_TAIL] == 0 and ca[KC_ROWS] == 0, ca
     xa, ia, ra, _ = apart.iterate(x0, k, META, max_iter=2, tol=0.0)
-    xf, i_f, rf, _ = fused.iterate(x0, k, META, max_iter=2, tol=0.0)
-    assert (ia, ra) == (i_f, rf)
-    assert np.array_equal(xa.view(np.uint32), xf.view(np.uint32))
+    for pick, counts in ((1, (0, 3)), (2, (2, 1))):
+        print(f"case wide fused {precision} rows_fused={pick}", flush=True)
+        fused = _model(st, cj, precision, f"rows_fused={pick}")
+        yf, cf = gpu_forward(fused, x0, k, timing=True)
+        assert (cf[KC_ROWS], cf[KC_ROWS_TAIL]) == counts, (pick, cf)
+        assert np.isfinite(yf).all()
+        assert np.array_equal(ya.view(np.uint32), yf.view(np.uint32)), pick
+        xf, i_f, rf, _ = fused.iterate(x0, k, META, max_iter=2, tol=0.0)
+        assert (ia, ra) == (i_f, rf)
+        assert np.array_equal(xa.view(np.uint32), xf.view(np.uint32)), pick
     if precision == "bf16x2":
         cpu = O.FormC.load(st, cj)
         d = S.rel_var(yf, S.bf16_weights(cpu).forward(x0, k, META, f64=True))
-        print(f"wide fused MLP bf16x2 raw output rel_var vs bf16-weight oracle {d}")
+        print(f"wide fused bf16x2 raw output rel_var vs bf16-weight oracle {d}")
         assert d.max() < S.Y_TOL["bf16x2"]
 
 

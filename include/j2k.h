@@ -52,8 +52,9 @@ extern "C" {
  * bf16 MFMA as a hi + lo bf16 pair (two MFMAs per tile), f32 accumulate */
 #define J2K_PREC_BF16X2 2
 /* FP8: the four block products on E4M3 codes (weights per-tensor scaled at
- * load; activations quantized in the GEMM with static per-tensor scales that
- * bound them), f32 accumulate; everything else f32. */
+ * load; activations quantized in the GEMM, each token with the scale of its
+ * own maximum, measured just before the product), f32 accumulate; everything
+ * else f32. */
 #define J2K_PREC_FP8 3
 
 #define J2K_FORM_PATCH2D 0 /* the reference forward (crates/engine/src/cpu.rs:249-352) */
@@ -101,15 +102,17 @@ int j2k_model_config(const void* model, j2k_config* out);
 
 /* Autotuner, no reference counterpart (the reference has one fixed launch
  * shape per kernel): times the candidate launch shapes of the BF16 / BF16X2
- * trunk (GEMM block height, attention K/V staging: joint, split or per time
- * plane, the d = 128 token-local work apart or fused into one launch of
- * four waves (in a row or 2 x 2, or with 64-column MLP chunks) or of eight
- * waves) on synthetic packs of 2^b tokens for
+ * trunk (GEMM block height, attention K/V staging: joint, split, per time
+ * plane or two queries per lane; the d = 128 token-local work apart or fused
+ * into one launch of four waves (in a row or 2 x 2, or with 64-column MLP
+ * chunks) or of eight waves; the d = 256 token-local work as tile GEMMs and
+ * LayerNorm launches, with the MLP in one launch, or all in one launch) on
+ * synthetic packs of 2^b tokens for
  * every b from 12 up to that of max_tokens (at most 2^20) not yet tuned, and
  * keeps the fastest per bucket of packed tokens. A choice never changes a
  * result. j2k_iterate* and j2k_server_create run it on first use unless the
  * environment sets J2K_AUTOTUNE=0; J2K_TUNE_FORCE =
- * "gemm_bm=64|128,attn_kv_mode=0|1|2,rows_fused=0..4" instead fixes the choices
+ * "gemm_bm=64|128,attn_kv_mode=0..4,rows_fused=0..4" instead fixes the choices
  * of models created after it is set. max_tokens = 0 only reports. `report`
  * (may be NULL) receives the table as JSON text, cut to report_len bytes.
  * F32 models keep their fixed shapes. Synchronises the model's stream; not

@@ -45,8 +45,9 @@ def device_count() -> int:
 class Model:
     """A checkpoint on one GPU. `precision` is "f32" (plain form), "bf16x2"
     (served form: bf16 weights on MFMA, f32 activations as bf16 pairs), "bf16"
-    (bf16 activations too) or "fp8" (the block products on E4M3 codes with
-    per-tensor scales); every form accumulates in f32."""
+    (bf16 activations too) or "fp8" (the block products on E4M3 codes,
+    weights with per-tensor scales, activations with per-token scales);
+    every form accumulates in f32."""
 
     def __init__(self, handle: int, precision: str):
         self._h = ctypes.c_void_p(handle)

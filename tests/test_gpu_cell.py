@@ -357,7 +357,8 @@ def test_device_iterations_replay_per_caller_buffers(model20_cell):
 
 @pytest.mark.parametrize("shape", ["model20", "wide"])
 def test_fp8_form_error_vs_f64_oracle(model20_cell, shape):
-    """The FP8 form (E4M3 block products, per-tensor scales, f32 accumulate)
+    """The FP8 form (E4M3 block products, per-tensor weight and per-token
+    activation scales, f32 accumulate)
     on a small grid: relative L2 against the f64 oracle after 2 iterations,
     reported per shape (no target is stated for FP8), and the energy bound."""
     st, cj = model20_cell if shape == "model20" else ck.random_cell_st(dim=256, layers=2, heads=8, ff=1024, seed=1,

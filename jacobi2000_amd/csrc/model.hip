@@ -359,8 +359,11 @@ template <class Act>
 void attend(Model& m, hipStream_t s, int M, const Act* QKV, Act* O, const AttnPlan& ap) {
     const int d = m.cfg.dim, H = m.cfg.heads;
     const AttnGeo geo = m.geo();
+    // untuned (the f32 and FP8 forms, or tuning off): K then V, the fastest
+    // mode at the 64 x 64 x 16 and 128 x 128 x 16 workloads
     int mode = m.tune.choice(TO_ATTN, M);
-    if (mode < 0 || !attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode)) mode = KV_JOINT;
+    if (mode < 0) mode = KV_SPLIT;
+    if (!attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode)) mode = KV_JOINT;
     const bool tiled = ap.ctiles && geo.kind == ATTN_CELL_ST &&
                        attn_cell_ok(d / H, geo.radius, geo.time_radius, sizeof(Act), mode);
     if (tiled) {

@@ -186,6 +186,19 @@ struct Server {
     size_t cap_jobs = 0;
     int* d_ridx = nullptr;
     size_t cap_ridx = 0;
+    // Retirement of a turn whose requests all end (no rollout window
+    // follows): its tables go through h_rtab, its download runs on s_copy
+    // behind the unpack, and the host copies into the requests' grids wait
+    // for the next turn's iteration to be launched (finish_retire), so the
+    // download and the copies overlap the GPU's next iteration. Until then
+    // the requests sit in `deferred`, counted as active for drain().
+    unsigned char* h_rtab = nullptr;
+    size_t cap_hrtab = 0;
+    hipStream_t s_copy = nullptr;
+    hipEvent_t ev_unpacked = nullptr, ev_out = nullptr;
+    std::vector<std::unique_ptr<Request>> deferred;
+    std::vector<size_t> deferred_off;
+    size_t deferred_bytes = 0;
     // J2K_SERVER_PHASES=1 at creation: seconds the worker spent per phase,
     // printed to stderr when the server is destroyed
     bool phases_on = false;
@@ -212,6 +225,12 @@ struct Server {
         (void)hipGetDevice(&prev);
         (void)hipSetDevice(m.device);
         (void)hipStreamSynchronize(m.stream);
+        if (s_copy) {
+            (void)hipStreamSynchronize(s_copy);
+            (void)hipStreamDestroy(s_copy);
+        }
+        if (ev_unpacked) (void)hipEventDestroy(ev_unpacked);
+        if (ev_out) (void)hipEventDestroy(ev_out);
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
         if (h_resid) (void)hipHostFree(h_resid);
@@ -219,6 +238,7 @@ struct Server {
             if (p) (void)hipHostFree(p);
         if (h_out) (void)hipHostFree(h_out);
         if (h_tab) (void)hipHostFree(h_tab);
+        if (h_rtab) (void)hipHostFree(h_rtab);
         if (prev >= 0) (void)hipSetDevice(prev);
     }
 
@@ -581,9 +601,11 @@ struct Server {
     // Final energy violation and grid of every request active[idx[j]] (reads
     // xc): one slice table for all of them (with each slice's place in the
     // packing's table, for its reference energy), one violation launch over
-    // it, one unpack launch over a job table, the copies out, one
-    // synchronisation.
-    void retire(const std::vector<size_t>& idx) {
+    // it, one unpack launch over a job table, the copies out. With `defer`
+    // the download is queued on s_copy and the rest is left to
+    // finish_retire(); else one synchronisation and the host copies now.
+    // Any earlier deferred retirement must be finished (it owns h_out).
+    void retire(const std::vector<size_t>& idx, bool defer) {
         hipStream_t s = m.stream;
         const size_t k = idx.size();
         std::vector<Slice> sl;
@@ -609,39 +631,78 @@ struct Server {
         reserve(d_ridx, cap_ridx, sidx.size());
         reserve(d_jobs, cap_jobs, k);
         reserve(d_viol, cap_viol, k);
+        // h_rtab, not h_tab: a deferred retirement's uploads may still be
+        // queued when the next repack writes h_tab
         const size_t o1 = aligned(sl.size() * sizeof(Slice)), o2 = o1 + aligned(sidx.size() * sizeof(int));
-        reserve_host(h_tab, cap_htab, o2 + k * sizeof(CellJob));
-        std::memcpy(h_tab, sl.data(), sl.size() * sizeof(Slice));
-        std::memcpy(h_tab + o1, sidx.data(), sidx.size() * sizeof(int));
-        std::memcpy(h_tab + o2, jobs.data(), k * sizeof(CellJob));
-        HIPCHK(hipMemcpyAsync(d_rsl, h_tab, sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_ridx, h_tab + o1, sidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(d_jobs, h_tab + o2, k * sizeof(CellJob), hipMemcpyHostToDevice, s));
+        reserve_host(h_rtab, cap_hrtab, o2 + k * sizeof(CellJob));
+        std::memcpy(h_rtab, sl.data(), sl.size() * sizeof(Slice));
+        std::memcpy(h_rtab + o1, sidx.data(), sidx.size() * sizeof(int));
+        std::memcpy(h_rtab + o2, jobs.data(), k * sizeof(CellJob));
+        HIPCHK(hipMemcpyAsync(d_rsl, h_rtab, sl.size() * sizeof(Slice), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_ridx, h_rtab + o1, sidx.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(d_jobs, h_rtab + o2, k * sizeof(CellJob), hipMemcpyHostToDevice, s));
         fill_u32(s, reinterpret_cast<unsigned*>(d_viol), (int)(2 * k), 0u);
         cell_violation(s, (int)sl.size(), d_rsl, xc, pack.eref_t, d_viol, m.cfg.g, m.cfg.norm_mean, m.cfg.norm_std,
                        m.cfg.norm_depth, m.cfg.energy_r_max, d_ridx);
         cell_unpack_jobs(s, (int)k, d_jobs, max_tok, xc, d_out);
         reserve_host(h_out, cap_hout, out_bytes + k * sizeof(double));
-        HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::vector<Copy> copies;
-        for (size_t j = 0; j < k; j++) {
-            Request& r = *active[idx[j]];
-            copies.push_back(Copy{r.grid.data(), h_out + out_off[j], 12 * (size_t)r.dims.tokens()});
-        }
-        copy_all(copies);
-        std::vector<double> v(k);
-        std::memcpy(v.data(), h_out + out_bytes, k * sizeof(double));
-        const double t = now_s();
         for (size_t j = 0; j < k; j++) {
             Request& r = *active[idx[j]];
             r.res.iters = r.iters;
             r.res.resid = r.resid;
             r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
-            r.res.violation = v[j];
-            r.res.done_s = t;
         }
+        if (defer) {
+            HIPCHK(hipEventRecord(ev_unpacked, s));
+            HIPCHK(hipStreamWaitEvent(s_copy, ev_unpacked, 0));
+            HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s_copy));
+            HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s_copy));
+            HIPCHK(hipEventRecord(ev_out, s_copy));
+            deferred_off = out_off;
+            deferred_bytes = out_bytes;
+            return;  // the caller moves the requests into `deferred`
+        }
+        HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::vector<Request*> rs(k);
+        for (size_t j = 0; j < k; j++) rs[j] = active[idx[j]].get();
+        land(rs, out_off, out_bytes);
+    }
+
+    // The downloaded grids and violations of retired requests rs, placed as
+    // retire() laid them out in h_out, into the requests.
+    void land(const std::vector<Request*>& rs, const std::vector<size_t>& out_off, size_t out_bytes) {
+        const size_t k = rs.size();
+        std::vector<Copy> copies;
+        for (size_t j = 0; j < k; j++)
+            copies.push_back(Copy{rs[j]->grid.data(), h_out + out_off[j], 12 * (size_t)rs[j]->dims.tokens()});
+        copy_all(copies);
+        std::vector<double> v(k);
+        std::memcpy(v.data(), h_out + out_bytes, k * sizeof(double));
+        const double t = now_s();
+        for (size_t j = 0; j < k; j++) {
+            rs[j]->res.violation = v[j];
+            rs[j]->res.done_s = t;
+        }
+    }
+
+    // The host half of a deferred retirement: wait for its download, land
+    // the results and publish the requests.
+    void finish_retire() {
+        if (deferred.empty()) return;
+        HIPCHK(hipEventSynchronize(ev_out));
+        std::vector<Request*> rs;
+        for (auto& r : deferred) rs.push_back(r.get());
+        land(rs, deferred_off, deferred_bytes);
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& r : deferred) {
+            const uint64_t id = r->id;
+            done[id] = std::move(r);
+        }
+        deferred.clear();
+        n_active = active.size();
+        cv_done.notify_all();
     }
 
     // Records the window just finished (r.grid holds it) into the trajectory
@@ -686,8 +747,19 @@ struct Server {
 
     void loop() {
         DeviceGuard dg(m.device);
+        HIPCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_unpacked, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
         for (;;) {
             bool changed = false;
+            {
+                bool work;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    work = stop || !pending.empty() || !active.empty();
+                }
+                if (!work) finish_retire();  // nothing to overlap it with
+            }
             {
                 std::unique_lock<std::mutex> lk(mu);
                 cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
@@ -703,7 +775,7 @@ struct Server {
                     pending.pop_front();
                     changed = true;
                 }
-                n_active = active.size();
+                n_active = active.size() + deferred.size();
                 peak_active = std::max(peak_active, (int)active.size());
                 peak_bytes = std::max(peak_bytes, used);
             }
@@ -713,7 +785,8 @@ struct Server {
             const double t_b = now_s();
             run_iteration();
             HIPCHK(hipMemcpyAsync(h_resid, pack.resid, 4 * active.size(), hipMemcpyDeviceToHost, m.stream));
-            prestage();  // beside the iteration
+            prestage();       // beside the iteration
+            finish_retire();  // the last turn's results, beside the iteration too
             HIPCHK(hipStreamSynchronize(m.stream));
             const double t_c = now_s();
             ph_turns += 1;
@@ -733,7 +806,26 @@ struct Server {
                     retiring.push_back(i);
                 }
             }
-            if (!retiring.empty()) retire(retiring);
+            // requests that all end can be landed during the next iteration;
+            // a rollout's next window needs its result now
+            bool defer = !retiring.empty();
+            for (size_t i : retiring) defer = defer && active[i]->ticks == 0;
+            if (!retiring.empty()) retire(retiring, defer);
+            if (defer) {
+                std::lock_guard<std::mutex> lk(mu);
+                std::vector<std::unique_ptr<Request>> keep;
+                std::vector<bool> leaving(active.size(), false);
+                for (size_t i : retiring) leaving[i] = true;
+                for (size_t i = 0; i < active.size(); i++) {
+                    if (leaving[i]) deferred.push_back(std::move(active[i]));
+                    else keep.push_back(std::move(active[i]));
+                }
+                active = std::move(keep);
+                n_active = active.size() + deferred.size();
+                packed = false;  // survivors move to a compact layout next turn
+                ph_finish += now_s() - t_c;
+                continue;
+            }
             for (size_t i : retiring) {
                 if (active[i]->ticks > 0 && next_window(*active[i])) {
                     rewindowed = true;  // stays active with its next window
@@ -751,7 +843,7 @@ struct Server {
                 else keep.push_back(std::move(active[i]));
             }
             active = std::move(keep);
-            n_active = active.size();
+            n_active = active.size() + deferred.size();
             packed = false;  // survivors move to a compact layout next turn
             cv_done.notify_all();
         }

@@ -60,6 +60,23 @@ def test_noalloc_forward_matches_forward(precision):
         assert np.array_equal(again.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("pick", [0, 1, 2])
+def test_noalloc_wide_token_local_launches(pick, monkeypatch):
+    """d = 256 with each way of running the token-local work forced (tile
+    GEMMs, the fused MLP, the whole-block launch): the allocation-free
+    forward gives the model's own bits, on two calls in a row, the second
+    on a smaller grid."""
+    monkeypatch.setenv("J2K_TUNE_FORCE", f"rows_fused={pick}")
+    st, cj = ck.random_cell_st(dim=256, layers=2, heads=8, ff=1024, seed=1, std=0.1, radius=2)
+    model = j.Model.from_bytes(st, cj, precision="bf16x2")
+    for i, (x0, k) in enumerate([S.request("reconstruction", 3, 24, 20), S.request("forward", 2, 16, 16)]):
+        print(f"case wide rows_fused={pick} call {i + 1}: grid {x0.shape}", flush=True)
+        want = _forward(model, x0, k, noalloc=False)
+        got = _forward(model, x0, k, noalloc=True)
+        assert np.isfinite(got).all()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), i
+
+
 def test_noalloc_refuses_a_grid_beyond_the_region():
     st, cj = S.teeth()
     model = j.Model.from_bytes(st, cj, precision="f32")

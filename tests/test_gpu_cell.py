@@ -529,22 +529,30 @@ def _windows_alone(model, state, window, ticks):
 def test_server_rollout_chains_windows(small, precision):
     """Chained rollouts (window 4, 9 ticks: windows of 4, 4 and 3 slices),
     three at once with plain requests in the mix, each equal to its windows
-    run one by one, every window within the energy bound."""
+    run one by one, every window within the energy bound; the plain
+    requests (retired in turns with and without a rollout window) equal to
+    their runs alone."""
     model = j.Model.from_bytes(*small, precision=precision)
     ny, nx, W, ticks = 16, 12, 4, 9
     states = [ck.drops_grid(1, ny, nx, 2, 40 + i)[:, 0].copy() for i in range(3)]
     alone = [_windows_alone(model, st, W, ticks) for st in states]
+    plain = _requests()[:3]
+    plain_alone = [model.iterate(j.fill_blank(g, k), k, META) for g, k in plain]
     h = _server(model, 1 << 34, 16)
     try:
-        ids = []
+        ids, plain_ids = [], []
         for i, st in enumerate(states):
             rid = ctypes.c_uint64()
             _lib.call("j2k_server_submit_rollout", h, st.ctypes.data_as(ctypes.c_void_p),
                       META.ctypes.data_as(ctypes.c_void_p), ny, nx, W, ticks, ctypes.byref(rid))
             ids.append(rid.value)
-            g, k = _requests()[i]
-            _submit(h, g, k)
+            g, k = plain[i]
+            plain_ids.append(_submit(h, g, k))
         _lib.call("j2k_server_drain", h)
+        for (g, k), rid, (xa, ita, _, _) in zip(plain, plain_ids, plain_alone):
+            res, out = _result(h, rid, g.shape)
+            print(f"plain {rid}: iters mix={res.iters} alone={ita} max_abs={np.abs(out - xa).max():.2e}")
+            assert res.iters == ita and np.abs(out - xa).max() <= 1e-6
         for rid, (traj, iters, viol) in zip(ids, alone):
             res, out = _result(h, rid, (3, ticks, ny, nx))
             n = ctypes.c_int()

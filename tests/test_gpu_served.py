@@ -448,6 +448,32 @@ def test_server_model20_shape_same_alone_and_in_the_mix(ckpts, name, precision):
         _lib.call("j2k_server_destroy", h)
 
 
+@pytest.mark.parametrize("pick", [1, 2])
+def test_server_wide_same_alone_and_in_the_mix(pick):
+    """The wide model (d = 256, 75-key stencil) served with the MLP in one
+    launch or the whole token-local work in one launch (forced): requests
+    of several sizes packed into ragged launches each equal the request run
+    alone (same iterations, 1e-6)."""
+    st, cj = ck.random_cell_st(dim=256, layers=2, heads=8, ff=1024, seed=1, std=0.1, radius=2)
+    model = _model(st, cj, "bf16x2", f"rows_fused={pick}")
+    reqs = []
+    for i, (T, ny, nx, kind) in enumerate([(4, 20, 20, "forward"), (3, 24, 16, "reconstruction"),
+                                            (8, 12, 12, "rewind"), (2, 30, 26, "forward")]):
+        reqs.append((ck.moving_drops_grid(T, ny, nx, 3, 300 + i), ck.mask_of(kind, T, ny, nx, seed=i)))
+    alone = [model.iterate(j.fill_blank(g, k), k, META) for g, k in reqs]
+    h = _server(model)
+    try:
+        ids = [_submit(h, g, k) for g, k in reqs]
+        _lib.call("j2k_server_drain", h)
+        for (g, k), rid, (xa, ita, ra, va) in zip(reqs, ids, alone):
+            res, out = _result(h, rid, g.shape)
+            d = np.abs(out - xa).max()
+            print(f"wide rows_fused={pick} request {rid} {g.shape}: iters mix={res.iters} alone={ita} max_abs={d:.2e}")
+            assert res.iters == ita and d <= 1e-6 and res.violation <= 1e-6
+    finally:
+        _lib.call("j2k_server_destroy", h)
+
+
 def _windows_alone(model, state, window, ticks):
     _, ny, nx = state.shape
     traj = np.zeros((3, ticks, ny, nx), np.float32)

@@ -527,7 +527,7 @@ struct Server {
     // admission order, up to max_active and the buffer's size) into the
     // staging buffer the next repack uploads from. Only the worker admits
     // requests, so the ones it takes here stay valid after the lock.
-    void prestage() {
+    bool prestage() {
         std::vector<Request*> take;
         {
             std::lock_guard<std::mutex> lk(mu);
@@ -555,6 +555,7 @@ struct Server {
         }
         copy_all(jobs);
         staged.insert(staged.end(), take.begin(), take.end());
+        return !take.empty();
     }
 
     // One iteration xc -> xn, then the names swap.
@@ -787,6 +788,15 @@ struct Server {
             HIPCHK(hipMemcpyAsync(h_resid, pack.resid, 4 * active.size(), hipMemcpyDeviceToHost, m.stream));
             prestage();       // beside the iteration
             finish_retire();  // the last turn's results, beside the iteration too
+            // requests that arrive while the iteration runs are staged as they come
+            for (;;) {
+                const hipError_t q = hipStreamQuery(m.stream);
+                if (q != hipErrorNotReady) {
+                    HIPCHK(q);
+                    break;
+                }
+                if (!prestage()) std::this_thread::yield();
+            }
             HIPCHK(hipStreamSynchronize(m.stream));
             const double t_c = now_s();
             ph_turns += 1;

@@ -196,6 +196,12 @@ struct Server {
     size_t cap_hrtab = 0;
     hipStream_t s_copy = nullptr;
     hipEvent_t ev_unpacked = nullptr, ev_out = nullptr;
+    // Pre-staged requests are also uploaded on s_copy while the iteration
+    // runs: d_in [0, uploaded_end) holds h_stage[up] [0, uploaded_end), once
+    // ev_uploaded has passed. d_in is free for it after the turn's pack
+    // launches (ev_packed).
+    hipEvent_t ev_packed = nullptr, ev_uploaded = nullptr;
+    size_t uploaded_end = 0;
     std::vector<std::unique_ptr<Request>> deferred;
     std::vector<size_t> deferred_off;
     size_t deferred_bytes = 0;
@@ -229,8 +235,8 @@ struct Server {
             (void)hipStreamSynchronize(s_copy);
             (void)hipStreamDestroy(s_copy);
         }
-        if (ev_unpacked) (void)hipEventDestroy(ev_unpacked);
-        if (ev_out) (void)hipEventDestroy(ev_out);
+        for (hipEvent_t e : {ev_unpacked, ev_out, ev_packed, ev_uploaded})
+            if (e) (void)hipEventDestroy(e);
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
         if (h_resid) (void)hipHostFree(h_resid);
@@ -348,6 +354,7 @@ struct Server {
                 for (Request* r : staged) r->stage_buf = -1;
                 staged.clear();
                 staged_end = 0;
+                uploaded_end = 0;
                 end = 0;
                 for (auto& r : active)
                     if (r->base < 0) end += aligned(13 * (size_t)r->dims.tokens());
@@ -366,8 +373,14 @@ struct Server {
                     o += aligned(13 * T);
                 }
             copy_all(jobs);
+            // the uploads already made on s_copy end before d_in may be
+            // replaced, and before the pack launches read it
+            HIPCHK(hipEventSynchronize(ev_uploaded));
+            if (end > cap_in) uploaded_end = 0;
             reserve(d_in, cap_in, end);
-            if (end) HIPCHK(hipMemcpyAsync(d_in, hb, end, hipMemcpyHostToDevice, s));
+            if (end > uploaded_end)
+                HIPCHK(hipMemcpyAsync(d_in + uploaded_end, hb + uploaded_end, end - uploaded_end, hipMemcpyHostToDevice, s));
+            uploaded_end = 0;
             // pre-staged requests left pending are staged again for a later turn
             for (Request* r : staged) r->stage_buf = -1;
             staged.clear();
@@ -420,6 +433,7 @@ struct Server {
         if (!jobs.empty())
             HIPCHK(hipMemcpyAsync(d_jobs, h_tab + off[5], jobs.size() * sizeof(CellJob), hipMemcpyHostToDevice, s));
         cell_pack_jobs(s, (int)n_pack, d_jobs, max_pack, d_in, alt0, altc);
+        HIPCHK(hipEventRecord(ev_packed, s));  // d_in is free for the next turn's uploads after this
         cell_move_jobs(s, (int)(jobs.size() - n_pack), d_jobs + n_pack, max_move, x0, xc, alt0, altc);
         int slices = 0;  // the slice table lists each request's slices in order
         for (size_t i = 0; i < active.size(); i++) {
@@ -555,6 +569,14 @@ struct Server {
         }
         copy_all(jobs);
         staged.insert(staged.end(), take.begin(), take.end());
+        // and on to the device while the iteration runs, when d_in holds them
+        if (!take.empty() && staged_end <= cap_in && uploaded_end < staged_end) {
+            HIPCHK(hipStreamWaitEvent(s_copy, ev_packed, 0));
+            HIPCHK(hipMemcpyAsync(d_in + uploaded_end, h_stage[up] + uploaded_end, staged_end - uploaded_end,
+                                  hipMemcpyHostToDevice, s_copy));
+            HIPCHK(hipEventRecord(ev_uploaded, s_copy));
+            uploaded_end = staged_end;
+        }
         return !take.empty();
     }
 
@@ -751,6 +773,8 @@ struct Server {
         HIPCHK(hipStreamCreateWithFlags(&s_copy, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_unpacked, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_packed, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_uploaded, hipEventDisableTiming));
         for (;;) {
             bool changed = false;
             {

@@ -24,9 +24,10 @@
 // Host traffic per turn: the new requests' grids and masks go up into their
 // own regions of one staging buffer and the retiring requests' grids come
 // down from another, with the launches between them back to back and one
-// synchronisation per phase, not per request. The next turn's pending
-// requests are copied into the other of two upload buffers while an
-// iteration runs.
+// synchronisation per phase, not per request. While an iteration runs, the
+// pending requests are copied into the other of two upload buffers and
+// uploaded on a copy stream, and the previous turn's retired grids, already
+// downloaded on that stream, are copied into their requests.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -186,12 +187,13 @@ struct Server {
     size_t cap_jobs = 0;
     int* d_ridx = nullptr;
     size_t cap_ridx = 0;
-    // Retirement of a turn whose requests all end (no rollout window
-    // follows): its tables go through h_rtab, its download runs on s_copy
-    // behind the unpack, and the host copies into the requests' grids wait
-    // for the next turn's iteration to be launched (finish_retire), so the
-    // download and the copies overlap the GPU's next iteration. Until then
-    // the requests sit in `deferred`, counted as active for drain().
+    // Retirement: its tables go through h_rtab, its download runs on s_copy
+    // behind the unpack, and the host copies (into a request's grid, or a
+    // rollout window's slices into its trajectory) wait for the next turn's
+    // iteration to be launched (finish_retire), so the download and the
+    // copies overlap the GPU's next iteration. Until then finished requests
+    // sit in `deferred`, counted as active for drain(); a rollout with
+    // windows to go stays active (its next window is made on the device).
     unsigned char* h_rtab = nullptr;
     size_t cap_hrtab = 0;
     hipStream_t s_copy = nullptr;
@@ -202,8 +204,15 @@ struct Server {
     // launches (ev_packed).
     hipEvent_t ev_packed = nullptr, ev_uploaded = nullptr;
     size_t uploaded_end = 0;
+    struct Landing {
+        Request* r;
+        size_t off;          // its grid in h_out
+        int T, t0;           // its slices; t0 >= 0: a rollout window's first tick
+        int iters;           // the iterations it took
+        bool converged, last;  // last: the rollout's final window
+    };
+    std::vector<Landing> landings;  // in retirement order (the violation words' order)
     std::vector<std::unique_ptr<Request>> deferred;
-    std::vector<size_t> deferred_off;
     size_t deferred_bytes = 0;
     // J2K_SERVER_PHASES=1 at creation: seconds the worker spent per phase,
     // printed to stderr when the server is destroyed
@@ -624,11 +633,10 @@ struct Server {
     // Final energy violation and grid of every request active[idx[j]] (reads
     // xc): one slice table for all of them (with each slice's place in the
     // packing's table, for its reference energy), one violation launch over
-    // it, one unpack launch over a job table, the copies out. With `defer`
-    // the download is queued on s_copy and the rest is left to
-    // finish_retire(); else one synchronisation and the host copies now.
-    // Any earlier deferred retirement must be finished (it owns h_out).
-    void retire(const std::vector<size_t>& idx, bool defer) {
+    // it, one unpack launch over a job table, and the download queued on
+    // s_copy; finish_retire() lands the results. The previous retirement
+    // must have been landed (its results are in h_out).
+    void retire(const std::vector<size_t>& idx) {
         hipStream_t s = m.stream;
         const size_t k = idx.size();
         std::vector<Slice> sl;
@@ -654,8 +662,8 @@ struct Server {
         reserve(d_ridx, cap_ridx, sidx.size());
         reserve(d_jobs, cap_jobs, k);
         reserve(d_viol, cap_viol, k);
-        // h_rtab, not h_tab: a deferred retirement's uploads may still be
-        // queued when the next repack writes h_tab
+        // h_rtab, not h_tab: these uploads may still be queued when the
+        // next repack writes h_tab
         const size_t o1 = aligned(sl.size() * sizeof(Slice)), o2 = o1 + aligned(sidx.size() * sizeof(int));
         reserve_host(h_rtab, cap_hrtab, o2 + k * sizeof(CellJob));
         std::memcpy(h_rtab, sl.data(), sl.size() * sizeof(Slice));
@@ -669,55 +677,75 @@ struct Server {
                        m.cfg.norm_depth, m.cfg.energy_r_max, d_ridx);
         cell_unpack_jobs(s, (int)k, d_jobs, max_tok, xc, d_out);
         reserve_host(h_out, cap_hout, out_bytes + k * sizeof(double));
+        HIPCHK(hipEventRecord(ev_unpacked, s));
+        HIPCHK(hipStreamWaitEvent(s_copy, ev_unpacked, 0));
+        HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s_copy));
+        HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s_copy));
+        HIPCHK(hipEventRecord(ev_out, s_copy));
+        deferred_bytes = out_bytes;
         for (size_t j = 0; j < k; j++) {
             Request& r = *active[idx[j]];
             r.res.iters = r.iters;
             r.res.resid = r.resid;
             r.res.converged = r.resid < m.cfg.tol ? 1 : 0;
+            landings.push_back(Landing{&r, out_off[j], r.dims.T, -1, r.iters, r.resid < m.cfg.tol, false});
         }
-        if (defer) {
-            HIPCHK(hipEventRecord(ev_unpacked, s));
-            HIPCHK(hipStreamWaitEvent(s_copy, ev_unpacked, 0));
-            HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s_copy));
-            HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s_copy));
-            HIPCHK(hipEventRecord(ev_out, s_copy));
-            deferred_off = out_off;
-            deferred_bytes = out_bytes;
-            return;  // the caller moves the requests into `deferred`
-        }
-        HIPCHK(hipMemcpyAsync(h_out, d_out, out_bytes, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipMemcpyAsync(h_out + out_bytes, d_viol, k * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        std::vector<Request*> rs(k);
-        for (size_t j = 0; j < k; j++) rs[j] = active[idx[j]].get();
-        land(rs, out_off, out_bytes);
     }
 
-    // The downloaded grids and violations of retired requests rs, placed as
-    // retire() laid them out in h_out, into the requests.
-    void land(const std::vector<Request*>& rs, const std::vector<size_t>& out_off, size_t out_bytes) {
-        const size_t k = rs.size();
+    // The host half of a retirement: wait for its download, copy each
+    // result into place (a request's grid; a rollout window's slices after
+    // its first into the trajectory, with its time, violation and
+    // iterations; the trajectory becomes the result after the last window)
+    // and publish the requests that ended.
+    void finish_retire() {
+        if (landings.empty()) return;
+        HIPCHK(hipEventSynchronize(ev_out));
+        const size_t k = landings.size();
         std::vector<Copy> copies;
-        for (size_t j = 0; j < k; j++)
-            copies.push_back(Copy{rs[j]->grid.data(), h_out + out_off[j], 12 * (size_t)rs[j]->dims.tokens()});
+        for (const Landing& L : landings) {
+            Request& r = *L.r;
+            const size_t n = (size_t)r.dims.ny * r.dims.nx;
+            const unsigned char* src = h_out + L.off;
+            if (L.t0 < 0) {
+                copies.push_back(Copy{r.grid.data(), src, 12 * (size_t)L.T * n});
+                continue;
+            }
+            for (int c = 0; c < 3; c++)  // slice 0 of a window is the previous window's last
+                copies.push_back(Copy{&r.traj[((size_t)c * r.ticks + L.t0 + 1) * n],
+                                      src + ((size_t)c * L.T + 1) * n * sizeof(float), (size_t)(L.T - 1) * n * sizeof(float)});
+        }
         copy_all(copies);
         std::vector<double> v(k);
-        std::memcpy(v.data(), h_out + out_bytes, k * sizeof(double));
+        std::memcpy(v.data(), h_out + deferred_bytes, k * sizeof(double));
         const double t = now_s();
         for (size_t j = 0; j < k; j++) {
-            rs[j]->res.violation = v[j];
-            rs[j]->res.done_s = t;
+            const Landing& L = landings[j];
+            Request& r = *L.r;
+            if (L.t0 < 0) {
+                r.res.violation = v[j];
+                r.res.done_s = t;
+                continue;
+            }
+            r.win_done.push_back(t);
+            r.win_viol.push_back(v[j]);
+            r.win_iters.push_back(L.iters);
+            r.converged_all = r.converged_all && L.converged;
+            if (!L.last) continue;
+            // the result of a rollout is its trajectory; counters over all windows
+            r.grid.swap(r.traj);
+            r.dims.T = r.ticks;
+            int it = 0;
+            double vm = 0;
+            for (size_t w = 0; w < r.win_iters.size(); w++) {
+                it += r.win_iters[w];
+                vm = std::max(vm, r.win_viol[w]);
+            }
+            r.res.iters = it;
+            r.res.violation = vm;
+            r.res.converged = r.converged_all ? 1 : 0;
+            r.res.done_s = t;
         }
-    }
-
-    // The host half of a deferred retirement: wait for its download, land
-    // the results and publish the requests.
-    void finish_retire() {
-        if (deferred.empty()) return;
-        HIPCHK(hipEventSynchronize(ev_out));
-        std::vector<Request*> rs;
-        for (auto& r : deferred) rs.push_back(r.get());
-        land(rs, deferred_off, deferred_bytes);
+        landings.clear();
         std::lock_guard<std::mutex> lk(mu);
         for (auto& r : deferred) {
             const uint64_t id = r->id;
@@ -728,44 +756,18 @@ struct Server {
         cv_done.notify_all();
     }
 
-    // Records the window just finished (r.grid holds it) into the trajectory
-    // and sets up the next one; false when the rollout is complete.
-    bool next_window(Request& r) {
-        const int T = r.dims.T;
-        const size_t n = (size_t)r.dims.ny * r.dims.nx;
-        const int t0 = r.wi * (r.window - 1);
-        for (int c = 0; c < 3; c++)
-            for (int t = 1; t < T; t++)
-                std::memcpy(&r.traj[((size_t)c * r.ticks + t0 + t) * n], &r.grid[((size_t)c * T + t) * n],
-                            n * sizeof(float));
-        r.win_done.push_back(r.res.done_s);
-        r.win_viol.push_back(r.res.violation);
-        r.win_iters.push_back(r.iters);
-        r.converged_all = r.converged_all && r.res.converged;
+    // A rollout whose window has just retired moves to its next window
+    // (made on the device from this one's last slice at the next repack);
+    // false after its last window. Its results land in finish_retire().
+    bool advance_window(Request& r) {
         r.wi++;
-        const int total = window_count(r.ticks, r.window);
-        if (r.wi < total) {  // the next window starts on the device from this one's last slice
-            r.prev_T = T;
-            r.dims.T = window_slices(r.ticks, r.window, r.wi);
-            r.grid.resize(3 * (size_t)r.dims.T * n);
-            r.iters = 0;
-            r.resid = 0.f;
-            r.watch = StallWatch{};
-            return true;
-        }
-        // the result of a rollout is its trajectory; counters over all windows
-        r.grid.swap(r.traj);
-        r.dims.T = r.ticks;
-        int it = 0;
-        double v = 0;
-        for (size_t w = 0; w < r.win_iters.size(); w++) {
-            it += r.win_iters[w];
-            v = std::max(v, r.win_viol[w]);
-        }
-        r.res.iters = it;
-        r.res.violation = v;
-        r.res.converged = r.converged_all ? 1 : 0;
-        return false;
+        if (r.wi >= window_count(r.ticks, r.window)) return false;
+        r.prev_T = r.dims.T;
+        r.dims.T = window_slices(r.ticks, r.window, r.wi);
+        r.iters = 0;
+        r.resid = 0.f;
+        r.watch = StallWatch{};
+        return true;
     }
 
     void loop() {
@@ -826,9 +828,6 @@ struct Server {
             ph_turns += 1;
             ph_repack += t_b - t_a;
             ph_iter += t_c - t_b;
-            std::vector<bool> out(active.size(), false);
-            bool any = false;
-            bool rewindowed = false;
             std::vector<size_t> retiring;
             for (size_t i = 0; i < active.size(); i++) {
                 Request& r = *active[i];
@@ -840,46 +839,31 @@ struct Server {
                     retiring.push_back(i);
                 }
             }
-            // requests that all end can be landed during the next iteration;
-            // a rollout's next window needs its result now
-            bool defer = !retiring.empty();
-            for (size_t i : retiring) defer = defer && active[i]->ticks == 0;
-            if (!retiring.empty()) retire(retiring, defer);
-            if (defer) {
+            if (retiring.empty()) continue;
+            retire(retiring);
+            std::vector<bool> leaving(active.size(), false);
+            for (size_t j = 0; j < retiring.size(); j++) {
+                Request& r = *active[retiring[j]];
+                Landing& L = landings[landings.size() - retiring.size() + j];
+                if (r.ticks > 0) {
+                    L.t0 = r.wi * (r.window - 1);
+                    if (advance_window(r)) continue;  // stays active with its next window
+                    L.last = true;
+                }
+                leaving[retiring[j]] = true;
+            }
+            {
                 std::lock_guard<std::mutex> lk(mu);
                 std::vector<std::unique_ptr<Request>> keep;
-                std::vector<bool> leaving(active.size(), false);
-                for (size_t i : retiring) leaving[i] = true;
                 for (size_t i = 0; i < active.size(); i++) {
                     if (leaving[i]) deferred.push_back(std::move(active[i]));
                     else keep.push_back(std::move(active[i]));
                 }
                 active = std::move(keep);
                 n_active = active.size() + deferred.size();
-                packed = false;  // survivors move to a compact layout next turn
-                ph_finish += now_s() - t_c;
-                continue;
             }
-            for (size_t i : retiring) {
-                if (active[i]->ticks > 0 && next_window(*active[i])) {
-                    rewindowed = true;  // stays active with its next window
-                    continue;
-                }
-                out[i] = any = true;
-            }
-            if (rewindowed) packed = false;
-            ph_finish += now_s() - t_c;
-            if (!any) continue;
-            std::lock_guard<std::mutex> lk(mu);
-            std::vector<std::unique_ptr<Request>> keep;
-            for (size_t i = 0; i < active.size(); i++) {
-                if (out[i]) done[active[i]->id] = std::move(active[i]);
-                else keep.push_back(std::move(active[i]));
-            }
-            active = std::move(keep);
-            n_active = active.size() + deferred.size();
             packed = false;  // survivors move to a compact layout next turn
-            cv_done.notify_all();
+            ph_finish += now_s() - t_c;
         }
     }
 

@@ -232,7 +232,9 @@ typedef struct {
 /* A persistent request loop on the model's device. Requests are admitted
  * while their device state fits `hbm_budget_bytes` and at most `max_active`
  * are on the GPU; all active requests' tokens are packed into one ragged
- * launch per layer; a request retires on its own convergence. */
+ * launch per layer; a request retires on its own convergence, and its
+ * result lands on the host while the next iteration runs (or at once when
+ * there is none): poll reports it from then on. */
 int j2k_server_create(void* model, size_t hbm_budget_bytes, int max_active, void** out_server);
 /* Queues a request (copies its inputs). grid [3,T,ny,nx] with blank cells
  * filled or not (the server fills them), known [T,ny,nx], meta [meta_dim]. */

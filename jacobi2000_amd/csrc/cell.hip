@@ -74,6 +74,25 @@ static void embed_and_trunk(Model& m, hipStream_t s, const CellPack& p, const fl
         return;
     }
     if (n_zero > 0) fill_u32(s, zero, n_zero, 0u);
+    if ((m.bf16() || m.split()) && mlp_fused_ok(d, m.cfg.ff_inner) && L0.wqkv_f && L0.w1_f &&
+        m.tune.choice(TO_ROWS, p.M) == 2) {
+        // d = 256 with the whole block in one launch: the embedding, LN1 and
+        // the first qkv product in one launch too
+        RowsArgs a;
+        a.X = X;
+        a.n1g = L0.n1g;
+        a.n1b = L0.n1b;
+        a.Wqkv = L0.wqkv_f;
+        a.bqkv = L0.bqkv;
+        a.QKV = m.get<char>(S_QKV, (size_t)p.M * 3 * d * (m.split() ? 4 : 2));
+        {
+            TScope t(m, s, KC_GEMM, 2.0 * p.M * d * 3 * d);
+            rows256_embed_qkv(s, p.M, d, m.split(), x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, a);
+        }
+        if (!trunk(m, s, p.M, X, TRUNK_QKV, plan_of(p), &ro))
+            cell_readout(s, p.M, d, X, ro.g, ro.b, ro.w, ro.bias, ro.x0, ro.xn, ro.y);
+        return;
+    }
     if (m.bf16()) {
         uint16_t* Hb = m.get<uint16_t>(S_HB, (size_t)p.M * d);
         cell_embed_ln(s, p.M, d, x4, m.embed_w, m.embed_b, p.mvec, p.tok_req, L0.n1g, L0.n1b, X, nullptr, Hb);

@@ -101,6 +101,11 @@ void mlp_fused(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs
 // a.Wqkv the next block's LN1 and QKV; the same bits as the tile GEMMs, the
 // fused MLP and k_layernorm (weights in fragment order).
 void rows256(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a);
+// dim == 256: the embedding (+ meta) and block 0's LN1 and qkv in one launch
+// (a.X, a.n1g / n1b, a.Wqkv in fragment order, a.bqkv, a.QKV); the same bits
+// as the embedding launch and the qkv tile GEMM.
+void rows256_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, const float* we, const float* be,
+                       const float* mvec, const int* tok_req, const RowsArgs& a);
 // QKV = Hin @ Wqkv + bqkv alone (dim == 128), the row-panel kernel's
 // arithmetic: the first layer's product after the embedding.
 void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a);

@@ -1,6 +1,7 @@
 // LayerNorm, patch2d glue kernels (patchify, meta linear, transpose,
 // upsample) and the cell_st per-token kernels (pack, embed+LN, readout).
 #include "common.h"
+#include "embedln.h"
 #include "kernels.h"
 
 namespace j2k {
@@ -349,18 +350,6 @@ __global__ __launch_bounds__(256) void k_cell_embed_ln(int M, int dim, const flo
     }
 }
 
-// Sum over the L lanes of one token's group (L a power of two up to 32).
-template <int L>
-__device__ __forceinline__ float group_sum(float x) {
-    static_assert(L >= 4 && L <= 32 && (L & (L - 1)) == 0, "group of 4 to 32 lanes");
-    x += quad_xor1(x);
-    x += quad_xor2(x);
-    if (L > 4) x += swz_xor<4>(x);
-    if (L > 8) x += swz_xor<8>(x);
-    if (L > 16) x += swz_xor<16>(x);
-    return x;
-}
-
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 // k_cell_embed_ln for dim = D (128 or 256): L = D/8 lanes per token, each
@@ -381,48 +370,11 @@ __global__ __launch_bounds__(256) void k_cell_embed_ln_v(int M, const float4* __
     const float4 f = x4[m];
     const float* mv = mvec + (tok_req ? (long)tok_req[m] * D : 0);
     const int d0 = j * 8;
-    float a[8];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const f4v bv = *reinterpret_cast<const f4v*>(be + d0 + 4 * h);
-        const f4v w0 = *reinterpret_cast<const f4v*>(we + d0 + 4 * h);
-        const f4v w1 = *reinterpret_cast<const f4v*>(we + D + d0 + 4 * h);
-        const f4v w2 = *reinterpret_cast<const f4v*>(we + 2 * D + d0 + 4 * h);
-        const f4v w3 = *reinterpret_cast<const f4v*>(we + 3 * D + d0 + 4 * h);
-        const f4v mw = *reinterpret_cast<const f4v*>(mv + d0 + 4 * h);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            float v = bv[k];
-            if (f.x != 0.f) v += f.x * w0[k];
-            if (f.y != 0.f) v += f.y * w1[k];
-            if (f.z != 0.f) v += f.z * w2[k];
-            if (f.w != 0.f) v += f.w * w3[k];
-            v += mw[k];
-            a[4 * h + k] = v;
-        }
-    }
+    float a[8], o[8];
+    embed_ln_token<D>(f, we, be, mv, g, b, d0, a, o);
     float* xr = X + (long)m * D + d0;
     *reinterpret_cast<f4v*>(xr) = f4v{a[0], a[1], a[2], a[3]};
     *reinterpret_cast<f4v*>(xr + 4) = f4v{a[4], a[5], a[6], a[7]};
-    float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) s += a[k];
-    const float mean = group_sum<L>(s) / (float)D;
-    float q = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const float c = a[k] - mean;
-        q += c * c;
-    }
-    const float inv = 1.0f / sqrtf(group_sum<L>(q) / (float)D + 1e-5f);
-    float o[8];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const f4v gv = *reinterpret_cast<const f4v*>(g + d0 + 4 * h);
-        const f4v bb = *reinterpret_cast<const f4v*>(b + d0 + 4 * h);
-#pragma unroll
-        for (int k = 0; k < 4; k++) o[4 * h + k] = (a[4 * h + k] - mean) * inv * gv[k] + bb[k];
-    }
     if (H) {
         float* hr = H + (long)m * D + d0;
         *reinterpret_cast<f4v*>(hr) = f4v{o[0], o[1], o[2], o[3]};

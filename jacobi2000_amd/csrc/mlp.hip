@@ -33,6 +33,7 @@
 // of the LayerNorms in the order every served kernel uses (lnstats.h).
 #include "common.h"
 #include "kernels.h"
+#include "embedln.h"
 #include "lnstats.h"
 
 namespace j2k {
@@ -713,6 +714,39 @@ __device__ __forceinline__ void ln_panel_k(const f4v (&xm)[4][D / 128], float* X
     __syncthreads();  // every read of Xs is done; the statistics are in
 }
 
+// QKV = H @ Wqkv + bqkv for a 64-token panel of d = 256 (H in the panel
+// Ah / Al), eight waves of 32 columns, three 256-column chunks; w holds the
+// first chunk's fragments on entry. f32 (split) or bf16 out.
+template <bool SPLIT>
+__device__ __forceinline__ void qkv_panel256(const uint16_t* Ah, const uint16_t* Al, bf8v (&w)[8][2],
+                                             const RowsArgs& a, int m0, int rows, int wn, int r16, int c4, int kq,
+                                             int lane) {
+    constexpr int D = 256, MI = 4, J = 2, K4D = 8;
+    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * D * (SPLIT ? 4 : 2),
+                              rows * 3 * D * (SPLIT ? 4 : 2));
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        asm volatile("" ::: "memory");  // H's fragments read per chunk, not held across all three
+        f4v acc[MI][J];
+        zero(acc);
+        mma_panel<SPLIT, MI, J, K4D>(acc, Ah, Al, w, r16, kq);
+        if (c < 2) load_w<J, K4D>(w, a.Wqkv, D, (c + 1) * D, 0, wn, lane);
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * D + wn + j * 16 + c4);
+#pragma unroll
+            for (int i = 0; i < MI; i++) {
+                f4v v;
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
+                const int o = (((i * 16 + r16) * 3 + c) * D + wn + j * 16 + c4) * 4;
+                if (SPLIT) bstore4(rQ, o, v);
+                else bstore4(rQ, o / 2, bf4(v));
+            }
+        }
+    }
+}
+
 template <bool SPLIT, bool QKV>
 __global__ __launch_bounds__(512, 1) void k_rows256(int M, int ff, RowsArgs a) {
     constexpr int D = 256, WN = 8, NT = 64 * WN, MI = 4, J = D / (16 * WN), P = SPLIT ? 2 : 1;
@@ -837,29 +871,47 @@ __global__ __launch_bounds__(512, 1) void k_rows256(int M, int ff, RowsArgs a) {
     load_w<J, K4D>(w, a.Wqkv, D, 0, 0, wn, lane);
     put_ln(xm, a.n1g, a.n1b);
     __syncthreads();
-    const auto rQ = rows_rsrc(static_cast<const char*>(a.QKV) + (long)m0 * 3 * D * (SPLIT ? 4 : 2),
-                              rows * 3 * D * (SPLIT ? 4 : 2));
+    qkv_panel256<SPLIT>(Ah, Al, w, a, m0, rows, wn, r16, c4, kq, lane);
+}
+
+// The embedding, block 0's LN1 and its qkv product for d = 256, over
+// 64-token panels of eight waves: each wave embeds two tokens at a time
+// with 32 lanes a token (the arithmetic of k_cell_embed_ln_v), stores X and
+// puts LN1's output straight into the panel; then the qkv product as in
+// k_rows256. The same bits as the embedding launch and the qkv tile GEMM.
+template <bool SPLIT>
+__global__ __launch_bounds__(512, 1) void k_rows256_embed(int M, const float4* __restrict__ x4,
+                                                          const float* __restrict__ we, const float* __restrict__ be,
+                                                          const float* __restrict__ mvec,
+                                                          const int* __restrict__ tok_req, RowsArgs a) {
+    constexpr int D = 256, WN = 8, J = D / (16 * WN), K4D = D / 32;
+    __shared__ __align__(16) unsigned char As[RB * D * 4];
+    uint16_t* const Ah = reinterpret_cast<uint16_t*>(As);
+    uint16_t* const Al = Ah + (SPLIT ? RB * D : 0);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.x * RB, rows = min(RB, M - m0);
+    const int wn = wave * 16 * J;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8, c4 = (lane >> 4) * 4;
+    bf8v w[K4D][J];
+    load_w<J, K4D>(w, a.Wqkv, D, 0, 0, wn, lane);
+    const int j32 = lane & 31, d0 = j32 * 8;
 #pragma unroll 1
-    for (int c = 0; c < 3; c++) {
-        asm volatile("" ::: "memory");  // H's fragments read per chunk, not held across all three
-        f4v acc[MI][J];
-        zero(acc);
-        mma_panel<SPLIT, MI, J, K4D>(acc, Ah, Al, w, r16, kq);
-        if (c < 2) load_w<J, K4D>(w, a.Wqkv, D, (c + 1) * D, 0, wn, lane);
-#pragma unroll
-        for (int j = 0; j < J; j++) {
-            const f4v b = *reinterpret_cast<const f4v*>(a.bqkv + c * D + wn + j * 16 + c4);
-#pragma unroll
-            for (int i = 0; i < MI; i++) {
-                f4v v;
-#pragma unroll
-                for (int r = 0; r < 4; r++) v[r] = acc[i][j][r] + b[r];
-                const int o = (((i * 16 + r16) * 3 + c) * D + wn + j * 16 + c4) * 4;
-                if (SPLIT) bstore4(rQ, o, v);
-                else bstore4(rQ, o / 2, bf4(v));
-            }
+    for (int pass = 0; pass < RB / 16; pass++) {
+        const int row = pass * 16 + wave * 2 + (lane >> 5), m = m0 + row;
+        float o[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        if (m < M) {  // both 32-lane halves of a token take the same branch
+            float e[8];
+            const float* mv = mvec + (tok_req ? (long)tok_req[m] * D : 0);
+            embed_ln_token<D>(x4[m], we, be, mv, a.n1g, a.n1b, d0, e, o);
+            float* xr = a.X + (long)m * D + d0;
+            *reinterpret_cast<f4v*>(xr) = f4v{e[0], e[1], e[2], e[3]};
+            *reinterpret_cast<f4v*>(xr + 4) = f4v{e[4], e[5], e[6], e[7]};
         }
+        put_a4<SPLIT>(Ah, Al, row, d0, f4v{o[0], o[1], o[2], o[3]});
+        put_a4<SPLIT>(Ah, Al, row, d0 + 4, f4v{o[4], o[5], o[6], o[7]});
     }
+    __syncthreads();
+    qkv_panel256<SPLIT>(Ah, Al, w, a, m0, rows, wn, r16, c4, kq, lane);
 }
 
 }  // namespace
@@ -948,6 +1000,20 @@ void rows_qkv(hipStream_t s, int M, int dim, bool split, const RowsArgs& a) {
 }
 
 bool mlp_fused_ok(int dim, int ff) { return dim == 256 && ff % 128 == 0; }
+
+void rows256_embed_qkv(hipStream_t s, int M, int dim, bool split, const float* x4, const float* we, const float* be,
+                       const float* mvec, const int* tok_req, const RowsArgs& a) {
+    if (M <= 0) return;
+    if (dim != 256) fail(J2K_ERR_LIMIT, "d = 256 embedding kernel needs dim == 256");
+    if (!x4 || !we || !be || !mvec || !a.X || !a.n1g || !a.n1b || !a.Wqkv || !a.bqkv || !a.QKV)
+        fail(J2K_ERR_INVALID, "d = 256 embedding kernel: missing operand");
+    const dim3 grid(cdiv(M, RB)), block(512);
+    if (split)
+        hipLaunchKernelGGL(k_rows256_embed<true>, grid, block, 0, s, M, (const float4*)x4, we, be, mvec, tok_req, a);
+    else
+        hipLaunchKernelGGL(k_rows256_embed<false>, grid, block, 0, s, M, (const float4*)x4, we, be, mvec, tok_req, a);
+    LAUNCHCHK("rows256_embed_qkv");
+}
 
 void rows256(hipStream_t s, int M, int dim, int ff, bool split, const RowsArgs& a) {
     if (M <= 0) return;

@@ -347,8 +347,9 @@ def test_wide_six_layers(precision):
 def test_wide_fused_mlp_bit_identical(precision):
     """d = 256: the MLP in one launch (its ff-wide intermediate on chip), and
     everything between two attentions in one launch (proj + residual, LN2,
-    MLP, the next block's LN1 and qkv), give the same raw output and
-    iterates, bit for bit, as the tile GEMMs and LayerNorm launches, on a
+    MLP, the next block's LN1 and qkv; with it the embedding, LN1 and the
+    first qkv in one launch), give the same raw output and iterates, bit
+    for bit, as the tile GEMMs and LayerNorm launches, on a
     grid whose token count is no multiple of the 64-token panel (3 x 24 x
     20); the fused MLP runs once per block, the whole-block launch once per
     block (with the next qkv but for the last block); bf16x2's raw output

@@ -360,6 +360,8 @@ struct Server {
             for (auto& r : active)
                 if (r->base < 0 && r->stage_buf != up) end += aligned(13 * (size_t)r->dims.tokens());
             if (end > cap_stage[up]) {  // grows (contents lost): stage every new request again
+                // an upload from the old buffer on s_copy ends before it is freed
+                HIPCHK(hipEventSynchronize(ev_uploaded));
                 for (Request* r : staged) r->stage_buf = -1;
                 staged.clear();
                 staged_end = 0;

@@ -156,13 +156,16 @@ __device__ __forceinline__ void row_store_quarter(T* row, int g, int sub, const 
 }
 
 // Waves per SIMD for a block of CNT threads: two blocks per CU when the LDS
-// footprint allows it (up to 80 KB) and the scores are those of a 27-key
-// stencil, else one, whose register file then holds twice as much.
+// footprint allows it (up to 80 KB) and a lane's scores and q / k /
+// accumulator quarters fit half the register file: a 27-key stencil, or a
+// wider one whose scores are kept a quarter per lane (up to 48 registers of
+// both); else one block, whose register file then holds twice as much.
 template <int HD, int RS, int RT, class T, int MODE>
 constexpr int cell_occupancy() {
+    constexpr int KMAX = (2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1);
     return (MODE == KV_JOINT ? 2 : 1) * (2 * RT + CT_T) * (CT_Y + 2 * RS) * (CT_X + 2 * RS) * HD * (int)sizeof(T) >
                            80 * 1024 ||
-                   (2 * RS + 1) * (2 * RS + 1) * (2 * RT + 1) > 32
+                   (KMAX > 32 && (KMAX + 3) / 4 + 3 * HD / 4 > 48)
                ? CNT / 256
                : 2 * CNT / 256;
 }
@@ -176,6 +179,12 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
     constexpr int VPR = HD / VEC;            // vectors per K (or V) row
     constexpr int HW = CT_X + 2 * RS, HH = CT_Y + 2 * RS, PLANE = HH * HW;
     constexpr int KS = (2 * RS + 1) * (2 * RS + 1), KMAX = KS * (2 * RT + 1);
+    // Above 32 keys, where that lets two blocks share a CU, the scores are
+    // kept per group of four keys, one a lane (quad_scatter4): a quarter of
+    // the registers, the same values.
+    constexpr int OCC = cell_occupancy<HD, RS, RT, T, MODE>();
+    constexpr bool GROUPED = KMAX > 32 && OCC > CNT / 256;
+    constexpr int NG = (KMAX + 3) / 4, NS = GROUPED ? NG : KMAX;  // score registers a lane
     constexpr int ROWS = MODE == KV_JOINT ? 2 : 1;  // K/V rows per cell in one staging phase
     constexpr int NV = ((2 * RT + CT_T) * PLANE * ROWS * VPR + CNT - 1) / CNT;  // staged vectors per thread
     extern __shared__ __align__(16) unsigned char smem[];
@@ -202,12 +211,19 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
         return sg.base + (t_lo + ct) * n + y * sg.nx + x;
     };
     // The first PF vectors of each thread are loaded into registers one phase
-    // ahead; the rest (wide stencils) are loaded when the phase is staged. PF
+    // ahead; the rest are loaded when the phase is staged, SB at a time. PF
     // is sized so that the prefetch (5 registers a vector), the scores and
     // the q / k / accumulator rows fit the register file without spilling.
-    constexpr int PF_FIT = (512 / cell_occupancy<HD, RS, RT, T, MODE>() * 7 / 10 - KMAX - 3 * D4) / 5;
-    constexpr int PF = NV < PF_FIT ? NV : (PF_FIT < 4 ? 4 : PF_FIT);
-    int grow[PF];
+    // The grouped (wider) stencils prefetch nothing and load a phase's rows
+    // in one batch: their second block per CU hides that latency, and the
+    // registers a prefetch would hold through the arithmetic do not fit
+    // beside it (a 75-key, 32-dimension head: 551 us per 262,144-token
+    // launch, against 597-714 with two to four vectors ahead and 764 with
+    // the scores of every key in registers at one block per CU).
+    constexpr int PF_FIT = (512 / OCC * 7 / 10 - NS - 3 * D4) / 5;
+    constexpr int PF = GROUPED ? 0 : NV < PF_FIT ? NV : (PF_FIT < 4 ? 4 : PF_FIT);
+    constexpr int SB = GROUPED ? NV : 8;
+    int grow[PF > 0 ? PF : 1];
 #pragma unroll
     for (int i = 0; i < PF; i++) grow[i] = row_of(tid + CNT * i);
     // Loads of phase (head h, rows `w`: 0 = K, 1 = V, -1 = both) and their
@@ -226,22 +242,22 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
             const int v = tid + CNT * i, c = v / (ROWS * VPR), e = v % VPR;                                        \
             if (v < nvec) *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = pre[i]; \
         }                                                                                                          \
-        _Pragma("unroll") for (int i0 = PF; i0 < NV; i0 += 8) {                                                    \
-            u32x4 tmp[8];                                                                                          \
-            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
+        _Pragma("unroll") for (int i0 = PF; i0 < NV; i0 += SB) {                                                   \
+            u32x4 tmp[SB];                                                                                         \
+            _Pragma("unroll") for (int j = 0; j < SB; j++) {                                                       \
                 const int i = i0 + j, v = tid + CNT * i, e = v % VPR;                                              \
                 if (i < NV)                                                                                        \
                     tmp[j] = *reinterpret_cast<const u32x4*>(qkv + (long)row_of(v) * ld + (1 + J2K_WHICH(v, w)) * dim + \
                                                              (h) * HD + e * VEC);                                  \
             }                                                                                                      \
-            _Pragma("unroll") for (int j = 0; j < 8; j++) {                                                        \
+            _Pragma("unroll") for (int j = 0; j < SB; j++) {                                                       \
                 const int i = i0 + j, v = tid + CNT * i, c = v / (ROWS * VPR), e = v % VPR;                        \
                 if (i < NV && v < nvec)                                                                            \
                     *reinterpret_cast<u32x4*>((J2K_WHICH(v, w) ? Vs : Ks) + (size_t)c * HD + e * VEC) = tmp[j];    \
             }                                                                                                      \
         }                                                                                                          \
     }
-    u32x4 pre[PF];
+    u32x4 pre[PF > 0 ? PF : 1];
     // the lane's query: cell q of the tile in slice tq (a slice past the
     // segment's end repeats the tile's first and stores nothing); its keys
     // are planes poff .. poff + ntq - 1 of the staged ones
@@ -252,6 +268,9 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
     const int tq_lo = max(tq - RT, 0), ntq = min(tq + RT, sg.T - 1) - tq_lo + 1, poff = tq_lo - t_lo;
     const long qrow = (long)sg.base + (long)tq * n + (long)(live ? qy : 0) * sg.nx + (live ? qx : 0);
     const float scale = 1.0f / sqrtf((float)HD);
+    // a wave's 64 lanes are 16 queries of one slice, so their time window
+    // is one: the grouped key loops branch on it as a scalar
+    const int ntq_w = __builtin_amdgcn_readfirstlane(ntq);
     // the query's first key row (time offset 0 of its window, y and x
     // offsets -RS): key (ct, dy, dx) lies a compile-time distance past it
     const int k0 = (poff * HH + (q >> 3)) * HW + (q & 7);
@@ -278,9 +297,35 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
         } else {
             J2K_FETCH(h, 1)  // V of this head, in flight during the scores
         }
-        float qv[D4], kv[D4], acc[D4], s[KMAX];
+        float qv[D4], kv[D4], acc[D4], s[NS];
         row_quarter<HD, T>(qkv + qrow * ld + h * HD, g, sub, qv);
         float mx = -INFINITY;
+        if constexpr (GROUPED) {
+#pragma unroll
+            for (int gi = 0; gi < NG; gi++) {
+                float pd[4];  // the lane's partial dot products of the group's four keys
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    constexpr int W = 2 * RS + 1;
+                    const int k = 4 * gi + j, ct = k / KS, dy = (k % KS) / W, dx = k % W;
+                    pd[j] = 0.f;
+                    if (k < KMAX && ct < ntq_w) {
+                        row_quarter<HD, T>(kb + ((ct * HH + dy) * HW + dx) * HD, g, sub, kv);
+                        float dot = 0.f;
+#pragma unroll
+                        for (int d = 0; d < D4; d++) dot += qv[d] * kv[d];
+                        pd[j] = dot;
+                    }
+                }
+                const float dot = quad_scatter4(pd, sub);  // key 4 gi + sub
+                const int k = 4 * gi + sub;
+                const float sc = k < KMAX && k / KS < ntq_w ? dot * scale : -INFINITY;
+                s[gi] = sc;
+                mx = fmaxf(mx, sc);
+            }
+            mx = fmaxf(mx, quad_xor1(mx));
+            mx = fmaxf(mx, quad_xor2(mx));
+        } else
 #pragma unroll
         for (int ct = 0; ct < 2 * RT + 1; ct++) {
             if (ct >= ntq) break;
@@ -310,6 +355,24 @@ __global__ __launch_bounds__(CNT, (cell_occupancy<HD, RS, RT, T, MODE>())) void 
 #pragma unroll
         for (int d = 0; d < D4; d++) acc[d] = 0.f;
         float l = 0.f;
+        if constexpr (GROUPED) {
+#pragma unroll
+            for (int gi = 0; gi < NG; gi++) {
+                const float po = __expf(s[gi] - mx);  // the weight of the lane's own key of the group
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    constexpr int W = 2 * RS + 1;
+                    const int k = 4 * gi + j, ct = k / KS, dy = (k % KS) / W, dx = k % W;
+                    if (k < KMAX && ct < ntq_w) {
+                        const float p = quad_bcast(po, j);
+                        l += p;
+                        row_quarter<HD, T>(vb + ((ct * HH + dy) * HW + dx) * HD, g, sub, kv);
+#pragma unroll
+                        for (int d = 0; d < D4; d++) acc[d] += p * kv[d];
+                    }
+                }
+            }
+        } else
 #pragma unroll
         for (int ct = 0; ct < 2 * RT + 1; ct++) {
             if (ct >= ntq) break;

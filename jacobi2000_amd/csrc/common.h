@@ -31,6 +31,26 @@ __device__ __forceinline__ float quad_xor1(float x) {
 __device__ __forceinline__ float quad_xor2(float x) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x4E, 0xF, 0xF, true));
 }
+// x of lane j of the quad, to all four (j a constant once unrolled).
+__device__ __forceinline__ float quad_bcast(float x, int j) {
+    const int v = __builtin_bit_cast(int, x);
+    switch (j & 3) {
+        case 0: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(v, 0x00, 0xF, 0xF, true));
+        case 1: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(v, 0x55, 0xF, 0xF, true));
+        case 2: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(v, 0xAA, 0xF, 0xF, true));
+        default: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(v, 0xFF, 0xF, 0xF, true));
+    }
+}
+// Reduce-scatter over a quad: lane j (= sub) gets p[j] summed over the four
+// lanes as (lane 0 + lane 1) + (lane 2 + lane 3), the value and rounding of
+// the butterfly x += quad_xor1(x); x += quad_xor2(x) on p[j], with three
+// exchanges for the four sums instead of eight.
+__device__ __forceinline__ float quad_scatter4(const float (&p)[4], int sub) {
+    const bool odd = sub & 1, hi = sub & 2;
+    const float a = (odd ? p[1] : p[0]) + quad_xor1(odd ? p[0] : p[1]);  // pair sums of keys 0 / 1
+    const float b = (odd ? p[3] : p[2]) + quad_xor1(odd ? p[2] : p[3]);  // and of keys 2 / 3
+    return (hi ? b : a) + quad_xor2(hi ? a : b);
+}
 
 // x of lane ^ M within each 32 lanes, M = 4, 8 or 16 (ds_swizzle in its
 // bit-mask mode: the LDS unit's crossbar without an address or a bank access).

@@ -240,9 +240,13 @@ def test_launch_choices_do_not_change_results(model20_cell, precision, spec):
         assert np.array_equal(xa, xb), (T, ny, nx)
 
 
-@pytest.mark.parametrize("precision", ["f32", "bf16x2"])
+@pytest.mark.parametrize("precision", ["f32", "bf16x2", "bf16"])
 def test_wide_stencil_staging_modes_agree(precision):
-    """The 75-key stencil (prefetch plus batched remainder) in every staging mode."""
+    """The 75-key stencil in every staging mode gives the same bits: K and V
+    together (every score in registers, a prefetch plus a batched
+    remainder), K then V (scores grouped by four keys over the quad, two
+    blocks per CU), one plane at a time, and two queries a lane; the time
+    windows clipped at both ends (3 slices) and partial tiles (24 x 20)."""
     st, cj = ck.random_cell_st(dim=256, layers=1, heads=8, ff=1024, seed=1, radius=2)
     models = [_model_env(st, cj, precision, J2K_TUNE_FORCE=f"attn_kv_mode={m}") for m in range(5)]
     T, ny, nx = 3, 24, 20

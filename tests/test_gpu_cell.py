@@ -257,6 +257,30 @@ def test_wide_stencil_staging_modes_agree(precision):
     assert all(np.array_equal(xs[0], x) for x in xs[1:])
 
 
+@pytest.mark.parametrize("heads", [2, 4])
+@pytest.mark.parametrize("precision", ["f32", "bf16x2"])
+def test_narrow_heads_wide_stencil(precision, heads):
+    """The 75-key stencil with 16- and 8-dimension heads (d = 32, 2 or 4
+    heads; scores grouped by four keys wherever two blocks share a CU):
+    every staging mode gives the same bits, and f32 stays within 1e-4 of
+    the oracle per field over two iterations (moving drops and r_max 1.05,
+    so that h, u and v all depend on the model)."""
+    st, cj = ck.random_cell_st(dim=32, layers=2, heads=heads, ff=64, seed=7, std=0.1, radius=2, r_max=1.05)
+    models = [_model_env(st, cj, precision, J2K_TUNE_FORCE=f"attn_kv_mode={m}") for m in range(5)]
+    T, ny, nx = 3, 24, 20
+    g = ck.moving_drops_grid(T, ny, nx, 2, 29)
+    k = ck.mask_of("forward", T, ny, nx)
+    x0 = O.fill_blank(g, k)
+    print(f"case narrow heads {precision} heads={heads}", flush=True)
+    xs = [m.iterate(x0, k, META, max_iter=2, tol=0.0)[0] for m in models]
+    assert all(np.array_equal(xs[0], x) for x in xs[1:])
+    if precision == "f32":
+        xc, _, _ = O.FormC.load(st, cj).iterate(x0, k, META, 2, 0.0)
+        d = np.abs(xs[0] - xc).reshape(3, -1).max(1)
+        print(f"narrow heads f32 max_abs per field {d}")
+        assert d.max() < 1e-4
+
+
 def test_autotune_table_and_results(model20_cell):
     """The tuner fills one entry per bucket with measured candidates, and a
     tuned model iterates exactly like an untuned one."""
